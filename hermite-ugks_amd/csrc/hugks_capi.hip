@@ -1,0 +1,501 @@
+// C-ABI of libhugks.so (declared in include/hugks.h): handle lifetime,
+// device allocation, step sequencing and data exchange.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hugks.h"
+#include "hugks_device.h"
+#include "hugks_kernels.h"
+
+using namespace hugks;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct KernelTimer {
+    const char* name;
+    double ms;
+    int calls;
+};
+
+}  // namespace
+
+struct hugks_handle {
+    hugks_desc d{};
+    int dim = 2;
+    bool fp64 = true;
+    Launch2D L{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::vector<void*> allocs;
+    std::string err;
+    bool timing = false;
+    std::vector<KernelTimer> timers;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t ncell = 0;
+};
+
+static int fail(hugks_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+static int hip_fail(hugks_handle* h, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(h, e == hipErrorOutOfMemory ? HUGKS_E_NOMEM : HUGKS_E_DEVICE, m);
+}
+
+template <typename P>
+static hipError_t dalloc(hugks_handle* h, P** p, size_t bytes) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes < 16 ? 16 : bytes);
+    if (e != hipSuccess) return e;
+    h->allocs.push_back(q);
+    *p = (P*)q;
+    return hipSuccess;
+}
+
+static void release(hugks_handle* h) {
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) hipFree(p);
+    h->allocs.clear();
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    h->stream = nullptr;
+}
+
+// Launch helper: optional per-kernel timing with HIP events on the
+// handle's stream.
+template <typename F>
+static hipError_t timed(hugks_handle* h, const char* name, F&& launch) {
+    if (!h->timing) return launch();
+    hipEventRecord(h->ev0, h->stream);
+    hipError_t e = launch();
+    hipEventRecord(h->ev1, h->stream);
+    hipEventSynchronize(h->ev1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    for (auto& t : h->timers)
+        if (t.name == name) {
+            t.ms += ms;
+            t.calls += 1;
+            return e;
+        }
+    h->timers.push_back({name, (double)ms, 1});
+    return e;
+}
+
+static hipError_t enqueue_step(hugks_handle* h) {
+    hipError_t e;
+    const Launch2D& L = h->L;
+    e = timed(h, "timestep", [&] { return launch_timestep(L, h->dim, h->stream); });
+    if (e != hipSuccess) return e;
+    if (h->dim == 2) {
+        e = timed(h, "flux2d", [&] { return launch_flux2d(L, h->stream); });
+        if (e != hipSuccess) return e;
+        e = timed(h, "update2d", [&] { return launch_update2d(L, h->stream); });
+        if (e != hipSuccess) return e;
+    } else {
+        e = timed(h, "step1d", [&] { return launch_step1d(L, h->stream); });
+        if (e != hipSuccess) return e;
+    }
+    return timed(h, "finalize", [&] { return launch_finalize(L, h->dim, h->stream); });
+}
+
+static int read_state(hugks_handle* h, hugks_step_info* info) {
+    DevState s;
+    hipError_t e = hipMemcpyAsync(&s, h->L.st, sizeof(s), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "reading device state");
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->steps = s.steps;
+        info->iter = s.iter;
+        info->stop_reason = s.stop_reason;
+        info->device_flags = s.flags;
+        info->simtime = s.simtime;
+        info->dt = s.dt;
+        for (int q = 0; q < 4; ++q) info->res[q] = s.res[q];
+        for (int q = 0; q < 3; ++q) info->steady_res[q] = s.steady[q];
+    }
+    if (s.flags & 1) return fail(h, HUGKS_E_NUMERIC, "non-finite state or T <= 0 on the device");
+    if (s.flags & 2) return fail(h, HUGKS_E_WALL, "zero outgoing wall mass coefficient");
+    return HUGKS_OK;
+}
+
+extern "C" {
+
+const char* hugks_last_error(const hugks_handle* h) {
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+const char* hugks_device_info(void) {
+    static std::string info;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        info = "no HIP device";
+        return info.c_str();
+    }
+    hipDeviceProp_t p;
+    hipGetDeviceProperties(&p, 0);
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s %s CUs=%d LDS/CU=%zu KiB HBM=%.1f GiB", p.name, p.gcnArchName,
+             p.multiProcessorCount, (size_t)p.maxSharedMemoryPerMultiProcessor / 1024,
+             p.totalGlobalMem / 1073741824.0);
+    info = buf;
+    return info.c_str();
+}
+
+int hugks_create(const hugks_desc* d, hugks_handle** out) {
+    if (!d || !out) return fail(nullptr, HUGKS_E_ARG, "null descriptor or output pointer");
+    *out = nullptr;
+    if (d->abi_version != HUGKS_ABI_VERSION)
+        return fail(nullptr, HUGKS_E_ARG, "descriptor abi_version mismatch");
+    if (d->dim != 1 && d->dim != 2) return fail(nullptr, HUGKS_E_ARG, "dim must be 1 or 2");
+    const int dim = d->dim;
+    if (d->nx < 2 || (dim == 2 && d->ny < 2) || (dim == 1 && d->ny != 1))
+        return fail(nullptr, HUGKS_E_ARG, "need nx >= 2 (and ny >= 2 in 2D, ny == 1 in 1D)");
+    if (d->nv < 2) return fail(nullptr, HUGKS_E_ARG, "nv must be >= 2");
+    if (d->nm < 0) return fail(nullptr, HUGKS_E_ARG, "Hermite order Nm must be non-negative");
+    if (d->precision < HUGKS_FP64 || d->precision > HUGKS_MIXED_BF16X3)
+        return fail(nullptr, HUGKS_E_ARG, "unknown precision");
+    if (!d->u || !d->weight || !d->prim0 || (dim == 2 && (!d->v || !d->accel_cell ||
+                                                          !d->accel_vface || !d->accel_hface)))
+        return fail(nullptr, HUGKS_E_ARG, "missing node, weight, prim0 or acceleration array");
+    if (!(d->dx > 0) || (dim == 2 && !(d->dy > 0)))
+        return fail(nullptr, HUGKS_E_ARG, "cell sizes must be positive");
+    for (int s = 0; s < (dim == 2 ? 4 : 2); ++s) {
+        if (d->bc[s] < HUGKS_BC_PERIODIC || d->bc[s] > HUGKS_BC_DIFFUSE)
+            return fail(nullptr, HUGKS_E_ARG, "unknown boundary kind");
+        if (d->bc[s] == HUGKS_BC_DIFFUSE && !(d->tw[s] > 0))
+            return fail(nullptr, HUGKS_E_ARG, "diffuse wall temperature must be positive");
+    }
+    if (dim == 2 && ((d->bc[0] == HUGKS_BC_PERIODIC) != (d->bc[1] == HUGKS_BC_PERIODIC) ||
+                     (d->bc[2] == HUGKS_BC_PERIODIC) != (d->bc[3] == HUGKS_BC_PERIODIC)))
+        return fail(nullptr, HUGKS_E_ARG, "periodic sides must come in pairs");
+    if (dim == 1 && (d->bc[0] == HUGKS_BC_PERIODIC || d->bc[1] == HUGKS_BC_PERIODIC))
+        return fail(nullptr, HUGKS_E_UNSUPPORTED, "1D supports specular and diffuse walls");
+    const int nn = dim == 2 ? d->nv * d->nv : d->nv;
+    const int ldv = (nn + 63) / 64 * 64;
+    if (dim == 2 && ldv > 2048)
+        return fail(nullptr, HUGKS_E_UNSUPPORTED, "nv*nv above 2048 has no 2D kernel");
+    if (dim == 1 && ldv > 256) return fail(nullptr, HUGKS_E_UNSUPPORTED, "1D supports nv <= 256");
+    if (dim == 2 && d->nm + 2 > 2 * d->nv)
+        return fail(nullptr, HUGKS_E_UNSUPPORTED, "Hermite order above 2*nv-2 is not supported");
+    for (int k = 0; k < nn; ++k)
+        if (!(d->weight[k] >= 0) || !std::isfinite(d->weight[k]))
+            return fail(nullptr, HUGKS_E_ARG, "weights must be finite and non-negative");
+    const size_t ncell = (size_t)d->nx * d->ny;
+    for (size_t c = 0; c < ncell; ++c) {
+        const double* p = d->prim0 + (size_t)(dim + 2) * c;
+        if (!(p[0] > 0) || !(p[dim + 1] > 0))
+            return fail(nullptr, HUGKS_E_ARG, "initial density and lambda must be positive");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, HUGKS_E_NOHIP, "no HIP device visible");
+
+    hugks_handle* h = new hugks_handle();
+    h->d = *d;
+    h->d.u = h->d.v = h->d.weight = h->d.prim0 = nullptr;
+    h->d.accel_cell = h->d.accel_vface = h->d.accel_hface = nullptr;
+    h->dim = dim;
+    h->fp64 = d->precision == HUGKS_FP64;
+    h->ncell = ncell;
+    hipGetDevice(&h->device);
+    Consts& c = h->L.c;
+    c.nx = d->nx; c.ny = d->ny; c.nv = d->nv; c.nn = nn; c.ldv = ldv; c.nm = d->nm;
+    for (int s = 0; s < 4; ++s) { c.bc[s] = d->bc[s]; c.tw[s] = d->tw[s]; }
+    c.tau_model = d->tau_model; c.dt_rule = d->dt_rule; c.clamp_dt = d->clamp_dt;
+    c.stop_on_residual = d->stop_on_residual;
+    c.dx = d->dx; c.dy = dim == 2 ? d->dy : 1.0; c.pi = d->pi; c.smv = d->smv; c.K = d->innerk;
+    c.gamma = d->gamma; c.prandtl = d->prandtl; c.shk = d->shakhov_coef; c.muref = d->muref;
+    c.omega = d->omega; c.cfl = d->cfl; c.maxtime = d->maxtime; c.eps_res = d->eps_res;
+    c.umax_dt = d->umax_dt; c.vmax_dt = d->vmax_dt; c.force_rate = d->force_rate;
+    c.tref = d->tref; c.phi = d->phi;
+    Launch2D& L = h->L;
+    L.fp64 = h->fp64;
+    const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { release(h); delete h; return hip_fail(nullptr, e, "stream"); }
+    hipEventCreate(&h->ev0);
+    hipEventCreate(&h->ev1);
+    const size_t fsz = ncell * ldv * es;
+    const size_t nvf = dim == 2 ? (size_t)(d->nx + 1) * d->ny : (size_t)d->nx + 1;
+    const size_t nhf = dim == 2 ? (size_t)d->nx * (d->ny + 1) : 1;
+    const int np = dim == 2 ? 5 : 9;
+    double* prim0_d = nullptr;
+#define ALLOC(ptr, bytes)                                                          \
+    do {                                                                           \
+        e = dalloc(h, &(ptr), (bytes));                                            \
+        if (e != hipSuccess) {                                                     \
+            int rc = hip_fail(nullptr, e, "device allocation");                    \
+            release(h);                                                            \
+            delete h;                                                              \
+            return rc;                                                             \
+        }                                                                          \
+    } while (0)
+    ALLOC(L.st, sizeof(DevState));
+    ALLOC(L.W, ncell * (dim + 2) * sizeof(double));
+    ALLOC(L.part, ncell * np * sizeof(double));
+    ALLOC(L.h, fsz);
+    ALLOC(L.b, fsz);
+    ALLOC(L.shx, fsz);
+    ALLOC(L.sbx, fsz);
+    if (dim == 2) {
+        ALLOC(L.shy, fsz);
+        ALLOC(L.sby, fsz);
+        ALLOC(L.hfh, nhf * ldv * es);
+        ALLOC(L.hfb, nhf * ldv * es);
+        ALLOC(L.hflux, nhf * 4 * sizeof(double));
+    }
+    ALLOC(L.vfh, nvf * ldv * es);
+    ALLOC(L.vfb, nvf * ldv * es);
+    ALLOC(L.vflux, nvf * 4 * sizeof(double));
+    double *un_d, *vn_d, *wn_d, *u1, *v1;
+    ALLOC(un_d, ldv * sizeof(double));
+    ALLOC(vn_d, ldv * sizeof(double));
+    ALLOC(wn_d, ldv * sizeof(double));
+    ALLOC(u1, d->nv * sizeof(double));
+    ALLOC(v1, d->nv * sizeof(double));
+    ALLOC(prim0_d, ncell * (dim + 2) * sizeof(double));
+    // per-node tables: u/v of node k and its weight, zero in the pad
+    std::vector<double> un(ldv, 0.0), vn(ldv, 0.0), wn(ldv, 0.0);
+    for (int k = 0; k < nn; ++k) {
+        if (dim == 2) {
+            un[k] = d->u[k % d->nv];
+            vn[k] = d->v[k / d->nv];
+        } else {
+            un[k] = d->u[k];
+        }
+        wn[k] = d->weight[k];
+    }
+    std::vector<double> vtab(d->nv, 0.0);
+    if (dim == 2) for (int k = 0; k < d->nv; ++k) vtab[k] = d->v[k];
+    hipMemcpy(un_d, un.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(vn_d, vn.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(wn_d, wn.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(u1, d->u, d->nv * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(v1, vtab.data(), d->nv * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(prim0_d, d->prim0, ncell * (dim + 2) * sizeof(double), hipMemcpyHostToDevice);
+    L.unode_d = un_d; L.vnode_d = vn_d; L.wnode_d = wn_d; L.u1 = u1; L.v1 = v1;
+    if (h->fp64) {
+        L.unode = un_d; L.vnode = vn_d; L.wnode = wn_d;
+    } else {
+        float *uf, *vf, *wf;
+        ALLOC(uf, ldv * sizeof(float));
+        ALLOC(vf, ldv * sizeof(float));
+        ALLOC(wf, ldv * sizeof(float));
+        std::vector<float> a(ldv), bb(ldv), cc(ldv);
+        for (int k = 0; k < ldv; ++k) { a[k] = (float)un[k]; bb[k] = (float)vn[k]; cc[k] = (float)wn[k]; }
+        hipMemcpy(uf, a.data(), ldv * sizeof(float), hipMemcpyHostToDevice);
+        hipMemcpy(vf, bb.data(), ldv * sizeof(float), hipMemcpyHostToDevice);
+        hipMemcpy(wf, cc.data(), ldv * sizeof(float), hipMemcpyHostToDevice);
+        L.unode = uf; L.vnode = vf; L.wnode = wf;
+    }
+    if (dim == 2) {
+        double *ac, *av, *ah;
+        ALLOC(ac, ncell * 2 * sizeof(double));
+        ALLOC(av, nvf * 2 * sizeof(double));
+        ALLOC(ah, nhf * 2 * sizeof(double));
+        hipMemcpy(ac, d->accel_cell, ncell * 2 * sizeof(double), hipMemcpyHostToDevice);
+        hipMemcpy(av, d->accel_vface, nvf * 2 * sizeof(double), hipMemcpyHostToDevice);
+        hipMemcpy(ah, d->accel_hface, nhf * 2 * sizeof(double), hipMemcpyHostToDevice);
+        L.acc_cell = ac; L.acc_v = av; L.acc_h = ah;
+        unsigned nwall = 0;
+        for (int s = 0; s < 4; ++s)
+            if (d->bc[s] != HUGKS_BC_PERIODIC) nwall += s < 2 ? d->ny : d->nx;
+        L.nwall = nwall;
+        L.update_lds = update2d_lds_bytes(c, h->fp64);
+        hipDeviceProp_t prop;
+        hipGetDeviceProperties(&prop, h->device);
+        if (L.update_lds > (size_t)prop.sharedMemPerBlockOptin && prop.sharedMemPerBlockOptin > 0) {
+            release(h);
+            delete h;
+            return fail(nullptr, HUGKS_E_UNSUPPORTED, "Hermite working set exceeds LDS");
+        }
+    }
+#undef ALLOC
+    // zero everything that pads or is read before written, then the
+    // initial state
+    hipMemsetAsync(L.shx, 0, fsz, h->stream);
+    hipMemsetAsync(L.sbx, 0, fsz, h->stream);
+    if (dim == 2) {
+        hipMemsetAsync(L.shy, 0, fsz, h->stream);
+        hipMemsetAsync(L.sby, 0, fsz, h->stream);
+        hipMemsetAsync(L.hfh, 0, nhf * ldv * es, h->stream);
+        hipMemsetAsync(L.hfb, 0, nhf * ldv * es, h->stream);
+        hipMemsetAsync(L.hflux, 0, nhf * 4 * sizeof(double), h->stream);
+    }
+    hipMemsetAsync(L.vfh, 0, nvf * ldv * es, h->stream);
+    hipMemsetAsync(L.vfb, 0, nvf * ldv * es, h->stream);
+    hipMemsetAsync(L.vflux, 0, nvf * 4 * sizeof(double), h->stream);
+    DevState s0{};
+    s0.iter = 1;
+    hipMemcpyAsync(L.st, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream);
+    e = launch_init_f(L, dim, prim0_d, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        int rc = hip_fail(nullptr, e, "initial state");
+        release(h);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return HUGKS_OK;
+}
+
+int hugks_destroy(hugks_handle* h) {
+    if (!h) return HUGKS_OK;
+    release(h);
+    delete h;
+    return HUGKS_OK;
+}
+
+int hugks_step(hugks_handle* h, hugks_step_info* info) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    hipError_t e = launch_begin(h->L.st, h->stream);
+    if (e == hipSuccess) e = enqueue_step(h);
+    if (e != hipSuccess) return hip_fail(h, e, "step launch");
+    return read_state(h, info);
+}
+
+int hugks_advance(hugks_handle* h, int32_t max_steps, hugks_step_info* info) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    hipError_t e = launch_begin(h->L.st, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "advance launch");
+    if (max_steps > 0) {
+        for (int32_t s = 0; s < max_steps; ++s) {
+            e = enqueue_step(h);
+            if (e != hipSuccess) return hip_fail(h, e, "advance launch");
+        }
+        return read_state(h, info);
+    }
+    // no cap: run in chunks until an exit condition holds
+    const int chunk = 64;
+    for (;;) {
+        for (int s = 0; s < chunk; ++s) {
+            e = enqueue_step(h);
+            if (e != hipSuccess) return hip_fail(h, e, "advance launch");
+        }
+        DevState s;
+        e = hipMemcpyAsync(&s, h->L.st, sizeof(s), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "advance state");
+        if (s.stopped || s.flags) break;
+    }
+    return read_state(h, info);
+}
+
+int hugks_get_macro(hugks_handle* h, double* w, double* q) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    const size_t ncell = h->ncell;
+    hipError_t e = hipSuccess;
+    if (w) e = hipMemcpyAsync(w, h->L.W, ncell * (h->dim + 2) * sizeof(double),
+                              hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && q) {
+        double* qd = nullptr;
+        e = hipMallocAsync((void**)&qd, ncell * h->dim * sizeof(double), h->stream);
+        if (e == hipSuccess) e = launch_macro(h->L, h->dim, qd, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(q, qd, ncell * h->dim * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream);
+        if (qd) hipFreeAsync(qd, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "get_macro");
+    return HUGKS_OK;
+}
+
+int hugks_get_f(hugks_handle* h, double* hf, double* bf) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    const Consts& c = h->L.c;
+    const size_t n = h->ncell * c.nn;
+    double* tmp = nullptr;
+    hipError_t e = hipMallocAsync((void**)&tmp, n * sizeof(double), h->stream);
+    for (int q = 0; q < 2 && e == hipSuccess; ++q) {
+        double* dst = q ? bf : hf;
+        if (!dst) continue;
+        e = launch_convert(q ? h->L.b : h->L.h, h->fp64, tmp, h->ncell, c.nn, c.ldv, true, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dst, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    if (tmp) hipFreeAsync(tmp, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "get_f");
+    return HUGKS_OK;
+}
+
+int hugks_set_f(hugks_handle* h, const double* hf, const double* bf, const double* w) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    const Consts& c = h->L.c;
+    const size_t n = h->ncell * c.nn;
+    double* tmp = nullptr;
+    hipError_t e = hipMallocAsync((void**)&tmp, n * sizeof(double), h->stream);
+    for (int q = 0; q < 2 && e == hipSuccess; ++q) {
+        const double* src = q ? bf : hf;
+        if (!src) continue;
+        e = hipMemcpyAsync(tmp, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess)
+            e = launch_convert_in(tmp, q ? h->L.b : h->L.h, h->fp64, h->ncell, c.nn, c.ldv, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    if (e == hipSuccess && w)
+        e = hipMemcpyAsync(h->L.W, w, h->ncell * (h->dim + 2) * sizeof(double),
+                           hipMemcpyHostToDevice, h->stream);
+    if (tmp) hipFreeAsync(tmp, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "set_f");
+    return HUGKS_OK;
+}
+
+int hugks_get_time(hugks_handle* h, double* simtime, int32_t* iter) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    DevState s;
+    hipError_t e = hipMemcpyAsync(&s, h->L.st, sizeof(s), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "get_time");
+    if (simtime) *simtime = s.simtime;
+    if (iter) *iter = s.iter;
+    return HUGKS_OK;
+}
+
+int hugks_kernel_times(hugks_handle* h, int32_t cap, const char** names, double* ms,
+                       int32_t* calls, int32_t* count) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    int n = 0;
+    for (auto& t : h->timers) {
+        if (n < cap) {
+            if (names) names[n] = t.name;
+            if (ms) ms[n] = t.ms;
+            if (calls) calls[n] = t.calls;
+        }
+        ++n;
+    }
+    if (count) *count = n;
+    return HUGKS_OK;
+}
+
+int hugks_reset_kernel_times(hugks_handle* h) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    h->timers.clear();
+    return HUGKS_OK;
+}
+
+int hugks_set_timing(hugks_handle* h, int32_t on) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    h->timing = on != 0;
+    return HUGKS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// Device-side building blocks shared by the Hermite-UGKS kernels (gfx950).
+//
+// Layout in HBM (one handle = one case):
+//   W        double[ncell][4]          conserved variables, cell c = i + nx*j
+//   h, b     T[ncell][ldv]             reduced distributions, node k = iu + nv*jv,
+//                                      ldv = nv*nv rounded up to 64 (pad is zero)
+//   sh?, sb? T[ncell][ldv]             van Leer slopes in x and y
+//   vfh,vfb  T[(nx+1)*ny][ldv]         vertical-face distribution fluxes
+//   hfh,hfb  T[nx*(ny+1)][ldv]         horizontal-face distribution fluxes
+//   vflux    double[(nx+1)*ny][4]      face conserved fluxes (FP64 always)
+//   hflux    double[nx*(ny+1)][4]
+// T is double (HUGKS_FP64) or float (mixed modes).  Velocity-space sums
+// are always accumulated in double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hugks {
+
+constexpr int WAVE = 64;
+
+// Per-launch constants (by value).
+struct Consts {
+    int nx, ny, nv, nn, ldv, nm;
+    int bc[4];
+    int tau_model, dt_rule, clamp_dt, stop_on_residual;
+    double dx, dy, pi, smv, K, gamma, prandtl, shk, muref, omega;
+    double cfl, maxtime, eps_res, umax_dt, vmax_dt, force_rate, tref, phi;
+    double tw[4];
+};
+
+// Loop state kept on the device so a sequence of steps needs no host
+// round trip.
+struct DevState {
+    double dt;
+    double simtime;
+    double res[4];
+    double steady[3];
+    int iter;
+    int stopped;      // set once an exit condition held; later steps no-op
+    int stop_reason;
+    int flags;        // bit 0: NaN/T<=0, bit 1: zero wall coefficient
+    int steps;        // steps executed since the last host reset
+    int pad_;
+};
+
+// ---------------------------------------------------------------- scalars
+// get_primary / get_conserved, 2d/src/gastheory2d.f90:28-46
+__device__ __forceinline__ void prim_from_w(const Consts& c, const double* w, double* p) {
+    p[0] = w[0];
+    p[1] = w[1] / w[0];
+    p[2] = w[2] / w[0];
+    p[3] = 0.5 * w[0] / (c.gamma - 1.0) / (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+}
+__device__ __forceinline__ void w_from_prim(const Consts& c, const double* p, double* w) {
+    w[0] = p[0];
+    w[1] = p[0] * p[1];
+    w[2] = p[0] * p[2];
+    w[3] = 0.5 * p[0] / p[3] / (c.gamma - 1.0) + 0.5 * p[0] * (p[1] * p[1] + p[2] * p[2]);
+}
+// get_tau, 2d/src/gastheory2d.f90:87-97
+__device__ __forceinline__ double tau_of(const Consts& c, double rho, double lam) {
+    if (c.tau_model == 0) return c.muref / rho;
+    return c.muref * 2.0 * pow(lam, 1.0 - c.omega) / rho;
+}
+// micro_slope, 2d/src/flux.f90:711-722
+__device__ __forceinline__ void micro_slope(const Consts& c, const double* p, const double* sw,
+                                            double* a) {
+    const double K = c.K;
+    a[3] = 4.0 * p[3] * p[3] / (K + 2.0) / p[0] *
+           (2.0 * sw[3] - 2.0 * p[1] * sw[1] - 2.0 * p[2] * sw[2] +
+            sw[0] * (p[1] * p[1] + p[2] * p[2] - 0.5 * (K + 2.0) / p[3]));
+    a[2] = 2.0 * p[3] / p[0] * (sw[2] - p[2] * sw[0]) - p[2] * a[3];
+    a[1] = 2.0 * p[3] / p[0] * (sw[1] - p[1] * sw[0]) - p[1] * a[3];
+    a[0] = sw[0] / p[0] - p[1] * a[1] - p[2] * a[2] -
+           0.5 * (p[1] * p[1] + p[2] * p[2] + 0.5 * (K + 2.0) / p[3]) * a[3];
+}
+
+// Half-range and full-range Maxwellian moments, 2d/src/flux.f90:730-762
+struct Moments {
+    double Mu[7], MuL[7], MuR[7], Mv[7], Mxi[3];
+};
+__device__ __forceinline__ void calc_moment(const Consts& c, const double* p, Moments& m) {
+    const double U = p[1], V = p[2], lam = p[3];
+    const double e = 0.5 * exp(-lam * U * U) / sqrt(c.pi * lam);
+    m.MuL[0] = 0.5 * erfc(-sqrt(lam) * U);
+    m.MuL[1] = U * m.MuL[0] + e;
+    m.MuR[0] = 0.5 * erfc(sqrt(lam) * U);
+    m.MuR[1] = U * m.MuR[0] - e;
+#pragma unroll
+    for (int i = 2; i <= 6; ++i) {
+        m.MuL[i] = U * m.MuL[i - 1] + 0.5 * (i - 1) * m.MuL[i - 2] / lam;
+        m.MuR[i] = U * m.MuR[i - 1] + 0.5 * (i - 1) * m.MuR[i - 2] / lam;
+    }
+#pragma unroll
+    for (int i = 0; i <= 6; ++i) m.Mu[i] = m.MuL[i] + m.MuR[i];
+    m.Mv[0] = 1.0;
+    m.Mv[1] = V;
+#pragma unroll
+    for (int i = 2; i <= 6; ++i) m.Mv[i] = V * m.Mv[i - 1] + 0.5 * (i - 1) * m.Mv[i - 2] / lam;
+    m.Mxi[0] = 1.0;
+    m.Mxi[1] = 0.5 * c.K / lam;
+    m.Mxi[2] = (c.K * c.K + 2.0 * c.K) / (4.0 * lam * lam);
+}
+// moment_uv, 2d/src/flux.f90:772-782
+__device__ __forceinline__ void moment_uv(const double* Mu, const double* Mv, const double* Mxi,
+                                          int al, int be, int th, double* o) {
+    o[0] = Mu[al] * Mv[be] * Mxi[th / 2];
+    o[1] = Mu[al + 1] * Mv[be] * Mxi[th / 2];
+    o[2] = Mu[al] * Mv[be + 1] * Mxi[th / 2];
+    o[3] = 0.5 * (Mu[al + 2] * Mv[be] * Mxi[th / 2] + Mu[al] * Mv[be + 2] * Mxi[th / 2] +
+                  Mu[al] * Mv[be] * Mxi[(th + 2) / 2]);
+}
+// moment_au, 2d/src/flux.f90:792-804
+__device__ __forceinline__ void moment_au(const double* a, const double* Mu, const double* Mv,
+                                          const double* Mxi, int al, int be, double* o) {
+    double t1[4], t2[4], t3[4], t4[4], t5[4], t6[4];
+    moment_uv(Mu, Mv, Mxi, al, be, 0, t1);
+    moment_uv(Mu, Mv, Mxi, al + 1, be, 0, t2);
+    moment_uv(Mu, Mv, Mxi, al, be + 1, 0, t3);
+    moment_uv(Mu, Mv, Mxi, al + 2, be, 0, t4);
+    moment_uv(Mu, Mv, Mxi, al, be + 2, 0, t5);
+    moment_uv(Mu, Mv, Mxi, al, be, 2, t6);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        o[q] = a[0] * t1[q] + a[1] * t2[q] + a[2] * t3[q] + 0.5 * a[3] * t4[q] +
+               0.5 * a[3] * t5[q] + 0.5 * a[3] * t6[q];
+}
+
+// Fortran sign(1d0, x): -1 for -0.0 as well (copysign semantics).
+__device__ __forceinline__ int upwind_delta(double x) { return copysign(1.0, x) > 0.0 ? 1 : 0; }
+__device__ __forceinline__ int upwind_delta(float x) { return copysignf(1.0f, x) > 0.0f ? 1 : 0; }
+
+// ----------------------------------------------------------- reductions
+// Sum of NV doubles over the block; every thread receives the totals.
+// Fixed order (wave butterfly, then waves in index order), so results
+// are bitwise reproducible run to run.  lds must hold NV*(NT/64) doubles.
+template <int NV, int NT>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double x = v[i];
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+        v[i] = x;
+    }
+    if (NT > WAVE) {
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) lds[wid * NV + i] = v[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = lds[i];
+#pragma unroll
+            for (int w = 1; w < NT / WAVE; ++w) s += lds[w * NV + i];
+            v[i] = s;
+        }
+        __syncthreads();
+    } else {
+        // one wave: broadcast lane 0's value so all lanes agree bitwise
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = __shfl(v[i], 0, WAVE);
+    }
+}
+
+template <typename T> struct Arith;           // per-node arithmetic type
+template <> struct Arith<double> { using A = double; };
+template <> struct Arith<float> { using A = float; };
+
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+__device__ __forceinline__ float fexp(float x) { return expf(x); }
+
+}  // namespace hugks

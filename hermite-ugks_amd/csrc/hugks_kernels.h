@@ -1,0 +1,46 @@
+// Host-side launch interface of the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "hugks.h"
+#include "hugks_device.h"
+
+namespace hugks {
+
+// Everything one 2D/1D step touches on the device (pointers are device
+// pointers; T-typed arrays are void* and interpreted by precision).
+struct Launch2D {
+    Consts c;
+    bool fp64;
+    DevState* st;
+    double* W;
+    double* part;        // per-cell residual partials [5*ncell]
+    void *h, *b, *shx, *shy, *sbx, *sby;
+    void *vfh, *vfb, *hfh, *hfb;
+    double *vflux, *hflux;
+    const void *unode, *vnode, *wnode;       // per-node tables in the arithmetic type
+    const double *unode_d, *vnode_d, *wnode_d;  // FP64 tables (moments)
+    const double *u1, *v1;                   // 1D node lists [nv] (double)
+    const double *acc_cell, *acc_v, *acc_h;
+    unsigned nwall;      // number of wall faces (non-periodic sides)
+    int update_nt;       // threads per update workgroup
+    size_t update_lds;   // dynamic LDS bytes of the update kernel
+};
+
+hipError_t launch_begin(DevState* st, hipStream_t s);
+hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s);
+hipError_t launch_flux2d(const Launch2D& L, hipStream_t s);
+hipError_t launch_update2d(const Launch2D& L, hipStream_t s);
+hipError_t launch_finalize(const Launch2D& L, int dim, hipStream_t s);
+hipError_t launch_init_f(const Launch2D& L, int dim, const double* prim0, hipStream_t s);
+hipError_t launch_macro(const Launch2D& L, int dim, double* q, hipStream_t s);
+hipError_t launch_convert(const void* src, bool src_fp64, double* dst, size_t ncell, int nn,
+                          int ldv, bool to_dense, hipStream_t s);
+hipError_t launch_convert_in(const double* src, void* dst, bool dst_fp64, size_t ncell, int nn,
+                             int ldv, hipStream_t s);
+
+// 1D path
+hipError_t launch_step1d(const Launch2D& L, hipStream_t s);
+
+size_t update2d_lds_bytes(const Consts& c, bool fp64);
+
+}  // namespace hugks

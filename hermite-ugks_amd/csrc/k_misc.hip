@@ -1,0 +1,304 @@
+// Time-step size, step finalisation (residual, loop state, exit tests),
+// initialisation of f, output moments and layout conversion (gfx950).
+#include "hugks_device.h"
+#include "hugks_kernels.h"
+
+namespace hugks {
+
+constexpr int RED_NT = 1024;
+
+template <int NV>
+__device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double x = v[i];
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, WAVE));
+        v[i] = x;
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) lds[wid * NV + i] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = lds[i];
+        for (int w = 1; w < RED_NT / WAVE; ++w) s = fmax(s, lds[w * NV + i]);
+        v[i] = s;
+    }
+    __syncthreads();
+}
+
+// Marks the start of a host call: later kernels run again even if a
+// previous call ended on an exit condition.
+__global__ void k_begin(DevState* st) {
+    if (threadIdx.x == 0) {
+        st->stopped = 0;
+        st->stop_reason = 0;
+        st->steps = 0;
+    }
+}
+
+// timestep, 2d/src/solver.f90:13-61 and 1d/src/solver.f90:16-47; one
+// workgroup.  The 1D loop tests simtime >= maxtime before the step
+// (1d/src/main.f90:84), so that test is made here.
+__global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState* st,
+                                                     const double* __restrict__ W) {
+    __shared__ double lds[RED_NT / WAVE];
+    if (st->stopped) return;
+    if (dim == 1 && st->simtime >= c.maxtime) {
+        if (threadIdx.x == 0) {
+            st->stopped = 1;
+            st->stop_reason = HUGKS_STOP_MAXTIME;
+        }
+        return;
+    }
+    const int ncell = c.nx * c.ny;
+    const double area = c.dx * c.dy;
+    double t[1] = {0.0};
+    for (int cell = threadIdx.x; cell < ncell; cell += RED_NT) {
+        if (dim == 2) {
+            double p[4];
+            prim_from_w(c, W + 4 * (size_t)cell, p);
+            const double sos = sqrt(0.5 * c.gamma / p[3]);
+            const double pu = fmax(c.umax_dt, fabs(p[1])) + sos;
+            const double pv = fmax(c.vmax_dt, fabs(p[2])) + sos;
+            const double rate = c.dt_rule == HUGKS_DT_2D_WALL_NORMAL ? pv / c.dy
+                                                                     : (c.dy * pu + c.dx * pv) / area;
+            t[0] = fmax(t[0], rate);
+        } else {
+            const double* w = W + 3 * (size_t)cell;
+            const double U = w[1] / w[0];
+            const double lam = 0.5 * w[0] / (c.gamma - 1.0) / (w[2] - 0.5 * w[1] * w[1] / w[0]);
+            const double sos = sqrt(0.5 * c.gamma / lam);
+            t[0] = fmax(t[0], (fmax(c.umax_dt, fabs(U)) + sos) / c.dx);
+        }
+    }
+    block_max<1>(t, lds);
+    if (threadIdx.x == 0) {
+        const double temp = fmax(t[0], c.force_rate);
+        const double tmin = 1.0 / temp;
+        double dt = c.cfl * tmin / 1.1;
+        if (c.clamp_dt) dt = fmin(dt, c.maxtime - st->simtime);
+        st->dt = dt;
+    }
+}
+
+// Residual and loop state after update(): 2d/src/solver.f90:393 with
+// 2d/src/main.f90:82,102-104; 1d/src/solver.f90:262 with 1d/src/main.f90:
+// 106-107.  Partials per cell: 2D (dw^2 x4, |rho|); 1D (dw^2 x3, |w| x3,
+// steady residual x3).  One workgroup, fixed summation order.
+__global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState* st,
+                                                     const double* __restrict__ part) {
+    __shared__ double lds[9 * (RED_NT / WAVE)];
+    if (st->stopped) return;
+    const int ncell = c.nx * c.ny;
+    const int np = dim == 2 ? 5 : 9;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    double mx[3] = {0, 0, 0};
+    for (int cell = threadIdx.x; cell < ncell; cell += RED_NT) {
+        const double* p = part + (size_t)np * cell;
+        if (dim == 2) {
+            for (int q = 0; q < 5; ++q) s[q] += p[q];
+        } else {
+            for (int q = 0; q < 6; ++q) s[q] += p[q];
+            for (int q = 0; q < 3; ++q) mx[q] = fmax(mx[q], p[6 + q]);
+        }
+    }
+    block_sum<6, RED_NT>(s, lds);
+    if (dim == 1) block_max<3>(mx, lds);
+    if (threadIdx.x == 0) {
+        const double n = (double)ncell;
+        bool all_small = true;
+        if (dim == 2) {
+            for (int q = 0; q < 4; ++q) {
+                st->res[q] = sqrt(n * s[q]) / (s[4] + c.smv);
+                all_small = all_small && st->res[q] < c.eps_res;
+            }
+            st->simtime = st->simtime + st->dt;
+            st->iter += 1;
+        } else {
+            for (int q = 0; q < 3; ++q) {
+                st->res[q] = sqrt(n * s[q]) / (s[3 + q] + c.smv);
+                st->steady[q] = mx[q];
+            }
+            st->iter += 1;
+            st->simtime = st->simtime + st->dt;
+            all_small = false;
+        }
+        st->steps += 1;
+        if (dim == 2 && st->simtime >= c.maxtime) {
+            st->stopped = 1;
+            st->stop_reason = HUGKS_STOP_MAXTIME;
+        } else if (dim == 2 && c.stop_on_residual && all_small) {
+            st->stopped = 1;
+            st->stop_reason = HUGKS_STOP_RESIDUAL;
+        }
+    }
+}
+
+// Reduced Maxwellian initial state, 2d/src/initialization.f90:227-235
+// (1d/src/initialization.f90:185-192) with W = get_conserved(prim0).
+__global__ void k_init_f(Consts c, int dim, const double* __restrict__ prim0,
+                         const double* __restrict__ u1, const double* __restrict__ v1,
+                         double* __restrict__ W, void* h, void* b, int fp64) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ncell = (size_t)c.nx * c.ny;
+    if (idx >= ncell * c.ldv) return;
+    const size_t cell = idx / c.ldv;
+    const int k = (int)(idx - cell * c.ldv);
+    double H = 0.0, B = 0.0;
+    if (dim == 2) {
+        const double* p = prim0 + 4 * cell;
+        if (k < c.nn) {
+            const int iu = k % c.nv, jv = k / c.nv;
+            const double cu = u1[iu] - p[1], cv = v1[jv] - p[2];
+            H = p[0] * (p[3] / c.pi) * exp(-p[3] * (cu * cu + cv * cv));
+            B = H * c.K / (2.0 * p[3]);
+        }
+        if (k == 0) w_from_prim(c, p, W + 4 * cell);
+    } else {
+        const double* p = prim0 + 3 * cell;
+        if (k < c.nn) {
+            const double cu = u1[k] - p[1];
+            H = p[0] * pow(p[2] / c.pi, 0.5) * exp(-p[2] * cu * cu);
+            B = H * c.K / (2.0 * p[2]);
+        }
+        if (k == 0) {
+            double* w = W + 3 * cell;
+            w[0] = p[0];
+            w[1] = p[0] * p[1];
+            w[2] = 0.5 * p[0] / p[2] / (c.gamma - 1.0) + 0.5 * p[0] * p[1] * p[1];
+        }
+    }
+    if (fp64) {
+        ((double*)h)[idx] = H;
+        ((double*)b)[idx] = B;
+    } else {
+        ((float*)h)[idx] = (float)H;
+        ((float*)b)[idx] = (float)B;
+    }
+}
+
+// Heat flux per cell for output: get_heat_flux(h, b, uspace, vspace, prim)
+// (2d/src/gastheory2d.f90:121-129; 1d/src/gastheory.f90:98-104).
+template <typename T>
+__global__ __launch_bounds__(256) void k_macro(Consts c, int dim, const double* __restrict__ W,
+                                               const T* __restrict__ h, const T* __restrict__ b,
+                                               const double* __restrict__ u1,
+                                               const double* __restrict__ v1,
+                                               const double* __restrict__ wnode,
+                                               double* __restrict__ q) {
+    __shared__ double lds[4 * 4];
+    const int cell = blockIdx.x;
+    double s[4] = {0, 0, 0, 0};
+    if (dim == 2) {
+        double p[4];
+        prim_from_w(c, W + 4 * (size_t)cell, p);
+        for (int k = threadIdx.x; k < c.nn; k += 256) {
+            const int iu = k % c.nv, jv = k / c.nv;
+            const double cu = u1[iu] - p[1], cv = v1[jv] - p[2];
+            const double c2 = cu * cu + cv * cv;
+            const double hh = (double)h[(size_t)cell * c.ldv + k];
+            const double bb = (double)b[(size_t)cell * c.ldv + k];
+            const double wt = wnode[k];
+            s[0] += wt * cu * c2 * hh;
+            s[1] += wt * cu * bb;
+            s[2] += wt * cv * c2 * hh;
+            s[3] += wt * cv * bb;
+        }
+    } else {
+        const double* w = W + 3 * (size_t)cell;
+        const double U = w[1] / w[0];
+        for (int k = threadIdx.x; k < c.nn; k += 256) {
+            const double cu = u1[k] - U;
+            const double hh = (double)h[(size_t)cell * c.ldv + k];
+            const double bb = (double)b[(size_t)cell * c.ldv + k];
+            s[0] += wnode[k] * cu * (cu * cu) * hh;
+            s[1] += wnode[k] * cu * bb;
+        }
+    }
+    block_sum<4, 256>(s, lds);
+    if (threadIdx.x == 0) {
+        if (dim == 2) {
+            q[2 * (size_t)cell] = 0.5 * (s[0] + s[1]);
+            q[2 * (size_t)cell + 1] = 0.5 * (s[2] + s[3]);
+        } else {
+            q[cell] = 0.5 * (s[0] + s[1]);
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_to_dense(const T* __restrict__ src, double* __restrict__ dst, size_t ncell,
+                           int nn, int ldv) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ncell * nn) return;
+    const size_t cell = idx / nn;
+    const int k = (int)(idx - cell * nn);
+    dst[idx] = (double)src[cell * ldv + k];
+}
+
+template <typename T>
+__global__ void k_from_dense(const double* __restrict__ src, T* __restrict__ dst, size_t ncell,
+                             int nn, int ldv) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ncell * ldv) return;
+    const size_t cell = idx / ldv;
+    const int k = (int)(idx - cell * ldv);
+    dst[idx] = k < nn ? (T)src[cell * nn + k] : (T)0;
+}
+
+hipError_t launch_finalize(const Launch2D& L, int dim, hipStream_t s) {
+    k_finalize<<<1, RED_NT, 0, s>>>(L.c, dim, L.st, L.part);
+    return hipGetLastError();
+}
+
+hipError_t launch_begin(DevState* st, hipStream_t s) {
+    k_begin<<<1, 64, 0, s>>>(st);
+    return hipGetLastError();
+}
+
+hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s) {
+    k_timestep<<<1, RED_NT, 0, s>>>(L.c, dim, L.st, L.W);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_f(const Launch2D& L, int dim, const double* prim0, hipStream_t s) {
+    const size_t n = (size_t)L.c.nx * L.c.ny * L.c.ldv;
+    k_init_f<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L.c, dim, prim0, L.u1, L.v1, L.W, L.h,
+                                                         L.b, L.fp64 ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_macro(const Launch2D& L, int dim, double* q, hipStream_t s) {
+    const unsigned ncell = (unsigned)(L.c.nx * L.c.ny);
+    if (L.fp64)
+        k_macro<double><<<ncell, 256, 0, s>>>(L.c, dim, L.W, (const double*)L.h,
+                                              (const double*)L.b, L.u1, L.v1, L.wnode_d, q);
+    else
+        k_macro<float><<<ncell, 256, 0, s>>>(L.c, dim, L.W, (const float*)L.h, (const float*)L.b,
+                                             L.u1, L.v1, L.wnode_d, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const void* src, bool src_fp64, double* dst, size_t ncell, int nn,
+                          int ldv, bool, hipStream_t s) {
+    const size_t n = ncell * nn;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (src_fp64) k_to_dense<double><<<g, 256, 0, s>>>((const double*)src, dst, ncell, nn, ldv);
+    else k_to_dense<float><<<g, 256, 0, s>>>((const float*)src, dst, ncell, nn, ldv);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_in(const double* src, void* dst, bool dst_fp64, size_t ncell, int nn,
+                             int ldv, hipStream_t s) {
+    const size_t n = ncell * ldv;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (dst_fp64) k_from_dense<double><<<g, 256, 0, s>>>(src, (double*)dst, ncell, nn, ldv);
+    else k_from_dense<float><<<g, 256, 0, s>>>(src, (float*)dst, ncell, nn, ldv);
+    return hipGetLastError();
+}
+
+}  // namespace hugks

@@ -1,0 +1,280 @@
+// 2D cell update with the Hermite spectral force term (gfx950).
+//
+// One workgroup per cell.  Reference: update(), 2d/src/solver.f90:245-395,
+// with the force of symmetric_hermite_force_term_2d,
+// 2d/src/hermite_symmetric2d.f90:23-104.
+//
+// The Hermite force is evaluated in separable form.  With g = weight*f
+// (nv x nv, u index fastest), Hx(r,iu) = He_r(xi_iu), Hy(s,jv) = He_s(eta_jv):
+//   Tm(r,jv) = sum_iu g(iu,jv) Hx(r,iu)                      (Nm+1) x nv
+//   D(r,s)   = (1/rt) sum_jv Tm(r,jv) Hy(s,jv) / (r! s!),     r+s <= Nm
+//   R(r,jv)  = a_x sum_s D(r-1,s) Hy(s,jv) + a_y sum_s D(r,s) Hy(s+1,jv)
+//   F(iu,jv) = -(1/sqrt(rt)) omega(iu,jv) sum_{r=0}^{Nm+1} Hx(r,iu) R(r,jv)
+// which is the reference's triangular double sum regrouped; it costs
+// O((Nm+1) nv^2) per cell instead of O((Nm+1)^2 nv^2 / 2).
+#include "hugks_device.h"
+#include "hugks_kernels.h"
+
+namespace hugks {
+
+// Rc of the solver's gas constant, lambda = 0.5/(Rc T) (2d/src/data2d.f90:54)
+constexpr double RC = 0.5;
+
+size_t update2d_lds_bytes(const Consts& c, bool fp64) {
+    const size_t nv = c.nv, nm = c.nm, nn = c.nn;
+    const size_t n = 2 * nn + 2 * (nm + 2) * nv + 2 * (nm + 1) * nv + 2 * (nm + 1) * (nm + 1) +
+                     2 * (nm + 2) * nv;
+    return n * (fp64 ? sizeof(double) : sizeof(float));
+}
+
+template <typename T, int EPT, int NT>
+__global__ __launch_bounds__(NT) void k_update2d(
+    Consts c, DevState* __restrict__ st, double* __restrict__ W, double* __restrict__ part,
+    T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
+    const T* __restrict__ hfh, const T* __restrict__ hfb, const double* __restrict__ vflux,
+    const double* __restrict__ hflux, const typename Arith<T>::A* __restrict__ unode,
+    const typename Arith<T>::A* __restrict__ vnode, const typename Arith<T>::A* __restrict__ wnode,
+    const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ acc_cell) {
+    using A = typename Arith<T>::A;
+    extern __shared__ double smem_raw[];
+    __shared__ double red[4 * (NT / WAVE)];
+    if (st->stopped) return;
+    const int tid = threadIdx.x;
+    const int cell = blockIdx.x;
+    const int nx = c.nx, nv = c.nv, nm = c.nm, nn = c.nn;
+    const int i = cell % nx, j = cell / nx;
+    const double dt = st->dt;
+    const double area = c.dx * c.dy;
+    const int vl = i + (nx + 1) * j, vr = vl + 1;
+    const int hd = i + nx * j, hu = i + nx * (j + 1);
+    const double ax = acc_cell[2 * cell], ay = acc_cell[2 * cell + 1];
+
+    // ---- macroscopic predictor (solver.f90:302-321)
+    double wo[4], po[4], wn[4], pn[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wo[q] = W[4 * (size_t)cell + q];
+    prim_from_w(c, wo, po);
+    const double tau_old = tau_of(c, po[0], po[3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        wn[q] = wo[q] + (vflux[4 * (size_t)vl + q] - vflux[4 * (size_t)vr + q] +
+                         hflux[4 * (size_t)hd + q] - hflux[4 * (size_t)hu + q]) / area;
+    prim_from_w(c, wn, pn);
+    pn[1] = pn[1] + ax * dt;
+    pn[2] = pn[2] + ay * dt;
+    w_from_prim(c, pn, wn);
+    const double tau = tau_of(c, pn[0], pn[3]);
+
+    // ---- LDS carve-up
+    A* G = reinterpret_cast<A*>(smem_raw);        // [2][nn]
+    A* Hx = G + 2 * nn;                           // [nm+2][nv]
+    A* Hy = Hx + (nm + 2) * nv;                   // [nm+2][nv]
+    A* Tm = Hy + (nm + 2) * nv;                   // [2][nm+1][nv]
+    A* D = Tm + 2 * (nm + 1) * nv;                // [2][nm+1][nm+1]
+    A* R = D + 2 * (nm + 1) * (nm + 1);           // [2][nm+2][nv]
+
+    // ---- load f^n, weighted copies, heat flux at t^n (solver.f90:337)
+    const double rt = RC / po[3];
+    const double sqrt_rt = sqrt(rt);
+    A rh[EPT], rb[EPT];
+    double q4[4] = {0.0, 0.0, 0.0, 0.0};
+    const A Uo = (A)po[1], Vo = (A)po[2];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int k = tid + e * NT;
+        rh[e] = rb[e] = (A)0;
+        if (k < nn) {
+            const size_t o = (size_t)cell * c.ldv + k;
+            rh[e] = (A)h[o];
+            rb[e] = (A)b[o];
+            const A wt = wnode[k];
+            const A cu = unode[k] - Uo, cv = vnode[k] - Vo;
+            const A c2 = cu * cu + cv * cv;
+            q4[0] += (double)(wt * cu * c2 * rh[e]);
+            q4[1] += (double)(wt * cu * rb[e]);
+            q4[2] += (double)(wt * cv * c2 * rh[e]);
+            q4[3] += (double)(wt * cv * rb[e]);
+            G[k] = wt * rh[e];
+            G[nn + k] = wt * rb[e];
+        }
+    }
+    // Hermite bases He_0..He_{Nm+1} at xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt)
+    // (build_probabilists_hermite, hermite_symmetric2d.f90:220-233)
+    if (tid < 2 * nv) {
+        const bool isx = tid < nv;
+        const int t = isx ? tid : tid - nv;
+        const A x = (A)((isx ? (u1[t] - po[1]) : (v1[t] - po[2])) / sqrt_rt);
+        A* Hb = isx ? Hx : Hy;
+        A p0 = (A)1, p1 = x;
+        Hb[t] = p0;
+        Hb[nv + t] = p1;
+        for (int n = 2; n <= nm + 1; ++n) {
+            const A p2 = x * p1 - (A)(n - 1) * p0;
+            Hb[n * nv + t] = p2;
+            p0 = p1;
+            p1 = p2;
+        }
+    }
+    block_sum<4, NT>(q4, red);   // also orders the LDS writes above
+    const double qf0 = 0.5 * (q4[0] + q4[1]), qf1 = 0.5 * (q4[2] + q4[3]);
+
+    // ---- Tm = Hx^T g  (contract the u index)
+    const int ntm = 2 * (nm + 1) * nv;
+    for (int t = tid; t < ntm; t += NT) {
+        const int f = t / ((nm + 1) * nv);
+        const int rem = t - f * (nm + 1) * nv;
+        const int r = rem / nv, jv = rem - r * nv;
+        const A* g = G + f * nn + nv * jv;
+        const A* hx = Hx + r * nv;
+        A acc = (A)0;
+        for (int iu = 0; iu < nv; ++iu) acc += g[iu] * hx[iu];
+        Tm[t] = acc;
+    }
+    __syncthreads();
+    // ---- D = jac * Tm Hy^T / (r! s!), triangular r+s <= Nm
+    const int m1 = nm + 1;
+    const double jac = 1.0 / rt;
+    for (int t = tid; t < 2 * m1 * m1; t += NT) {
+        const int f = t / (m1 * m1);
+        const int rem = t - f * m1 * m1;
+        const int r = rem / m1, s = rem - r * m1;
+        A val = (A)0;
+        if (r + s <= nm) {
+            const A* tm = Tm + (f * m1 + r) * nv;
+            const A* hy = Hy + s * nv;
+            A acc = (A)0;
+            for (int jv = 0; jv < nv; ++jv) acc += tm[jv] * hy[jv];
+            double fr = 1.0, fs = 1.0;
+            for (int q = 2; q <= r; ++q) fr *= (double)q;
+            for (int q = 2; q <= s; ++q) fs *= (double)q;
+            val = (A)(jac / (fr * fs)) * acc;
+        }
+        D[t] = val;
+    }
+    __syncthreads();
+    // ---- R(r,jv) = a_x sum_s D(r-1,s) Hy(s,jv) + a_y sum_s D(r,s) Hy(s+1,jv)
+    const A axA = (A)ax, ayA = (A)ay;
+    const int m2 = nm + 2;
+    for (int t = tid; t < 2 * m2 * nv; t += NT) {
+        const int f = t / (m2 * nv);
+        const int rem = t - f * m2 * nv;
+        const int r = rem / nv, jv = rem - r * nv;
+        const A* Df = D + f * m1 * m1;
+        A acc = (A)0;
+        if (r >= 1) {
+            A sx = (A)0;
+            for (int s = 0; s <= nm - (r - 1); ++s) sx += Df[(r - 1) * m1 + s] * Hy[s * nv + jv];
+            acc += axA * sx;
+        }
+        if (r <= nm) {
+            A sy = (A)0;
+            for (int s = 0; s <= nm - r; ++s) sy += Df[r * m1 + s] * Hy[(s + 1) * nv + jv];
+            acc += ayA * sy;
+        }
+        R[t] = acc;
+    }
+    __syncthreads();
+
+    // ---- force and implicit BGK update (solver.f90:337-388)
+    const A inv_sqrt_rt = (A)(1.0 / sqrt_rt);
+    const A two_pi = (A)(2.0 * c.pi);
+    const A lamo = (A)po[3], lamn = (A)pn[3];
+    const A Hco = (A)(po[0] * (po[3] / c.pi)), Hcn = (A)(pn[0] * (pn[3] / c.pi));
+    const A kfo = (A)(c.K / (2.0 * po[3])), kfn = (A)(c.K / (2.0 * pn[3]));
+    const A cso = (A)(c.shk * (1.0 - c.prandtl) * po[3] * po[3] / po[0]);
+    const A csn = (A)(c.shk * (1.0 - c.prandtl) * pn[3] * pn[3] / pn[0]);
+    const A Un = (A)pn[1], Vn = (A)pn[2];
+    const A q0 = (A)qf0, q1 = (A)qf1, K = (A)c.K;
+    const A dtA = (A)dt, half_dt = (A)(0.5 * dt);
+    const A itau = (A)(1.0 / tau), itau_o = (A)(1.0 / tau_old);
+    const A den = (A)(1.0 + 0.5 * dt / tau);
+    const A iarea = (A)(1.0 / area);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int k = tid + e * NT;
+        if (k < nn) {
+            const int iu = k % nv, jv = k / nv;
+            const A xi = Hx[nv + iu], eta = Hy[nv + jv];
+            const A om = fexp((A)-0.5 * (xi * xi + eta * eta)) / two_pi;
+            A sh = (A)0, sb = (A)0;
+            for (int r = 0; r <= nm + 1; ++r) {
+                const A hx = Hx[r * nv + iu];
+                sh += hx * R[r * nv + jv];
+                sb += hx * R[(m2 + r) * nv + jv];
+            }
+            const A Fh = -inv_sqrt_rt * om * sh;
+            const A Fb = -inv_sqrt_rt * om * sb;
+            // Maxwellians and Shakhov parts at t^n and t^{n+1}, both with the
+            // heat flux of h^n (solver.f90:337-347)
+            const A u = unode[k], v = vnode[k];
+            const A cuo = u - Uo, cvo = v - Vo, c2o = cuo * cuo + cvo * cvo;
+            const A Ho = Hco * fexp(-lamo * c2o), Bo = Ho * kfo;
+            const A cqo = cso * (cuo * q0 + cvo * q1);
+            const A Hpo = Ho + cqo * ((A)2 * lamo * c2o + K - (A)5) * Ho;
+            const A Bpo = Bo + cqo * ((A)2 * lamo * c2o + K - (A)3) * Bo;
+            const A cun = u - Un, cvn = v - Vn, c2n = cun * cun + cvn * cvn;
+            const A Hn = Hcn * fexp(-lamn * c2n), Bn = Hn * kfn;
+            const A cqn = csn * (cun * q0 + cvn * q1);
+            const A Hpn = Hn + cqn * ((A)2 * lamn * c2n + K - (A)5) * Hn;
+            const A Bpn = Bn + cqn * ((A)2 * lamn * c2n + K - (A)3) * Bn;
+            const A dfh = ((A)vfh[(size_t)vl * c.ldv + k] - (A)vfh[(size_t)vr * c.ldv + k] +
+                           (A)hfh[(size_t)hd * c.ldv + k] - (A)hfh[(size_t)hu * c.ldv + k]) * iarea;
+            const A dfb = ((A)vfb[(size_t)vl * c.ldv + k] - (A)vfb[(size_t)vr * c.ldv + k] +
+                           (A)hfb[(size_t)hd * c.ldv + k] - (A)hfb[(size_t)hu * c.ldv + k]) * iarea;
+            const A hn = (rh[e] + dfh + half_dt * (Hpn * itau + (Hpo - rh[e]) * itau_o) - Fh * dtA) / den;
+            const A bn = (rb[e] + dfb + half_dt * (Bpn * itau + (Bpo - rb[e]) * itau_o) - Fb * dtA) / den;
+            const size_t o = (size_t)cell * c.ldv + k;
+            h[o] = (T)hn;
+            b[o] = (T)bn;
+        }
+    }
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            W[4 * (size_t)cell + q] = wn[q];
+            part[5 * (size_t)cell + q] = (wo[q] - wn[q]) * (wo[q] - wn[q]);
+        }
+        part[5 * (size_t)cell + 4] = fabs(wn[0]);
+        if (!(pn[3] > 0.0) || !isfinite(pn[0]) || !isfinite(pn[1]) || !isfinite(pn[2]) ||
+            !isfinite(pn[3]) || !(pn[0] > 0.0))
+            atomicOr(&st->flags, 1);
+    }
+}
+
+template <typename T>
+static hipError_t launch_update_t(const Launch2D& L, hipStream_t s) {
+    const Consts& c = L.c;
+    constexpr int NT = 256;
+    const int ept = (c.ldv + NT - 1) / NT;
+    const unsigned ncell = (unsigned)(c.nx * c.ny);
+    using A = typename Arith<T>::A;
+#define HUGKS_UPD_CASE(E)                                                                       \
+    case E:                                                                                     \
+        (void)hipFuncSetAttribute((const void*)k_update2d<T, E, NT>,                            \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.update_lds);     \
+        k_update2d<T, E, NT><<<ncell, NT, L.update_lds, s>>>(                                   \
+            c, L.st, L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,           \
+            (const T*)L.hfh, (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.unode,              \
+            (const A*)L.vnode, (const A*)L.wnode, L.u1, L.v1, L.acc_cell);                      \
+        break;
+    switch (ept) {
+        HUGKS_UPD_CASE(1)
+        HUGKS_UPD_CASE(2)
+        HUGKS_UPD_CASE(3)
+        HUGKS_UPD_CASE(4)
+        HUGKS_UPD_CASE(5)
+        HUGKS_UPD_CASE(6)
+        HUGKS_UPD_CASE(7)
+        HUGKS_UPD_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef HUGKS_UPD_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
+    return L.fp64 ? launch_update_t<double>(L, s) : launch_update_t<float>(L, s);
+}
+
+}  // namespace hugks

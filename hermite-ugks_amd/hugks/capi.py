@@ -1,0 +1,232 @@
+"""ctypes binding of libhugks.so (include/hugks.h).
+
+The structures mirror the C declarations field for field; ``Solver`` wraps
+one handle.  Loading fails loudly when the library is missing: there is no
+fallback implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(HERE), "libhugks.so")
+
+ABI_VERSION = 1
+OK = 0
+E_ARG, E_NOMEM, E_DEVICE, E_UNSUPPORTED, E_NUMERIC, E_WALL, E_NOHIP = -1, -2, -3, -4, -5, -6, -7
+FP64, MIXED_FP32, MIXED_BF16X3 = 0, 1, 2
+BC_PERIODIC, BC_SPECULAR, BC_DIFFUSE = 0, 1, 2
+TAU_MU_OVER_RHO, TAU_VHS = 0, 1
+DT_2D_GENERAL, DT_2D_WALL_NORMAL, DT_1D = 0, 1, 2
+FORCE_HERMITE_SYMMETRIC, FORCE_HERMITE_SYMMETRIC_B, FORCE_HERMITE_LEGACY = 0, 1, 2
+STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("dim", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+        ("nv", C.c_int32), ("nm", C.c_int32), ("precision", C.c_int32),
+        ("tau_model", C.c_int32), ("dt_rule", C.c_int32), ("clamp_dt", C.c_int32),
+        ("stop_on_residual", C.c_int32), ("bc", C.c_int32 * 4), ("force_variant", C.c_int32),
+        ("reserved_i", C.c_int32 * 2),
+        ("dx", C.c_double), ("dy", C.c_double), ("pi", C.c_double), ("smv", C.c_double),
+        ("innerk", C.c_double), ("gamma", C.c_double), ("prandtl", C.c_double),
+        ("shakhov_coef", C.c_double), ("muref", C.c_double), ("omega", C.c_double),
+        ("cfl", C.c_double), ("maxtime", C.c_double), ("eps_res", C.c_double),
+        ("umax_dt", C.c_double), ("vmax_dt", C.c_double), ("force_rate", C.c_double),
+        ("tw", C.c_double * 4), ("tref", C.c_double), ("phi", C.c_double),
+        ("reserved_d", C.c_double * 4),
+        ("u", _dp), ("v", _dp), ("weight", _dp), ("prim0", _dp), ("accel_cell", _dp),
+        ("accel_vface", _dp), ("accel_hface", _dp),
+    ]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32), ("iter", C.c_int32), ("stop_reason", C.c_int32),
+        ("device_flags", C.c_int32), ("simtime", C.c_double), ("dt", C.c_double),
+        ("res", C.c_double * 4), ("steady_res", C.c_double * 3),
+    ]
+
+    def as_dict(self):
+        return {"steps": self.steps, "iter": self.iter, "stop_reason": self.stop_reason,
+                "device_flags": self.device_flags, "simtime": self.simtime, "dt": self.dt,
+                "res": list(self.res), "steady_res": list(self.steady_res)}
+
+
+EXPORTS = [
+    "hugks_create", "hugks_destroy", "hugks_last_error", "hugks_step", "hugks_advance",
+    "hugks_get_macro", "hugks_get_f", "hugks_set_f", "hugks_get_time", "hugks_kernel_times",
+    "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info",
+]
+
+_lib = None
+
+
+def load(path=LIBPATH):
+    """Loads libhugks.so once; raises if it is missing or incomplete."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libhugks.so not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name in EXPORTS:
+        if not hasattr(lib, name):
+            raise RuntimeError(f"libhugks.so lacks {name}")
+    H = C.c_void_p
+    lib.hugks_create.argtypes = [C.POINTER(Desc), C.POINTER(H)]
+    lib.hugks_destroy.argtypes = [H]
+    lib.hugks_last_error.argtypes = [H]
+    lib.hugks_last_error.restype = C.c_char_p
+    lib.hugks_step.argtypes = [H, C.POINTER(StepInfo)]
+    lib.hugks_advance.argtypes = [H, C.c_int32, C.POINTER(StepInfo)]
+    lib.hugks_get_macro.argtypes = [H, _dp, _dp]
+    lib.hugks_get_f.argtypes = [H, _dp, _dp]
+    lib.hugks_set_f.argtypes = [H, _dp, _dp, _dp]
+    lib.hugks_get_time.argtypes = [H, _dp, C.POINTER(C.c_int32)]
+    lib.hugks_kernel_times.argtypes = [H, C.c_int32, C.POINTER(C.c_char_p), _dp,
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    lib.hugks_reset_kernel_times.argtypes = [H]
+    lib.hugks_set_timing.argtypes = [H, C.c_int32]
+    lib.hugks_device_info.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+class HugksError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hugks error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Case:
+    """A fully specified case: descriptor scalars plus the arrays it points
+    to (kept alive here)."""
+
+    def __init__(self, desc, arrays, meta=None):
+        self.desc = desc
+        self.arrays = arrays
+        self.meta = meta or {}
+
+    @property
+    def ncell(self):
+        return self.desc.nx * self.desc.ny
+
+    @property
+    def nn(self):
+        return self.desc.nv ** self.desc.dim
+
+    def with_precision(self, precision):
+        d = Desc.from_buffer_copy(self.desc)
+        d.precision = precision
+        return Case(d, self.arrays, dict(self.meta))
+
+    def bind(self):
+        """Points the descriptor at the numpy arrays and returns it."""
+        a = self.arrays
+        d = self.desc
+        d.u = _ptr(a["u"])
+        d.v = _ptr(a.get("v"))
+        d.weight = _ptr(a["weight"])
+        d.prim0 = _ptr(a["prim0"])
+        d.accel_cell = _ptr(a.get("accel_cell"))
+        d.accel_vface = _ptr(a.get("accel_vface"))
+        d.accel_hface = _ptr(a.get("accel_hface"))
+        return d
+
+
+class Solver:
+    """One library handle (one case on the current HIP device)."""
+
+    def __init__(self, case):
+        self.lib = load()
+        self.case = case
+        self.h = C.c_void_p()
+        d = case.bind()
+        rc = self.lib.hugks_create(C.byref(d), C.byref(self.h))
+        if rc != OK:
+            raise HugksError(rc, self.lib.hugks_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != OK:
+            raise HugksError(rc, self.lib.hugks_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.hugks_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self):
+        info = StepInfo()
+        self._check(self.lib.hugks_step(self.h, C.byref(info)))
+        return info
+
+    def advance(self, max_steps):
+        info = StepInfo()
+        self._check(self.lib.hugks_advance(self.h, int(max_steps), C.byref(info)))
+        return info
+
+    def macro(self, with_q=True):
+        d = self.case.desc
+        w = np.empty((self.case.ncell, d.dim + 2))
+        q = np.empty((self.case.ncell, d.dim)) if with_q else None
+        self._check(self.lib.hugks_get_macro(self.h, _ptr(w), _ptr(q)))
+        return (w, q) if with_q else w
+
+    def get_f(self):
+        n = self.case.ncell * self.case.nn
+        hf = np.empty(n)
+        bf = np.empty(n)
+        self._check(self.lib.hugks_get_f(self.h, _ptr(hf), _ptr(bf)))
+        return hf.reshape(self.case.ncell, self.case.nn), bf.reshape(self.case.ncell, self.case.nn)
+
+    def set_f(self, hf, bf, w=None):
+        hf = np.ascontiguousarray(hf, dtype=np.float64)
+        bf = np.ascontiguousarray(bf, dtype=np.float64)
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+        self._check(self.lib.hugks_set_f(self.h, _ptr(hf), _ptr(bf), _ptr(w)))
+
+    def time(self):
+        t = C.c_double()
+        it = C.c_int32()
+        self._check(self.lib.hugks_get_time(self.h, C.byref(t), C.byref(it)))
+        return t.value, it.value
+
+    def set_timing(self, on):
+        self._check(self.lib.hugks_set_timing(self.h, 1 if on else 0))
+
+    def reset_kernel_times(self):
+        self._check(self.lib.hugks_reset_kernel_times(self.h))
+
+    def kernel_times(self):
+        cap = 32
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        calls = (C.c_int32 * cap)()
+        n = C.c_int32()
+        self._check(self.lib.hugks_kernel_times(self.h, cap, names, ms, calls, C.byref(n)))
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(min(n.value, cap))}
+
+
+def device_info():
+    return load().hugks_device_info().decode()

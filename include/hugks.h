@@ -1,0 +1,194 @@
+/*
+ * hugks.h -- C-ABI of the MI355X Hermite-UGKS time-step library
+ * (libhugks.so, HIP/gfx950).  Plain C types only; callable from Fortran
+ * through ISO_C_BINDING (hermite-ugks_amd/driver/hugks_capi.f90), from C,
+ * and from Python ctypes.
+ *
+ * The reference solver has no FFI: its hot path is four argument-less
+ * Fortran subroutines acting on module globals, called once per step by
+ * the main program:
+ *     timestep()       2d/src/solver.f90:13-61    1d/src/solver.f90:16-47
+ *     interpolation()  2d/src/solver.f90:64-122   1d/src/solver.f90:49-65
+ *     evolution()      2d/src/solver.f90:161-243  1d/src/solver.f90:106-139
+ *     update()         2d/src/solver.f90:245-395  1d/src/solver.f90:141-264
+ * called at 2d/src/main.f90:78-82 and 1d/src/main.f90:86-89.  The entry
+ * points below replace exactly that seam.  Everything else (namelist,
+ * case selection, velocity quadrature, initial condition, output) stays in
+ * the caller, which hands over the data those routines read from the
+ * reference modules CONFIG/ALLDATA/VELOCITY.
+ *
+ * Conventions
+ *   - Every function returns an int status (HUGKS_OK == 0, negative on
+ *     error) and never aborts or throws across the ABI.
+ *   - The caller owns all host arrays; they are read or written during the
+ *     call only.  The library owns its device memory (allocated in
+ *     hugks_create, released in hugks_destroy) and one HIP stream per
+ *     handle.
+ *   - Cell arrays are cell-major with x fastest (the iteration order of
+ *     the reference's ctr(i,j) in Fortran), i.e. cell c = i + nx*j.
+ *     Per-cell velocity arrays are u fastest (Fortran h(unum,vnum)),
+ *     node k = iu + nv*jv.
+ *   - Calls on one handle are not re-entrant; distinct handles are
+ *     independent.  All calls are synchronous on return.
+ */
+#ifndef HUGKS_H
+#define HUGKS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HUGKS_ABI_VERSION 1
+
+/* status codes */
+#define HUGKS_OK              0
+#define HUGKS_E_ARG          -1  /* invalid descriptor or argument           */
+#define HUGKS_E_NOMEM        -2  /* device allocation failed                 */
+#define HUGKS_E_DEVICE       -3  /* HIP runtime error                        */
+#define HUGKS_E_UNSUPPORTED  -4  /* size/precision without a kernel          */
+#define HUGKS_E_NUMERIC      -5  /* NaN/Inf or T <= 0 met on the device      */
+#define HUGKS_E_WALL         -6  /* zero outgoing wall mass coefficient      */
+#define HUGKS_E_NOHIP        -7  /* no HIP device visible                    */
+
+/* precision of f storage and per-node arithmetic */
+#define HUGKS_FP64           0  /* everything FP64 (parity path)            */
+#define HUGKS_MIXED_FP32     1  /* FP32 f/flux, FP64 moments and W,
+                                   FP32 Hermite contractions                 */
+#define HUGKS_MIXED_BF16X3   2  /* as MIXED_FP32, Hermite as bf16x3 MFMA     */
+
+/* boundary kind per side (x-lo, x-hi, y-lo, y-hi) */
+#define HUGKS_BC_PERIODIC    0  /* 2d/src/boundary.f90:16-53                 */
+#define HUGKS_BC_SPECULAR    1  /* 2d/src/flux.f90:551-689, 1d:188-239       */
+#define HUGKS_BC_DIFFUSE     2  /* 2d/src/flux.f90:474-549, 1d:299-360       */
+
+/* collision-time model, 2d/src/gastheory2d.f90:87-97 */
+#define HUGKS_TAU_MU_OVER_RHO 0 /* case 5: muref/rho                        */
+#define HUGKS_TAU_VHS         1 /* muref*2*lambda^(1-omega)/rho             */
+
+/* CFL rule */
+#define HUGKS_DT_2D_GENERAL     0 /* 2d/src/solver.f90:45-46                 */
+#define HUGKS_DT_2D_WALL_NORMAL 1 /* case 5, 2d/src/solver.f90:43            */
+#define HUGKS_DT_1D             2 /* 1d/src/solver.f90:38                    */
+
+/* force_type of the reference &params namelist (2d/src/solver.f90:352-378,
+ * 1d/src/solver.f90:233-242); the three 2D Hermite forms are one operator
+ * (2d/tests/test_symmetric_hermite.f90:81-84 pins them equal to 1e-12) */
+#define HUGKS_FORCE_HERMITE_SYMMETRIC   0 /* 2d/src/hermite_symmetric2d.f90:23  */
+#define HUGKS_FORCE_HERMITE_SYMMETRIC_B 1 /* 2d/src/hermite_symmetric2d.f90:120 */
+#define HUGKS_FORCE_HERMITE_LEGACY      2 /* 2d/src/gravity.f90:189; 1D cal_fv */
+
+/* reason an advance stopped */
+#define HUGKS_STOP_NONE      0  /* step cap reached                         */
+#define HUGKS_STOP_MAXTIME   1  /* simtime >= maxtime                       */
+#define HUGKS_STOP_RESIDUAL  2  /* all(res < eps_res) (2D case 5)           */
+
+typedef struct hugks_desc {
+    int32_t abi_version;      /* = HUGKS_ABI_VERSION                          */
+    int32_t dim;              /* 1 or 2                                       */
+    int32_t nx, ny;           /* physical cells (ny = 1 for dim = 1)          */
+    int32_t nv;               /* velocity nodes per dimension                 */
+    int32_t nm;               /* Hermite force order (Nm >= 0)                */
+    int32_t precision;        /* HUGKS_FP64 / _MIXED_FP32 / _MIXED_BF16X3     */
+    int32_t tau_model;        /* HUGKS_TAU_*                                  */
+    int32_t dt_rule;          /* HUGKS_DT_*                                   */
+    int32_t clamp_dt;         /* 1: dt = min(dt, maxtime - simtime) (2D)      */
+    int32_t stop_on_residual; /* 1: advance stops when all(res < eps_res)     */
+    int32_t bc[4];            /* x-lo, x-hi, y-lo, y-hi (1D uses bc[0..1])    */
+    int32_t force_variant;    /* HUGKS_FORCE_*: the reference operator named
+                                 by force_type; the library evaluates all of
+                                 them with one separable Hermite operator    */
+    int32_t reserved_i[2];
+    double dx, dy;            /* cell sizes (uniform grid; area = dx*dy)      */
+    double pi;                /* the caller's PI constant (3.1415926535d0)    */
+    double smv;               /* limiter/residual guard (2D 1e-8, 1D tiny)    */
+    double innerk;            /* internal degrees of freedom K                */
+    double gamma;             /* ratio of specific heats                      */
+    double prandtl;           /* Prandtl number (Shakhov)                     */
+    double shakhov_coef;      /* Shakhov prefactor as the caller evaluates it */
+    double muref;             /* reference viscosity                          */
+    double omega;             /* VHS exponent                                 */
+    double cfl, maxtime;
+    double eps_res;           /* residual stop threshold                      */
+    double umax_dt, vmax_dt;  /* velocity bounds of the CFL rule              */
+    double force_rate;        /* |phi|/velocity-gap, fixed lower bound of 1/dt */
+    double tw[4];             /* wall temperature per side (diffuse walls)    */
+    double tref;              /* temperature scale of the 1D steady residual  */
+    double phi;               /* 1D acceleration (2D uses the arrays below)   */
+    double reserved_d[4];
+    const double *u;          /* [nv] u nodes                                 */
+    const double *v;          /* [nv] v nodes (2D)                            */
+    const double *weight;     /* [nv*nv] (2D, u fastest) or [nv] (1D)         */
+    const double *prim0;      /* [(dim+2)*ncell] initial (rho,U,[V],lambda)   */
+    const double *accel_cell; /* 2D: [2*ncell] (ax, ay) per cell              */
+    const double *accel_vface;/* 2D: [2*(nx+1)*ny] (fn, ft), face i+(nx+1)*j  */
+    const double *accel_hface;/* 2D: [2*nx*(ny+1)] (fn, ft), face i+nx*j      */
+} hugks_desc;
+
+typedef struct hugks_step_info {
+    int32_t steps;            /* steps taken by this call                     */
+    int32_t iter;             /* reference iteration index (starts at 1)      */
+    int32_t stop_reason;      /* HUGKS_STOP_*                                 */
+    int32_t device_flags;     /* bit 0 NaN/T<=0, bit 1 zero wall coefficient  */
+    double simtime;
+    double dt;                /* dt of the last step                          */
+    double res[4];            /* residual per conserved component             */
+    double steady_res[3];     /* 1D primitive-change residual                 */
+} hugks_step_info;
+
+typedef struct hugks_handle hugks_handle;
+
+/* Replaces initialize()'s allocation and distribution-function setup
+ * (2d/src/initialization.f90:28-43,195-237; 1d:27-38,166-194): copies the
+ * descriptor's arrays, allocates device state, sets h,b to the reduced
+ * Maxwellian of prim0 (2d/src/gastheory2d.f90:6-13) and W to its
+ * conserved variables (2d/src/gastheory2d.f90:28-36). */
+int hugks_create(const hugks_desc *desc, hugks_handle **out);
+int hugks_destroy(hugks_handle *h);
+
+/* Message of the last failing call on h (or of the last failed create
+ * when h is NULL).  Valid until the next call. */
+const char *hugks_last_error(const hugks_handle *h);
+
+/* One reference step: timestep, interpolation, evolution, update,
+ * simtime += dt, iter += 1 (2d/src/main.f90:78-82,102; 1d:86-89,106-107). */
+int hugks_step(hugks_handle *h, hugks_step_info *info);
+
+/* Up to max_steps steps, with the loop exits of the reference main
+ * program evaluated on the device (2d/src/main.f90:103-104; 1d:84).
+ * max_steps <= 0 runs until an exit condition. */
+int hugks_advance(hugks_handle *h, int32_t max_steps, hugks_step_info *info);
+
+/* Output fields of write_macro (2d/src/output2d.f90:156-161;
+ * 1d/src/output.f90:309-321): w = conserved variables
+ * [(dim+2)*ncell], q = heat flux [dim*ncell] from
+ * get_heat_flux(h, b, prim(w)).  Either pointer may be NULL. */
+int hugks_get_macro(hugks_handle *h, double *w, double *q);
+
+/* Distribution functions as FP64, [ncell * nv^dim] each. */
+int hugks_get_f(hugks_handle *h, double *hf, double *bf);
+/* Overwrites h, b and (when w != NULL) W. */
+int hugks_set_f(hugks_handle *h, const double *hf, const double *bf,
+                const double *w);
+
+/* Loop state (simtime, iter) of the handle. */
+int hugks_get_time(hugks_handle *h, double *simtime, int32_t *iter);
+
+/* Per-kernel device time accumulated since the last reset, from HIP
+ * events around each launch.  names/ms/calls have room for cap entries;
+ * *count receives the number of kernels.  names point into the library. */
+int hugks_kernel_times(hugks_handle *h, int32_t cap, const char **names,
+                       double *ms, int32_t *calls, int32_t *count);
+int hugks_reset_kernel_times(hugks_handle *h);
+/* 1: time every launch with HIP events (adds a sync per launch);
+ * 0: off (default). */
+int hugks_set_timing(hugks_handle *h, int32_t on);
+
+/* Library/device information string (gfx target, CU count). */
+const char *hugks_device_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HUGKS_H */

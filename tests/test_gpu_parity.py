@@ -1,0 +1,81 @@
+"""Parity of the HIP library (through the C-ABI) with the FP64 oracle.
+
+FP64 path: after N steps every conserved field agrees with the oracle to
+1e-10 relative to that field's largest magnitude.  Mixed path (FP32 f,
+FP64 moments): 1e-5 on the same measure.
+"""
+import numpy as np
+import pytest
+
+from hugks import capi, cases
+import oracle_py
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+TOL_MIXED = 1e-5
+
+
+def poiseuille(nx=4, ny=16, nv=12, nm=6, kn=0.1, phi=0.05, force="hermite_symmetric", tw=(1.0, 1.0)):
+    return dict(nx=nx, ny=ny, xlength=0.1, ylength=1.0, nv=nv, vmax=5.0, Tref=1.0,
+                quad_type="gauss_hermite", force_type=force, Nm=nm, cfl=0.8, maxtime=200.0,
+                case_id=5, knudsen=kn, phi=phi, TwL=tw[0], TwR=tw[1])
+
+
+def rt(nx=12, ny=16, nv=12, nm=6, kn=1e-3, phi=1.0, xlength=0.5):
+    return dict(nx=nx, ny=ny, xlength=xlength, ylength=1.0, nv=nv, vmax=5.0, Tref=1.0,
+                quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=nm, cfl=0.5,
+                maxtime=0.14, case_id=6, knudsen=kn, phi=phi, TwL=1.0, TwR=1.0)
+
+
+def field_err(a, b):
+    scale = np.maximum(np.max(np.abs(b), axis=0), 1e-300)
+    return np.max(np.abs(a - b), axis=0) / scale
+
+
+def run_pair(case, steps):
+    print(f"case nx={case.desc.nx} ny={case.desc.ny} nv={case.desc.nv} nm={case.desc.nm} "
+          f"prec={case.desc.precision} steps={steps}", flush=True)
+    with capi.Solver(case) as s:
+        info = s.advance(steps)
+        w, q = s.macro()
+        hf, bf = s.get_f()
+    o = oracle_py.Oracle(case.with_precision(capi.FP64))
+    o.step(steps)
+    wr, hr, br = o.get()
+    qr = o.heat_flux()
+    t, it, dt = o.time()
+    return dict(info=info, w=w, q=q, h=hf, b=bf, wr=wr, qr=qr, hr=hr, br=br, t=t, it=it, dt=dt)
+
+
+@pytest.mark.parametrize("name,params,variant,steps", [
+    ("poiseuille", poiseuille(), "reference", 20),
+    ("poiseuille_legacy_force", poiseuille(force="hermite", tw=(1.0, 1.3)), "reference", 10),
+    ("rt_reference", rt(xlength=1.0, phi=-1.5, kn=0.01), "reference", 10),
+    ("rt_cartesian", rt(), "cartesian", 10),
+])
+def test_fp64_parity(have_gpu, name, params, variant, steps):
+    case = cases.build_2d(params, capi.FP64, rt_variant=variant, seed=3)
+    r = run_pair(case, steps)
+    assert r["info"].steps == steps
+    assert r["info"].iter == r["it"]
+    assert abs(r["info"].simtime - r["t"]) <= 1e-12 * r["t"]
+    err = field_err(r["w"], r["wr"])
+    print(name, "W err", err)
+    assert np.all(err <= TOL_FP64), err
+    fe = np.max(np.abs(r["h"] - r["hr"])) / np.max(np.abs(r["hr"]))
+    assert fe <= TOL_FP64, fe
+    qe = np.max(np.abs(r["q"] - r["qr"])) / max(np.max(np.abs(r["qr"])), 1e-300)
+    assert qe <= 1e-8, qe
+
+
+@pytest.mark.parametrize("name,params,variant,steps", [
+    ("poiseuille", poiseuille(), "reference", 20),
+    ("rt_cartesian", rt(), "cartesian", 10),
+])
+def test_mixed_parity(have_gpu, name, params, variant, steps):
+    case = cases.build_2d(params, capi.MIXED_FP32, rt_variant=variant, seed=3)
+    r = run_pair(case, steps)
+    err = field_err(r["w"], r["wr"])
+    print(name, "mixed W err", err)
+    assert np.all(err <= TOL_MIXED), err
