@@ -1,0 +1,54 @@
+"""Error measures used by the parity tests, smoke and bench.
+
+field_errors(w, wr): per conserved component, max over cells of |w - wr|
+divided by the field's scale: max |rho| for density, max |E| for energy
+and, for both momentum components, the largest momentum magnitude
+|(rho U, rho V)| (a component that is zero up to round-off, such as the
+wall-normal momentum of Poiseuille flow, is measured against the momentum
+the flow actually carries).
+
+prim_errors(w, wr): per primitive field (rho, U, V, T) on the same idea:
+velocities against the largest speed, T against max T.
+"""
+import numpy as np
+
+
+def field_errors(w, wr):
+    w = np.asarray(w)
+    wr = np.asarray(wr)
+    d = w.shape[1] - 2
+    err = np.empty(w.shape[1])
+    err[0] = np.max(np.abs(w[:, 0] - wr[:, 0])) / np.max(np.abs(wr[:, 0]))
+    mom = np.sqrt(np.sum(wr[:, 1:1 + d] ** 2, axis=1))
+    ms = max(np.max(mom), 1e-300)
+    for q in range(1, 1 + d):
+        err[q] = np.max(np.abs(w[:, q] - wr[:, q])) / ms
+    err[-1] = np.max(np.abs(w[:, -1] - wr[:, -1])) / np.max(np.abs(wr[:, -1]))
+    return err
+
+
+def primitives(w, gamma):
+    w = np.asarray(w)
+    d = w.shape[1] - 2
+    rho = w[:, 0]
+    vel = w[:, 1:1 + d] / rho[:, None]
+    ke = 0.5 * np.sum(w[:, 1:1 + d] ** 2, axis=1) / rho
+    lam = 0.5 * rho / (gamma - 1.0) / (w[:, -1] - ke)
+    return rho, vel, 1.0 / lam
+
+
+def prim_errors(w, wr, gamma):
+    """Relative errors of rho, each velocity component and T."""
+    r, v, t = primitives(w, gamma)
+    rr, vr, tr = primitives(wr, gamma)
+    out = [np.max(np.abs(r - rr)) / np.max(np.abs(rr))]
+    vs = max(np.max(np.sqrt(np.sum(vr ** 2, axis=1))), 1e-300)
+    for q in range(v.shape[1]):
+        out.append(np.max(np.abs(v[:, q] - vr[:, q])) / vs)
+    out.append(np.max(np.abs(t - tr)) / np.max(np.abs(tr)))
+    return np.array(out)
+
+
+def q_error(q, qr):
+    qs = max(np.max(np.sqrt(np.sum(np.asarray(qr) ** 2, axis=1))), 1e-300)
+    return np.max(np.abs(np.asarray(q) - np.asarray(qr))) / qs
