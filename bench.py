@@ -1,0 +1,220 @@
+"""Benchmark of the Hermite-UGKS time step on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|c2]
+                [--no-cpu] [--no-fp64]
+
+A step is one full reference time step (timestep, interpolation,
+evolution, update) over the whole phase space of the workload.  The
+default workload is the large 2D Rayleigh-Taylor case (512 x 512 cells,
+40 x 40 Gauss-Hermite nodes, Nm = 12, about 419M phase-space points), the
+largest single-GPU configuration of the benchmark set.  The headline value
+is the mixed-precision path's phase-space updates per second
+(cells * nv^2 per step); the plain FP64 path is timed on the same
+workload and the mixed-path error against it is reported per field.
+
+Prints one JSON line (rank 0).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "hermite-ugks_amd"), os.path.join(ROOT, "tests")]
+
+from hugks import capi, cases  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6
+
+WORKLOADS = {
+    # large 2D Rayleigh-Taylor: 512x512, GH40, Kn=1e-3, Nm=12
+    "c5": dict(nx=512, ny=512, xlength=0.5, ylength=1.0, nv=40, vmax=5.0, Tref=1.0,
+               quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=12, cfl=0.5,
+               maxtime=0.14, case_id=6, knudsen=1e-3, phi=1.0, TwL=1.0, TwR=1.0),
+    # 2D Rayleigh-Taylor 120x120, GH28, Kn=1e-3, Nm=10
+    "c3": dict(nx=120, ny=120, xlength=0.5, ylength=1.0, nv=28, vmax=5.0, Tref=1.0,
+               quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=10, cfl=0.5,
+               maxtime=0.14, case_id=6, knudsen=1e-3, phi=1.0, TwL=1.0, TwR=1.0),
+    # 2D force-driven Poiseuille 4x64, GH24, Kn=0.1, phi=0.05, Nm=8
+    "c2": dict(nx=4, ny=64, xlength=0.1, ylength=1.0, nv=24, vmax=5.0, Tref=1.0,
+               quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=8, cfl=0.8,
+               maxtime=200.0, case_id=5, knudsen=0.1, phi=0.05, TwL=1.0, TwR=1.0),
+}
+VARIANT = {"c5": "cartesian", "c3": "cartesian", "c2": "reference"}
+NAMES = {"c5": "rt2d_512x512_gh40_nm12", "c3": "rt2d_120x120_gh28_nm10",
+         "c2": "poiseuille2d_4x64_gh24_nm8"}
+
+
+def algorithmic_bytes(case, kernel):
+    """Algorithmic HBM bytes of one launch: each array the kernel must
+    touch counted once (N = cells*nv^2, s = bytes per stored element)."""
+    d = case.desc
+    s = 8 if d.precision == capi.FP64 else 4
+    nn = d.nv * d.nv
+    N = d.nx * d.ny * nn
+    nvf = (d.nx + 1) * d.ny * nn
+    nhf = d.nx * (d.ny + 1) * nn
+    if kernel == "slope2d":          # read h, b; write 4 slope arrays
+        return 6 * s * N
+    if kernel == "flux_vface":       # read h, b and 4 slopes; write 2 face arrays
+        return 6 * s * N + 2 * s * nvf
+    if kernel == "flux_hface":
+        return 6 * s * N + 2 * s * nhf
+    if kernel == "update2d":         # read h, b, 4 face arrays; write h, b
+        return 4 * s * N + 2 * s * (nvf + nhf)
+    return None
+
+
+def run_gpu(case, warmup, steps, timing=True):
+    import torch
+    s = capi.Solver(case)
+    s.advance(warmup)
+    torch.cuda.synchronize()
+    s.set_timing(timing)
+    s.reset_kernel_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = s.advance(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = s.kernel_times()
+    w = s.macro(with_q=False)
+    s.close()
+    return dict(seconds=t1 - t0, info=info, kernels=kt, w=w)
+
+
+def cpu_baseline(wl, params, budget_rows):
+    """Reference Fortran solver (oracle/_ref) on a bounded sample of the
+    workload: the full row width and velocity grid, budget_rows rows,
+    timed loop of a few steps.  Falls back to the C restatement."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness2d")
+    d = WORKLOADS[wl]
+    rows = min(budget_rows, d["ny"])
+    steps = 1
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as td:
+            nml = os.path.join(td, "bench.namelist")
+            with open(nml, "w") as f:
+                f.write("&params\n")
+                for k, v in params.items():
+                    f.write(f"  {k} = {v!r}\n" if isinstance(v, str) else f"  {k} = {v}\n")
+                f.write("/\n")
+            env = dict(os.environ)
+            r = subprocess.run([exe, nml, "-", str(steps), "1" if VARIANT[wl] == "cartesian" else "0",
+                                "1", str(rows)], capture_output=True, text=True, cwd=td, env=env,
+                               timeout=600)
+            line = [ln for ln in r.stdout.splitlines() if "HARNESS_TIMING" in ln]
+            if r.returncode == 0 and line:
+                loop_s = float(line[0].split("loop_s=")[1].split()[0])
+                n = d["nx"] * rows * d["nv"] ** 2 * steps
+                return {"value": n / loop_s, "unit": "phase-space updates/s", "cores": 8,
+                        "kind": "reference",
+                        "sample": f"{d['nx']}x{rows} rows of the workload (full nv={d['nv']}, "
+                                  f"Nm={d['Nm']}), {steps} step, reference Fortran built with "
+                                  f"amdflang -O2 -fopenmp; 8 OpenMP threads (hard-coded in the "
+                                  f"reference), update() serial; loop {loop_s:.1f} s"}
+    # C restatement (single thread)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    p = dict(params, ny=max(2, rows // 4))
+    case = cases.build_2d(p, capi.FP64, rt_variant=VARIANT[wl], seed=1)
+    o = oracle_py.Oracle(case)
+    t0 = time.perf_counter()
+    o.step(1)
+    dt = time.perf_counter() - t0
+    n = case.ncell * case.nn
+    return {"value": n / dt, "unit": "phase-space updates/s", "cores": 1, "kind": "port",
+            "sample": f"{p['nx']}x{p['ny']} cells of the workload, 1 step, C restatement"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=16)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
+              f"(multi-GPU slab decomposition is a design note, not an implementation)",
+              file=sys.stderr)
+        sys.exit(2)
+    wl = a.workload
+    params = WORKLOADS[wl]
+    mixed = cases.build_2d(params, capi.MIXED_FP32, rt_variant=VARIANT[wl], seed=1)
+    N = mixed.ncell * mixed.nn
+    rm = run_gpu(mixed, a.warmup, a.steps)
+    value = N * a.steps / rm["seconds"]
+    out = {}
+    if not a.no_fp64:
+        fp64 = mixed.with_precision(capi.FP64)
+        rf = run_gpu(fp64, a.warmup, a.steps)
+        from metrics import field_errors, prim_errors
+        gamma = mixed.desc.gamma
+        pe = prim_errors(rm["w"], rf["w"], gamma)
+        fe = field_errors(rm["w"], rf["w"])
+        area = mixed.desc.dx * mixed.desc.dy
+        w0 = np.zeros_like(rf["w"])
+        out["fp64"] = {
+            "value": N * a.steps / rf["seconds"], "ms_per_step": 1e3 * rf["seconds"] / a.steps,
+            "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in rf["kernels"].items()},
+        }
+        out["mixed_vs_fp64"] = {
+            "speedup": value / out["fp64"]["value"],
+            "rel_err": {"rho": pe[0], "U": pe[1], "V": pe[2], "T": pe[3]},
+            "rel_err_conserved": {"rho": fe[0], "rhoU": fe[1], "rhoV": fe[2], "E": fe[3]},
+            "mass_rel_diff": float(abs(rm["w"][:, 0].sum() - rf["w"][:, 0].sum()) /
+                                   rf["w"][:, 0].sum()),
+            "energy_rel_diff": float(abs(rm["w"][:, 3].sum() - rf["w"][:, 3].sum()) /
+                                     rf["w"][:, 3].sum()),
+            "steps_compared": a.warmup + a.steps,
+        }
+        del w0, area
+    # dominant kernel of the headline (mixed) path
+    kt = rm["kernels"]
+    dom = max(kt, key=lambda k: kt[k][0])
+    avg_ms = kt[dom][0] / kt[dom][1]
+    ab = algorithmic_bytes(mixed, dom)
+    roof = None
+    if ab is not None:
+        gbs = ab / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": ab}
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(wl, params, a.cpu_rows)
+        except Exception as e:  # reported, not fatal
+            cpu = {"error": str(e)}
+    ms = 1e3 * rm["seconds"] / a.steps
+    line = {
+        "metric": "phase-space updates/s (cells*nv^2), mixed FP32/FP64 path",
+        "value": value, "unit": "phase-space updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 storage/flux, f64 moments",
+        "data": "synthetic (seeded initial condition)",
+        "config": {"workload": NAMES[wl], "cells": [params["nx"], params["ny"]],
+                   "nv": params["nv"], "Nm": params["Nm"], "knudsen": params["knudsen"],
+                   "phase_space_points": N, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
+    }
+    line.update(out)
+    if cpu and "value" in cpu:
+        line["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

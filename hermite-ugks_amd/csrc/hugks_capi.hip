@@ -37,6 +37,13 @@ struct hugks_handle {
     std::string err;
     bool timing = false;
     std::vector<KernelTimer> timers;
+    struct Rec {
+        const char* name;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t open_ev = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t ncell = 0;
 };
@@ -69,46 +76,82 @@ static void release(hugks_handle* h) {
     h->allocs.clear();
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
+    for (auto& r : h->pending) {
+        hipEventDestroy(r.e0);
+        hipEventDestroy(r.e1);
+    }
+    h->pending.clear();
+    for (hipEvent_t e : h->pool) hipEventDestroy(e);
+    h->pool.clear();
     if (h->stream) hipStreamDestroy(h->stream);
     h->stream = nullptr;
 }
 
-// Launch helper: optional per-kernel timing with HIP events on the
-// handle's stream.
-template <typename F>
-static hipError_t timed(hugks_handle* h, const char* name, F&& launch) {
-    if (!h->timing) return launch();
-    hipEventRecord(h->ev0, h->stream);
-    hipError_t e = launch();
-    hipEventRecord(h->ev1, h->stream);
-    hipEventSynchronize(h->ev1);
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, h->ev0, h->ev1);
-    for (auto& t : h->timers)
-        if (t.name == name) {
-            t.ms += ms;
-            t.calls += 1;
-            return e;
-        }
-    h->timers.push_back({name, (double)ms, 1});
+// Per-kernel timing: every launch is bracketed by two HIP events recorded
+// on the handle's stream (no synchronisation); elapsed times are collected
+// when the host asks for them or when many records are pending.
+static void harvest(hugks_handle* h) {
+    if (h->pending.empty()) return;
+    hipStreamSynchronize(h->stream);
+    for (auto& r : h->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.e0, r.e1);
+        bool found = false;
+        for (auto& t : h->timers)
+            if (t.name == r.name) {
+                t.ms += ms;
+                t.calls += 1;
+                found = true;
+                break;
+            }
+        if (!found) h->timers.push_back({r.name, (double)ms, 1});
+        h->pool.push_back(r.e0);
+        h->pool.push_back(r.e1);
+    }
+    h->pending.clear();
+}
+
+static hipEvent_t pool_event(hugks_handle* h) {
+    if (h->pool.empty()) {
+        hipEvent_t e = nullptr;
+        hipEventCreate(&e);
+        return e;
+    }
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
     return e;
+}
+
+static void timing_hook(void* ctx, int phase, const char* name) {
+    hugks_handle* h = (hugks_handle*)ctx;
+    if (phase == 0) {
+        if (h->pending.size() >= 4096) harvest(h);
+        h->open_ev = pool_event(h);
+        hipEventRecord(h->open_ev, h->stream);
+    } else {
+        hipEvent_t e1 = pool_event(h);
+        hipEventRecord(e1, h->stream);
+        h->pending.push_back({name, h->open_ev, e1});
+    }
 }
 
 static hipError_t enqueue_step(hugks_handle* h) {
     hipError_t e;
-    const Launch2D& L = h->L;
-    e = timed(h, "timestep", [&] { return launch_timestep(L, h->dim, h->stream); });
+    Launch2D& L = h->L;
+    L.hook = h->timing ? timing_hook : nullptr;
+    L.hook_ctx = h;
+    e = launch_timestep(L, h->dim, h->stream);
     if (e != hipSuccess) return e;
     if (h->dim == 2) {
-        e = timed(h, "flux2d", [&] { return launch_flux2d(L, h->stream); });
+        e = launch_flux2d(L, h->stream);
         if (e != hipSuccess) return e;
-        e = timed(h, "update2d", [&] { return launch_update2d(L, h->stream); });
+        e = launch_update2d(L, h->stream);
         if (e != hipSuccess) return e;
     } else {
-        e = timed(h, "step1d", [&] { return launch_step1d(L, h->stream); });
+        e = launch_step1d(L, h->stream);
         if (e != hipSuccess) return e;
     }
-    return timed(h, "finalize", [&] { return launch_finalize(L, h->dim, h->stream); });
+    return launch_finalize(L, h->dim, h->stream);
 }
 
 static int read_state(hugks_handle* h, hugks_step_info* info) {
@@ -459,6 +502,49 @@ int hugks_set_f(hugks_handle* h, const double* hf, const double* bf, const doubl
     return HUGKS_OK;
 }
 
+int hugks_debug_fetch(hugks_handle* h, int32_t which, double* out) {
+    if (!h || !out) return fail(h, HUGKS_E_ARG, "null handle or output");
+    const Consts& c = h->L.c;
+    const bool two = h->dim == 2;
+    const size_t nvf = two ? (size_t)(c.nx + 1) * c.ny : (size_t)c.nx + 1;
+    const size_t nhf = two ? (size_t)c.nx * (c.ny + 1) : 0;
+    const void* src = nullptr;
+    size_t items = 0;
+    switch (which) {
+        case HUGKS_ARR_SHX: src = h->L.shx; items = h->ncell; break;
+        case HUGKS_ARR_SHY: src = two ? h->L.shy : nullptr; items = h->ncell; break;
+        case HUGKS_ARR_SBX: src = h->L.sbx; items = h->ncell; break;
+        case HUGKS_ARR_SBY: src = two ? h->L.sby : nullptr; items = h->ncell; break;
+        case HUGKS_ARR_VFH: src = h->L.vfh; items = nvf; break;
+        case HUGKS_ARR_VFB: src = h->L.vfb; items = nvf; break;
+        case HUGKS_ARR_HFH: src = two ? h->L.hfh : nullptr; items = nhf; break;
+        case HUGKS_ARR_HFB: src = two ? h->L.hfb : nullptr; items = nhf; break;
+        case HUGKS_ARR_VFLUX: {
+            hipError_t e = hipMemcpyAsync(out, h->L.vflux, nvf * 4 * sizeof(double),
+                                          hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            return e == hipSuccess ? HUGKS_OK : hip_fail(h, e, "debug_fetch");
+        }
+        case HUGKS_ARR_HFLUX: {
+            if (!two) return fail(h, HUGKS_E_ARG, "no horizontal faces in 1D");
+            hipError_t e = hipMemcpyAsync(out, h->L.hflux, nhf * 4 * sizeof(double),
+                                          hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            return e == hipSuccess ? HUGKS_OK : hip_fail(h, e, "debug_fetch");
+        }
+        default: return fail(h, HUGKS_E_ARG, "unknown array id");
+    }
+    if (!src) return fail(h, HUGKS_E_ARG, "array does not exist for this dimension");
+    const size_t n = items * c.nn;
+    double* tmp = nullptr;
+    hipError_t e = hipMallocAsync((void**)&tmp, n * sizeof(double), h->stream);
+    if (e == hipSuccess) e = launch_convert(src, h->fp64, tmp, items, c.nn, c.ldv, true, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (tmp) hipFreeAsync(tmp, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? HUGKS_OK : hip_fail(h, e, "debug_fetch");
+}
+
 int hugks_get_time(hugks_handle* h, double* simtime, int32_t* iter) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
     DevState s;
@@ -473,6 +559,7 @@ int hugks_get_time(hugks_handle* h, double* simtime, int32_t* iter) {
 int hugks_kernel_times(hugks_handle* h, int32_t cap, const char** names, double* ms,
                        int32_t* calls, int32_t* count) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    harvest(h);
     int n = 0;
     for (auto& t : h->timers) {
         if (n < cap) {
@@ -488,6 +575,7 @@ int hugks_kernel_times(hugks_handle* h, int32_t cap, const char** names, double*
 
 int hugks_reset_kernel_times(hugks_handle* h) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    harvest(h);
     h->timers.clear();
     return HUGKS_OK;
 }

@@ -6,6 +6,10 @@
 
 namespace hugks {
 
+// Optional per-launch hook (timing): called with phase 0 right before and
+// phase 1 right after each kernel launch on the launch stream.
+typedef void (*LaunchHook)(void* ctx, int phase, const char* name);
+
 // Everything one 2D/1D step touches on the device (pointers are device
 // pointers; T-typed arrays are void* and interpreted by precision).
 struct Launch2D {
@@ -24,6 +28,20 @@ struct Launch2D {
     unsigned nwall;      // number of wall faces (non-periodic sides)
     int update_nt;       // threads per update workgroup
     size_t update_lds;   // dynamic LDS bytes of the update kernel
+    LaunchHook hook;     // may be null
+    void* hook_ctx;
+};
+
+// Brackets one launch with the hook.
+struct HookScope {
+    const Launch2D& L;
+    const char* name;
+    HookScope(const Launch2D& l, const char* n) : L(l), name(n) {
+        if (L.hook) L.hook(L.hook_ctx, 0, name);
+    }
+    ~HookScope() {
+        if (L.hook) L.hook(L.hook_ctx, 1, name);
+    }
 };
 
 hipError_t launch_begin(DevState* st, hipStream_t s);

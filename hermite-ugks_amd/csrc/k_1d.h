@@ -3,6 +3,7 @@
 // or cell (1D velocity grids are short: nv <= 64*EPT).
 // References: 1d/src/solver.f90:16-264, 1d/src/flux.f90:13-360,
 // 1d/src/gravity.f90:18-123.
+#pragma once
 #include "hugks_device.h"
 #include "hugks_kernels.h"
 
@@ -450,42 +451,6 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
         if (!(pn[2] > 0.0) || !(pn[0] > 0.0) || !isfinite(pn[0]) || !isfinite(pn[1]) || !isfinite(pn[2]))
             atomicOr(&st->flags, 1);
     }
-}
-
-template <typename T>
-static hipError_t step1d_t(const Launch2D& L, hipStream_t s) {
-    const Consts& c = L.c;
-    using A = typename Arith<T>::A;
-    const size_t n = (size_t)c.nx * c.ldv;
-    k_slope1d<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(c, L.st, (const T*)L.h, (const T*)L.b,
-                                                              (T*)L.shx, (T*)L.sbx);
-    const int ept = (c.ldv + 63) / 64;
-    const A* un = (const A*)L.unode;
-    const A* wn = (const A*)L.wnode;
-#define HUGKS_1D_CASE(E)                                                                       \
-    case E:                                                                                    \
-        k_wall1d<T, E><<<2, 64, 0, s>>>(c, L.st, (const T*)L.h, (const T*)L.b, (const T*)L.shx, \
-                                        (const T*)L.sbx, un, wn, (T*)L.vfh, (T*)L.vfb, L.vflux); \
-        if (c.nx > 1)                                                                          \
-            k_flux1d<T, E><<<c.nx - 1, 64, 0, s>>>(c, L.st, L.W, (const T*)L.h, (const T*)L.b,  \
-                                                   (const T*)L.shx, (const T*)L.sbx, un, wn,   \
-                                                   (T*)L.vfh, (T*)L.vfb, L.vflux);             \
-        k_update1d<T, E><<<c.nx, 64, 0, s>>>(c, L.st, L.W, L.part, (T*)L.h, (T*)L.b,            \
-                                             (const T*)L.vfh, (const T*)L.vfb, L.vflux, un, wn); \
-        break;
-    switch (ept) {
-        HUGKS_1D_CASE(1)
-        HUGKS_1D_CASE(2)
-        HUGKS_1D_CASE(3)
-        HUGKS_1D_CASE(4)
-        default: return hipErrorInvalidValue;
-    }
-#undef HUGKS_1D_CASE
-    return hipGetLastError();
-}
-
-hipError_t launch_step1d(const Launch2D& L, hipStream_t s) {
-    return L.fp64 ? step1d_t<double>(L, s) : step1d_t<float>(L, s);
 }
 
 }  // namespace hugks

@@ -1,0 +1,200 @@
+// 2D slope and wall-face flux kernels (gfx950); the interior-face flux
+// kernel is in k_flux_face.h.
+#pragma once
+#include "hugks_device.h"
+#include "hugks_kernels.h"
+
+namespace hugks {
+
+// ------------------------------------------------------------------ slopes
+// interp_inner (van Leer) / interp_boundary (one-sided), 2d/src/solver.f90:
+// 64-159.  Periodic directions use wrapped neighbours everywhere (the ghost
+// copy of 2d/src/boundary.f90:16-53); wall directions use one-sided slopes
+// in the two boundary cells.
+template <typename T>
+__global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __restrict__ st,
+                                                 const T* __restrict__ h, const T* __restrict__ b,
+                                                 T* __restrict__ shx, T* __restrict__ shy,
+                                                 T* __restrict__ sbx, T* __restrict__ sby) {
+    using A = typename Arith<T>::A;
+    if (st->stopped) return;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t ncell = (size_t)c.nx * c.ny;
+    if (idx >= ncell * c.ldv) return;
+    const int cell = (int)(idx / c.ldv);
+    const int k = (int)(idx - (size_t)cell * c.ldv);
+    const int i = cell % c.nx, j = cell / c.nx;
+    const A smv = (A)c.smv;
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+        const bool per = c.bc[2 * dir] == HUGKS_BC_PERIODIC;
+        const int n = dir == 0 ? c.nx : c.ny;
+        const int p = dir == 0 ? i : j;
+        const A len = (A)(dir == 0 ? c.dx : c.dy);
+        int cl, cr;
+        bool inner = true;
+        if (per) {
+            const int pl = (p - 1 + n) % n, pr = (p + 1) % n;
+            cl = dir == 0 ? pl + c.nx * j : i + c.nx * pl;
+            cr = dir == 0 ? pr + c.nx * j : i + c.nx * pr;
+        } else if (p == 0) {
+            cl = cell;
+            cr = dir == 0 ? cell + 1 : cell + c.nx;
+            inner = false;
+        } else if (p == n - 1) {
+            cl = dir == 0 ? cell - 1 : cell - c.nx;
+            cr = cell;
+            inner = false;
+        } else {
+            cl = dir == 0 ? cell - 1 : cell - c.nx;
+            cr = dir == 0 ? cell + 1 : cell + c.nx;
+        }
+        const A den = (A)0.5 * len + (A)0.5 * len;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const T* f = q == 0 ? h : b;
+            const A fl = (A)f[(size_t)cl * c.ldv + k], fr = (A)f[(size_t)cr * c.ldv + k];
+            A s;
+            if (inner) {
+                const A fn = (A)f[(size_t)cell * c.ldv + k];
+                const A sL = (fn - fl) / den, sR = (fr - fn) / den;
+                s = (copysign((A)1, sR) + copysign((A)1, sL)) * fabs(sR) * fabs(sL) /
+                    (fabs(sR) + fabs(sL) + smv);
+            } else {
+                s = (fr - fl) / den;
+            }
+            T* out = dir == 0 ? (q == 0 ? shx : sbx) : (q == 0 ? shy : sby);
+            out[(size_t)cell * c.ldv + k] = (T)s;
+        }
+    }
+}
+
+// -------------------------------------------------------------- wall faces
+// One wavefront per wall face (four per workgroup).  Faces are numbered over
+// the non-periodic sides in the order x-lo (ny), x-hi (ny), y-lo (nx),
+// y-hi (nx).  Diffuse isothermal wall: calc_flux_boundary,
+// 2d/src/flux.f90:474-549; specular wall: calc_flux_specular_vertical /
+// _horizon, 2d/src/flux.f90:551-689.  The incoming half-range values are
+// re-read in each pass rather than held across the reductions.
+template <typename T>
+__global__ __launch_bounds__(256) void k_flux_wall(
+    Consts c, DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
+    const typename Arith<T>::A* __restrict__ vnode, const typename Arith<T>::A* __restrict__ wnode,
+    T* __restrict__ vfh, T* __restrict__ vfb, double* __restrict__ vflux, T* __restrict__ hfh,
+    T* __restrict__ hfb, double* __restrict__ hflux, int nwall) {
+    using A = typename Arith<T>::A;
+    if (st->stopped) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    int f = blockIdx.x * 4 + (int)(threadIdx.x / WAVE);
+    if (f >= nwall) return;
+    int side = -1, idx = 0;
+    for (int s = 0; s < 4; ++s) {
+        if (c.bc[s] == HUGKS_BC_PERIODIC) continue;
+        const int n = s < 2 ? c.ny : c.nx;
+        if (f < n) { side = s; idx = f; break; }
+        f -= n;
+    }
+    if (side < 0) return;
+    const int dir = side / 2;           // 0: vertical face, 1: horizontal face
+    const int rot = (side & 1) ? -1 : 1;
+    int cell, face;
+    if (dir == 0) {
+        cell = ((side & 1) ? c.nx - 1 : 0) + c.nx * idx;
+        face = ((side & 1) ? c.nx : 0) + (c.nx + 1) * idx;
+    } else {
+        cell = idx + c.nx * ((side & 1) ? c.ny - 1 : 0);
+        face = idx + c.nx * ((side & 1) ? c.ny : 0);
+    }
+    const T* sN = dir == 0 ? shx : shy;
+    const T* bN = dir == 0 ? sbx : sby;
+    const double dt = st->dt;
+    const double len = dir == 0 ? c.dx : c.dy;
+    const double flen = dir == 0 ? c.dy : c.dx;
+    const A hrl = (A)rot * (A)0.5 * (A)len;
+    const bool diffuse = c.bc[side] == HUGKS_BC_DIFFUSE;
+    const int nv = c.nv, nn = c.nn;
+    const size_t base = (size_t)cell * c.ldv;
+    const double lamw = 1.0 / c.tw[side];
+
+    // incoming value at node k (cell value extrapolated to the face,
+    // flux.f90:509-510), and for a specular wall its mirror node
+    // (flux.f90:589-596 / 659-666)
+    auto node = [&](int k, A& vn, A& vt, A& wt, int& d, A& hf, A& bf) {
+        const A u = unode[k], v = vnode[k];
+        wt = wnode[k];
+        vn = dir == 0 ? u : v;
+        vt = dir == 0 ? v : -u;
+        d = (copysign((A)1, vn) * (A)rot) > (A)0 ? 1 : 0;
+        int ks = k;
+        if (!diffuse && d) {
+            const int iu = k % nv, jv = k / nv;
+            ks = dir == 0 ? (nv - 1 - iu) + nv * jv : iu + nv * (nv - 1 - jv);
+        }
+        hf = (A)h[base + ks] - hrl * (A)sN[base + ks];
+        bf = (A)b[base + ks] - hrl * (A)bN[base + ks];
+    };
+
+    double rhow = 0.0;
+    if (diffuse) {
+        double r[2] = {0.0, 0.0};
+        for (int k = lane; k < nn; k += WAVE) {
+            A vn, vt, wt, hf, bf;
+            int d;
+            node(k, vn, vt, wt, d, hf, bf);
+            if (!d) r[0] += (double)(wt * vn * hf);
+            else r[1] += (double)(wt * vn) * exp(-lamw * ((double)vn * vn + (double)vt * vt));
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            double x = r[i];
+#pragma unroll
+            for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+            r[i] = __shfl(x, 0, WAVE);
+        }
+        const double R2 = (lamw / c.pi) * r[1];
+        if (R2 == 0.0 && lane == 0) atomicOr(&st->flags, 2);
+        rhow = -r[0] / R2;
+    }
+    const A Hc = (A)(rhow * (lamw / c.pi)), kf = (A)(c.K / (2.0 * lamw));
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    T* oh = (dir == 0 ? vfh : hfh) + (size_t)face * c.ldv;
+    T* ob = (dir == 0 ? vfb : hfb) + (size_t)face * c.ldv;
+    const A dtl = (A)(dt * flen);
+    for (int k = lane; k < nn; k += WAVE) {
+        A vn, vt, wt, hf, bf;
+        int d;
+        node(k, vn, vt, wt, d, hf, bf);
+        if (diffuse && d) {
+            hf = Hc * fexp(-(A)lamw * (vn * vn + vt * vt));
+            bf = hf * kf;
+        }
+        const A wvh = wt * vn * hf;
+        s[0] += (double)wvh;
+        s[1] += (double)(vn * wvh);
+        s[2] += (double)(vt * wvh);
+        s[3] += (double)(wt * vn * ((vn * vn + vt * vt) * hf + bf));
+        oh[k] = (T)(dtl * (vn * hf));
+        ob[k] = (T)(dtl * (vn * bf));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double x = s[i];
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+        s[i] = __shfl(x, 0, WAVE);
+    }
+    if (lane == 0) {
+        double g1 = s[1], g2 = s[2];
+        if (dir == 1) { g1 = -s[2]; g2 = s[1]; }
+        double* out = (dir == 0 ? vflux : hflux) + 4 * (size_t)face;
+        const double sc = dt * flen;
+        out[0] = sc * s[0];
+        out[1] = sc * g1;
+        out[2] = sc * g2;
+        out[3] = sc * (0.5 * s[3]);
+    }
+}
+
+}  // namespace hugks

@@ -1,5 +1,6 @@
 // Time-step size, step finalisation (residual, loop state, exit tests),
 // initialisation of f, output moments and layout conversion (gfx950).
+#pragma once
 #include "hugks_device.h"
 #include "hugks_kernels.h"
 
@@ -248,57 +249,6 @@ __global__ void k_from_dense(const double* __restrict__ src, T* __restrict__ dst
     const size_t cell = idx / ldv;
     const int k = (int)(idx - cell * ldv);
     dst[idx] = k < nn ? (T)src[cell * nn + k] : (T)0;
-}
-
-hipError_t launch_finalize(const Launch2D& L, int dim, hipStream_t s) {
-    k_finalize<<<1, RED_NT, 0, s>>>(L.c, dim, L.st, L.part);
-    return hipGetLastError();
-}
-
-hipError_t launch_begin(DevState* st, hipStream_t s) {
-    k_begin<<<1, 64, 0, s>>>(st);
-    return hipGetLastError();
-}
-
-hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s) {
-    k_timestep<<<1, RED_NT, 0, s>>>(L.c, dim, L.st, L.W);
-    return hipGetLastError();
-}
-
-hipError_t launch_init_f(const Launch2D& L, int dim, const double* prim0, hipStream_t s) {
-    const size_t n = (size_t)L.c.nx * L.c.ny * L.c.ldv;
-    k_init_f<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L.c, dim, prim0, L.u1, L.v1, L.W, L.h,
-                                                         L.b, L.fp64 ? 1 : 0);
-    return hipGetLastError();
-}
-
-hipError_t launch_macro(const Launch2D& L, int dim, double* q, hipStream_t s) {
-    const unsigned ncell = (unsigned)(L.c.nx * L.c.ny);
-    if (L.fp64)
-        k_macro<double><<<ncell, 256, 0, s>>>(L.c, dim, L.W, (const double*)L.h,
-                                              (const double*)L.b, L.u1, L.v1, L.wnode_d, q);
-    else
-        k_macro<float><<<ncell, 256, 0, s>>>(L.c, dim, L.W, (const float*)L.h, (const float*)L.b,
-                                             L.u1, L.v1, L.wnode_d, q);
-    return hipGetLastError();
-}
-
-hipError_t launch_convert(const void* src, bool src_fp64, double* dst, size_t ncell, int nn,
-                          int ldv, bool, hipStream_t s) {
-    const size_t n = ncell * nn;
-    const unsigned g = (unsigned)((n + 255) / 256);
-    if (src_fp64) k_to_dense<double><<<g, 256, 0, s>>>((const double*)src, dst, ncell, nn, ldv);
-    else k_to_dense<float><<<g, 256, 0, s>>>((const float*)src, dst, ncell, nn, ldv);
-    return hipGetLastError();
-}
-
-hipError_t launch_convert_in(const double* src, void* dst, bool dst_fp64, size_t ncell, int nn,
-                             int ldv, hipStream_t s) {
-    const size_t n = ncell * ldv;
-    const unsigned g = (unsigned)((n + 255) / 256);
-    if (dst_fp64) k_from_dense<double><<<g, 256, 0, s>>>(src, (double*)dst, ncell, nn, ldv);
-    else k_from_dense<float><<<g, 256, 0, s>>>(src, (float*)dst, ncell, nn, ldv);
-    return hipGetLastError();
 }
 
 }  // namespace hugks

@@ -12,6 +12,7 @@
 //   F(iu,jv) = -(1/sqrt(rt)) omega(iu,jv) sum_{r=0}^{Nm+1} Hx(r,iu) R(r,jv)
 // which is the reference's triangular double sum regrouped; it costs
 // O((Nm+1) nv^2) per cell instead of O((Nm+1)^2 nv^2 / 2).
+#pragma once
 #include "hugks_device.h"
 #include "hugks_kernels.h"
 
@@ -20,12 +21,6 @@ namespace hugks {
 // Rc of the solver's gas constant, lambda = 0.5/(Rc T) (2d/src/data2d.f90:54)
 constexpr double RC = 0.5;
 
-size_t update2d_lds_bytes(const Consts& c, bool fp64) {
-    const size_t nv = c.nv, nm = c.nm, nn = c.nn;
-    const size_t n = 2 * nn + 2 * (nm + 2) * nv + 2 * (nm + 1) * nv + 2 * (nm + 1) * (nm + 1) +
-                     2 * (nm + 2) * nv;
-    return n * (fp64 ? sizeof(double) : sizeof(float));
-}
 
 template <typename T, int EPT, int NT>
 __global__ __launch_bounds__(NT) void k_update2d(
@@ -37,7 +32,7 @@ __global__ __launch_bounds__(NT) void k_update2d(
     const double* __restrict__ u1, const double* __restrict__ v1,
     const double* __restrict__ acc_cell) {
     using A = typename Arith<T>::A;
-    extern __shared__ double smem_raw[];
+    HIP_DYNAMIC_SHARED(double, smem_raw)
     __shared__ double red[4 * (NT / WAVE)];
     if (st->stopped) return;
     const int tid = threadIdx.x;
@@ -240,41 +235,6 @@ __global__ __launch_bounds__(NT) void k_update2d(
             !isfinite(pn[3]) || !(pn[0] > 0.0))
             atomicOr(&st->flags, 1);
     }
-}
-
-template <typename T>
-static hipError_t launch_update_t(const Launch2D& L, hipStream_t s) {
-    const Consts& c = L.c;
-    constexpr int NT = 256;
-    const int ept = (c.ldv + NT - 1) / NT;
-    const unsigned ncell = (unsigned)(c.nx * c.ny);
-    using A = typename Arith<T>::A;
-#define HUGKS_UPD_CASE(E)                                                                       \
-    case E:                                                                                     \
-        (void)hipFuncSetAttribute((const void*)k_update2d<T, E, NT>,                            \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.update_lds);     \
-        k_update2d<T, E, NT><<<ncell, NT, L.update_lds, s>>>(                                   \
-            c, L.st, L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,           \
-            (const T*)L.hfh, (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.unode,              \
-            (const A*)L.vnode, (const A*)L.wnode, L.u1, L.v1, L.acc_cell);                      \
-        break;
-    switch (ept) {
-        HUGKS_UPD_CASE(1)
-        HUGKS_UPD_CASE(2)
-        HUGKS_UPD_CASE(3)
-        HUGKS_UPD_CASE(4)
-        HUGKS_UPD_CASE(5)
-        HUGKS_UPD_CASE(6)
-        HUGKS_UPD_CASE(7)
-        HUGKS_UPD_CASE(8)
-        default: return hipErrorInvalidValue;
-    }
-#undef HUGKS_UPD_CASE
-    return hipGetLastError();
-}
-
-hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
-    return L.fp64 ? launch_update_t<double>(L, s) : launch_update_t<float>(L, s);
 }
 
 }  // namespace hugks

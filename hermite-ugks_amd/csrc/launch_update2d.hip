@@ -1,0 +1,50 @@
+// Host launchers for the kernels of k_update2d.h.
+#include "k_update2d.h"
+
+namespace hugks {
+
+size_t update2d_lds_bytes(const Consts& c, bool fp64) {
+    const size_t nv = c.nv, nm = c.nm, nn = c.nn;
+    const size_t n = 2 * nn + 2 * (nm + 2) * nv + 2 * (nm + 1) * nv + 2 * (nm + 1) * (nm + 1) +
+                     2 * (nm + 2) * nv;
+    return n * (fp64 ? sizeof(double) : sizeof(float));
+}
+
+template <typename T>
+static hipError_t launch_update_t(const Launch2D& L, hipStream_t s) {
+    const Consts& c = L.c;
+    constexpr int NT = 256;
+    const int ept = (c.ldv + NT - 1) / NT;
+    const unsigned ncell = (unsigned)(c.nx * c.ny);
+    using A = typename Arith<T>::A;
+#define HUGKS_UPD_CASE(E)                                                                       \
+    case E: {                                                                                   \
+        HookScope hs(L, "update2d");                                                            \
+        (void)hipFuncSetAttribute((const void*)k_update2d<T, E, NT>,                            \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.update_lds);     \
+        hipLaunchKernelGGL((k_update2d<T, E, NT>), dim3(ncell), dim3(NT), L.update_lds, s, \
+            c, L.st, L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,           \
+            (const T*)L.hfh, (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.unode,              \
+            (const A*)L.vnode, (const A*)L.wnode, L.u1, L.v1, L.acc_cell);                      \
+        break;                                                                                  \
+    }
+    switch (ept) {
+        HUGKS_UPD_CASE(1)
+        HUGKS_UPD_CASE(2)
+        HUGKS_UPD_CASE(3)
+        HUGKS_UPD_CASE(4)
+        HUGKS_UPD_CASE(5)
+        HUGKS_UPD_CASE(6)
+        HUGKS_UPD_CASE(7)
+        HUGKS_UPD_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef HUGKS_UPD_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
+    return L.fp64 ? launch_update_t<double>(L, s) : launch_update_t<float>(L, s);
+}
+
+}  // namespace hugks

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(os.path.dirname(HERE), "libhugks.so")
+LIBPATH = os.environ.get("HUGKS_LIB") or os.path.join(os.path.dirname(HERE), "libhugks.so")
 
 ABI_VERSION = 1
 OK = 0
@@ -60,8 +60,9 @@ class StepInfo(C.Structure):
 EXPORTS = [
     "hugks_create", "hugks_destroy", "hugks_last_error", "hugks_step", "hugks_advance",
     "hugks_get_macro", "hugks_get_f", "hugks_set_f", "hugks_get_time", "hugks_kernel_times",
-    "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info",
+    "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
 ]
+ARR_SHX, ARR_SHY, ARR_SBX, ARR_SBY, ARR_VFH, ARR_VFB, ARR_HFH, ARR_HFB, ARR_VFLUX, ARR_HFLUX = range(10)
 
 _lib = None
 
@@ -93,6 +94,7 @@ def load(path=LIBPATH):
     lib.hugks_reset_kernel_times.argtypes = [H]
     lib.hugks_set_timing.argtypes = [H, C.c_int32]
     lib.hugks_device_info.restype = C.c_char_p
+    lib.hugks_debug_fetch.argtypes = [H, C.c_int32, _dp]
     _lib = lib
     return lib
 
@@ -205,6 +207,21 @@ class Solver:
         bf = np.ascontiguousarray(bf, dtype=np.float64)
         w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
         self._check(self.lib.hugks_set_f(self.h, _ptr(hf), _ptr(bf), _ptr(w)))
+
+    def debug_fetch(self, which):
+        """Intermediate array of the last step (ARR_* ids), dense FP64."""
+        d = self.case.desc
+        two = d.dim == 2
+        nvf = (d.nx + 1) * d.ny if two else d.nx + 1
+        nhf = d.nx * (d.ny + 1) if two else 0
+        nn = self.case.nn
+        rows = {ARR_SHX: (self.case.ncell, nn), ARR_SHY: (self.case.ncell, nn),
+                ARR_SBX: (self.case.ncell, nn), ARR_SBY: (self.case.ncell, nn),
+                ARR_VFH: (nvf, nn), ARR_VFB: (nvf, nn), ARR_HFH: (nhf, nn), ARR_HFB: (nhf, nn),
+                ARR_VFLUX: (nvf, 4), ARR_HFLUX: (nhf, 4)}[which]
+        out = np.empty(rows)
+        self._check(self.lib.hugks_debug_fetch(self.h, int(which), _ptr(out)))
+        return out
 
     def time(self):
         t = C.c_double()

@@ -172,6 +172,24 @@ int hugks_get_f(hugks_handle *h, double *hf, double *bf);
 int hugks_set_f(hugks_handle *h, const double *hf, const double *bf,
                 const double *w);
 
+/* Intermediate arrays of the last step, for tests and diagnostics (the
+ * reference's tests read the module globals ctr%sh, vface%flux_h, ...
+ * directly, e.g. 2d/tests/test_force_and_ghosts.f90:57-58).  Written to
+ * out as FP64, dense: slopes [ncell*nn], face distribution fluxes
+ * [nfaces*nn] (vertical faces (nx+1)*ny, horizontal nx*(ny+1)), face
+ * conserved fluxes [nfaces*4]. */
+#define HUGKS_ARR_SHX    0   /* x slope of h (1D: the slope)               */
+#define HUGKS_ARR_SHY    1
+#define HUGKS_ARR_SBX    2
+#define HUGKS_ARR_SBY    3
+#define HUGKS_ARR_VFH    4   /* vertical-face flux of h (1D: face flux)    */
+#define HUGKS_ARR_VFB    5
+#define HUGKS_ARR_HFH    6
+#define HUGKS_ARR_HFB    7
+#define HUGKS_ARR_VFLUX  8   /* vertical-face conserved flux (1D: face)   */
+#define HUGKS_ARR_HFLUX  9
+int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
+
 /* Loop state (simtime, iter) of the handle. */
 int hugks_get_time(hugks_handle *h, double *simtime, int32_t *iter);
 

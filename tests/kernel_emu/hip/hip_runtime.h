@@ -1,0 +1,195 @@
+// CPU emulation of the subset of the HIP runtime and device intrinsics the
+// library uses.  TEST INFRASTRUCTURE ONLY: it lets the unmodified kernel
+// and launcher sources (hermite-ugks_amd/csrc) be compiled by g++ and run
+// on the host, one workgroup at a time with one fiber per work-item,
+// under AddressSanitizer.  __syncthreads is a barrier over the workgroup,
+// shuffles exchange through a workgroup buffer, __shared__ arrays are
+// static (one workgroup runs at a time), dynamic LDS is a heap block of the
+// exact launch size, and fresh allocations are filled with NaN bytes.
+#pragma once
+#include <math.h>
+
+#include <ucontext.h>
+
+#include <tuple>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+#define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(::emu::dyn_smem);
+
+struct dim3 {
+    unsigned x, y, z;
+    constexpr dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+namespace emu {
+// Work-items of one workgroup are cooperative fibers on one OS thread; a
+// barrier switches back to the scheduler, which resumes every live fiber
+// in turn (a work-item that returned no longer takes part, as an exited
+// wave no longer takes part in s_barrier).
+struct Fiber {
+    ucontext_t ctx;
+    char* stack;
+    size_t size;
+    bool done;
+    dim3 t;
+};
+extern dim3 tid, bid, bdim;
+extern unsigned char* dyn_smem;
+extern unsigned char xbuf[1024 * 8];
+extern long launches;
+extern ucontext_t sched_ctx;
+extern Fiber* cur;
+extern void (*body)(void*);
+extern void* body_arg;
+void yield_to_sched();
+void fiber_entry();
+void run_block(Fiber* fibers, unsigned nt);
+
+template <class K, class... A>
+struct Call {
+    K kernel;
+    std::tuple<A...> args;
+    static void run(void* p) {
+        Call* c = (Call*)p;
+        std::apply(c->kernel, c->args);
+    }
+};
+
+template <class K, class... A>
+void launch(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
+    const unsigned nt = b.x * b.y * b.z;
+    const unsigned long nblk = (unsigned long)g.x * g.y * g.z;
+    ++launches;
+    Call<K, A...> call{kernel, std::make_tuple(args...)};
+    body = &Call<K, A...>::run;
+    body_arg = &call;
+    unsigned char* d = (unsigned char*)malloc(shmem ? shmem : 1);
+    dyn_smem = d;
+    bdim = b;
+    std::vector<Fiber> fibers(nt);
+    const size_t ss = 256 * 1024;
+    for (auto& f : fibers) {
+        f.size = ss;
+        f.stack = (char*)malloc(ss);
+    }
+    for (unsigned long blk = 0; blk < nblk; ++blk) {
+        memset(d, 0xff, shmem ? shmem : 1);
+        bid = dim3((unsigned)(blk % g.x), (unsigned)((blk / g.x) % g.y),
+                   (unsigned)(blk / ((unsigned long)g.x * g.y)));
+        for (unsigned t = 0; t < nt; ++t)
+            fibers[t].t = dim3(t % b.x, (t / b.x) % b.y, t / (b.x * b.y));
+        run_block(fibers.data(), nt);
+    }
+    for (auto& f : fibers) free(f.stack);
+    free(d);
+}
+}  // namespace emu
+
+#define threadIdx (::emu::tid)
+#define blockIdx (::emu::bid)
+#define blockDim (::emu::bdim)
+
+inline void __syncthreads() { emu::yield_to_sched(); }
+
+template <class T>
+inline T __shfl_xor(T v, int mask, int width = 64) {
+    static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+    const unsigned t = threadIdx.x;
+    std::memcpy(emu::xbuf + 8 * t, &v, sizeof(T));
+    __syncthreads();
+    const unsigned src = (t & ~(unsigned)(width - 1)) | ((t & (unsigned)(width - 1)) ^ (unsigned)mask);
+    T r;
+    std::memcpy(&r, emu::xbuf + 8 * src, sizeof(T));
+    __syncthreads();
+    return r;
+}
+template <class T>
+inline T __shfl(T v, int lane, int width = 64) {
+    const unsigned t = threadIdx.x;
+    std::memcpy(emu::xbuf + 8 * t, &v, sizeof(T));
+    __syncthreads();
+    const unsigned src = (t & ~(unsigned)(width - 1)) | ((unsigned)lane & (unsigned)(width - 1));
+    T r;
+    std::memcpy(&r, emu::xbuf + 8 * src, sizeof(T));
+    __syncthreads();
+    return r;
+}
+inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+
+// ---- runtime API
+typedef enum hipError_t {
+    hipSuccess = 0,
+    hipErrorInvalidValue = 1,
+    hipErrorOutOfMemory = 2,
+    hipErrorNoDevice = 100,
+} hipError_t;
+typedef struct emu_stream* hipStream_t;
+typedef struct emu_event* hipEvent_t;
+enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
+                     hipMemcpyDeviceToDevice, hipMemcpyDefault };
+enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
+#define hipStreamNonBlocking 1
+struct hipDeviceProp_t {
+    char name[256];
+    char gcnArchName[256];
+    int multiProcessorCount;
+    size_t maxSharedMemoryPerMultiProcessor;
+    size_t totalGlobalMem;
+    size_t sharedMemPerBlockOptin;
+};
+#define hipLaunchKernelGGL(kernel, g, b, sh, st, ...) ::emu::launch(kernel, g, b, sh, __VA_ARGS__)
+
+inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "success" : "emu error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
+    memset(p, 0, sizeof(*p));
+    strcpy(p->name, "CPU emulation");
+    strcpy(p->gcnArchName, "emu");
+    p->multiProcessorCount = 1;
+    p->maxSharedMemoryPerMultiProcessor = 160 * 1024;
+    p->totalGlobalMem = (size_t)1 << 34;
+    p->sharedMemPerBlockOptin = 160 * 1024;
+    return hipSuccess;
+}
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+    *s = (hipStream_t)malloc(1);
+    return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t s) { free(s); return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = (hipEvent_t)malloc(1); return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { free(e); return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+inline hipError_t hipMalloc(void** p, size_t n) {
+    *p = malloc(n);
+    if (!*p) return hipErrorOutOfMemory;
+    memset(*p, 0xff, n);
+    return hipSuccess;
+}
+inline hipError_t hipFree(void* p) { free(p); return hipSuccess; }
+inline hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) { return hipMalloc(p, n); }
+inline hipError_t hipFreeAsync(void* p, hipStream_t) { free(p); return hipSuccess; }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+    memcpy(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
+inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
