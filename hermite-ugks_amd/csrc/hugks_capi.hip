@@ -216,6 +216,18 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         return fail(nullptr, HUGKS_E_ARG, "missing node, weight, prim0 or acceleration array");
     if (!(d->dx > 0) || (dim == 2 && !(d->dy > 0)))
         return fail(nullptr, HUGKS_E_ARG, "cell sizes must be positive");
+    if (d->tau_model != HUGKS_TAU_MU_OVER_RHO && d->tau_model != HUGKS_TAU_VHS)
+        return fail(nullptr, HUGKS_E_ARG, "unknown collision-time model");
+    if (dim == 2 ? (d->dt_rule != HUGKS_DT_2D_GENERAL && d->dt_rule != HUGKS_DT_2D_WALL_NORMAL)
+                 : d->dt_rule != HUGKS_DT_1D)
+        return fail(nullptr, HUGKS_E_ARG, "time-step rule does not match the dimension");
+    if (d->force_variant < HUGKS_FORCE_HERMITE_SYMMETRIC ||
+        d->force_variant > HUGKS_FORCE_HERMITE_LEGACY)
+        return fail(nullptr, HUGKS_E_ARG, "unknown Hermite force variant");
+    if (!(d->cfl > 0) || !(d->prandtl > 0) || !(d->muref > 0) || !(d->gamma > 1) ||
+        !(d->innerk >= 0) || !(d->pi > 0))
+        return fail(nullptr, HUGKS_E_ARG,
+                    "cfl, prandtl, muref, pi must be positive, gamma > 1, innerk >= 0");
     for (int s = 0; s < (dim == 2 ? 4 : 2); ++s) {
         if (d->bc[s] < HUGKS_BC_PERIODIC || d->bc[s] > HUGKS_BC_DIFFUSE)
             return fail(nullptr, HUGKS_E_ARG, "unknown boundary kind");

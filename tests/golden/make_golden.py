@@ -1,0 +1,59 @@
+"""Writes the golden fixtures of tests/golden/*.npz by running the compiled
+reference solver (oracle/_ref, built from the reference sources by
+oracle/Makefile) on the namelists in tests/golden/cases/.
+
+Each fixture holds the inputs (the namelist text, step count, velocity
+grid) and the reference's outputs after that many steps: conserved
+variables w, distributions h and b, the dt of every step, simtime, the
+final iteration index and residual.
+
+Run in a container that has the reference sources:
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+import refdump  # noqa: E402
+
+# name: (namelist, dim, steps, cartesian RT, seed)
+FIXTURES = {
+    "ref2d_poiseuille_smoke": ("ref2d_poiseuille_smoke.namelist", 2, 10, 0, 0),
+    "ref2d_poiseuille_symb_smoke": ("ref2d_poiseuille_symb_smoke.namelist", 2, 10, 0, 0),
+    "ref2d_rt_smoke": ("ref2d_rt_smoke.namelist", 2, 10, 0, 0),
+    "p2d_poiseuille_4x8": ("p2d_poiseuille_4x8.namelist", 2, 25, 0, 0),
+    "p2d_rt_cartesian_6x10": ("p2d_rt_cartesian_6x10.namelist", 2, 15, 1, 5),
+    "ref1d_fourier_smoke": ("ref1d_fourier_smoke.namelist", 1, 40, 0, 0),
+    "p1d_sod_40": ("p1d_sod_40.namelist", 1, 60, 0, 0),
+    "p1d_hydrostatic_30": ("p1d_hydrostatic_30.namelist", 1, 40, 0, 0),
+}
+
+
+def run(name):
+    nml, dim, steps, cart, seed = FIXTURES[name]
+    path = os.path.join(HERE, "cases", nml)
+    exe = os.path.join(ROOT, "oracle", "_ref", f"ref_harness{dim}d")
+    with tempfile.TemporaryDirectory() as td:
+        dump = os.path.join(td, "dump.bin")
+        args = [exe, path, dump, str(steps)] + ([str(cart), str(seed)] if dim == 2 else [])
+        r = subprocess.run(args, capture_output=True, text=True, cwd=td, check=True)
+        d = refdump.read2d(dump) if dim == 2 else refdump.read1d(dump)
+    out = {k: np.asarray(v) for k, v in d.items()}
+    out["namelist"] = np.array(open(path).read())
+    out["dim"] = np.array(dim)
+    out["rt_cartesian"] = np.array(cart)
+    out["seed"] = np.array(seed)
+    out["requested_steps"] = np.array(steps)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    return r.stdout.strip().splitlines()[-2:]
+
+
+if __name__ == "__main__":
+    for n in (sys.argv[1:] or FIXTURES):
+        print(n, run(n))
