@@ -342,7 +342,9 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
                                                  const T* __restrict__ fh, const T* __restrict__ fb,
                                                  const double* __restrict__ flux,
                                                  const typename Arith<T>::A* __restrict__ unode,
-                                                 const typename Arith<T>::A* __restrict__ wnode) {
+                                                 const typename Arith<T>::A* __restrict__ wnode,
+                                                 const double* __restrict__ unode_d,
+                                                 const double* __restrict__ wnode_d) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
     const int cell = blockIdx.x, lane = threadIdx.x;
@@ -356,7 +358,11 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
     pn[1] = pn[1] + phi * dt;
     cons1(c, pn, wn);
     const double tau = c.muref * 2.0 * pow(pn[2], 1.0 - c.omega) / pn[0];
-    // heat flux at t^n and the Hermite coefficients of h^n, b^n
+    // heat flux at t^n and the Hermite coefficients of h^n, b^n.  The heat
+    // flux, the Maxwellian and the implicit update are FP64 whatever the
+    // storage type (an FP32 exponent would put a systematic relative error
+    // of order lambda c^2 * 2^-24 into the moments of the next step); the
+    // Hermite force uses the storage arithmetic.
     const double t = 1.0 / po[2];
     const A sc = (A)(sqrt(t)), s2 = (A)sqrt(2.0), Uo = (A)po[1];
     const int N = c.nm;
@@ -370,9 +376,10 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
             const size_t o = (size_t)cell * c.ldv + k;
             rh[e] = (A)h[o];
             rb[e] = (A)b[o];
-            const A u = unode[k], wt = wnode[k], cu = u - Uo;
-            q2[0] += (double)(wt * cu * (cu * cu) * rh[e]);
-            q2[1] += (double)(wt * cu * rb[e]);
+            const A u = unode[k];
+            const double wd = wnode_d[k], cd = unode_d[k] - po[1];
+            q2[0] += wd * cd * (cd * cd) * (double)rh[e];
+            q2[1] += wd * cd * (double)rb[e];
             vv[e] = (u - Uo) / sc * s2;
         }
     }
@@ -412,28 +419,27 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
             p1[e] = p2;
         }
     }
-    const A lamn = (A)pn[2], Un = (A)pn[1];
-    const A Hcn = (A)(pn[0] * pow(pn[2] / c.pi, 0.5)), kfn = (A)(c.K / (2.0 * pn[2]));
-    const A csn = (A)(c.shk * (1.0 - c.prandtl) * pn[2] * pn[2] / pn[0] * qf);
+    const double Hcn = pn[0] * pow(pn[2] / c.pi, 0.5), kfn = c.K / (2.0 * pn[2]);
+    const double csn = c.shk * (1.0 - c.prandtl) * pn[2] * pn[2] / pn[0] * qf;
     const A inv_sqrt_2pi = (A)(1.0 / sqrt(2.0 * c.pi));
-    const A dtA = (A)dt, itau = (A)(1.0 / tau), den = (A)(1.0 + dt / tau), phA = (A)phi;
-    const A idx = (A)(1.0 / c.dx);
+    const double itau = 1.0 / tau, den = 1.0 + dt / tau;
+    const double idx = 1.0 / c.dx;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = lane + 64 * e;
         if (k < c.nn) {
-            const A u = unode[k], cun = u - Un;
-            const A H = Hcn * fexp(-lamn * cun * cun), B = H * kfn;
-            const A l2 = (A)2 * lamn * cun * cun;
-            const A Hp = H + csn * cun * (l2 + (A)c.K - (A)5) * H;
-            const A Bp = B + csn * cun * (l2 + (A)c.K - (A)3) * B;
+            const double cun = unode_d[k] - pn[1];
+            const double H = Hcn * exp(-pn[2] * cun * cun), B = H * kfn;
+            const double l2 = 2.0 * pn[2] * cun * cun;
+            const double Hp = H + csn * cun * (l2 + c.K - 5.0) * H;
+            const double Bp = B + csn * cun * (l2 + c.K - 3.0) * B;
             const A wv = fexp(-vv[e] * vv[e] / (A)2) * inv_sqrt_2pi;
-            const A dvh = -rt2 * wv * Fh[e], dvb = -rt2 * wv * Fb[e];
-            const A dfh = ((A)fh[(size_t)cell * c.ldv + k] - (A)fh[(size_t)(cell + 1) * c.ldv + k]) * idx;
-            const A dfb = ((A)fb[(size_t)cell * c.ldv + k] - (A)fb[(size_t)(cell + 1) * c.ldv + k]) * idx;
+            const double dvh = (double)(-rt2 * wv * Fh[e]), dvb = (double)(-rt2 * wv * Fb[e]);
+            const double dfh = ((double)fh[(size_t)cell * c.ldv + k] - (double)fh[(size_t)(cell + 1) * c.ldv + k]) * idx;
+            const double dfb = ((double)fb[(size_t)cell * c.ldv + k] - (double)fb[(size_t)(cell + 1) * c.ldv + k]) * idx;
             const size_t o = (size_t)cell * c.ldv + k;
-            h[o] = (T)((rh[e] + dfh + dtA * (Hp * itau) - dvh * dtA * phA) / den);
-            b[o] = (T)((rb[e] + dfb + dtA * (Bp * itau) - dvb * dtA * phA) / den);
+            h[o] = (T)(((double)rh[e] + dfh + dt * (Hp * itau) - dvh * dt * phi) / den);
+            b[o] = (T)(((double)rb[e] + dfb + dt * (Bp * itau) - dvb * dt * phi) / den);
         }
     }
     if (lane == 0) {

@@ -29,8 +29,8 @@ __global__ __launch_bounds__(NT) void k_update2d(
     const T* __restrict__ hfh, const T* __restrict__ hfb, const double* __restrict__ vflux,
     const double* __restrict__ hflux, const typename Arith<T>::A* __restrict__ unode,
     const typename Arith<T>::A* __restrict__ vnode, const typename Arith<T>::A* __restrict__ wnode,
-    const double* __restrict__ u1, const double* __restrict__ v1,
-    const double* __restrict__ acc_cell) {
+    const double* __restrict__ wnode_d, const double* __restrict__ u1,
+    const double* __restrict__ v1, const double* __restrict__ acc_cell) {
     using A = typename Arith<T>::A;
     HIP_DYNAMIC_SHARED(double, smem_raw)
     __shared__ double red[4 * (NT / WAVE)];
@@ -69,29 +69,35 @@ __global__ __launch_bounds__(NT) void k_update2d(
     A* D = Tm + 2 * (nm + 1) * nv;                // [2][nm+1][nm+1]
     A* R = D + 2 * (nm + 1) * (nm + 1);           // [2][nm+2][nv]
 
-    // ---- load f^n, weighted copies, heat flux at t^n (solver.f90:337)
+    // ---- load f^n, weighted copies, heat flux at t^n (solver.f90:337).
+    // The heat flux and, below, the Maxwellians and the implicit update are
+    // evaluated in FP64 whatever the storage type: the new f is close to
+    // the Maxwellian of the new state whenever dt >> tau, and an FP32
+    // exponent would carry a systematic relative error of order
+    // lambda c^2 * 2^-24 into every moment of the next step; the Hermite
+    // force (scaled by dt * a) is evaluated in the storage arithmetic.
     const double rt = RC / po[3];
     const double sqrt_rt = sqrt(rt);
-    A rh[EPT], rb[EPT];
+    T rh[EPT], rb[EPT];
     double q4[4] = {0.0, 0.0, 0.0, 0.0};
-    const A Uo = (A)po[1], Vo = (A)po[2];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = tid + e * NT;
-        rh[e] = rb[e] = (A)0;
+        rh[e] = rb[e] = (T)0;
         if (k < nn) {
             const size_t o = (size_t)cell * c.ldv + k;
-            rh[e] = (A)h[o];
-            rb[e] = (A)b[o];
-            const A wt = wnode[k];
-            const A cu = unode[k] - Uo, cv = vnode[k] - Vo;
-            const A c2 = cu * cu + cv * cv;
-            q4[0] += (double)(wt * cu * c2 * rh[e]);
-            q4[1] += (double)(wt * cu * rb[e]);
-            q4[2] += (double)(wt * cv * c2 * rh[e]);
-            q4[3] += (double)(wt * cv * rb[e]);
-            G[k] = wt * rh[e];
-            G[nn + k] = wt * rb[e];
+            rh[e] = h[o];
+            rb[e] = b[o];
+            const double wt = wnode_d[k];
+            const double cu = u1[k % nv] - po[1], cv = v1[k / nv] - po[2];
+            const double c2 = cu * cu + cv * cv;
+            q4[0] += wt * cu * c2 * (double)rh[e];
+            q4[1] += wt * cu * (double)rb[e];
+            q4[2] += wt * cv * c2 * (double)rh[e];
+            q4[3] += wt * cv * (double)rb[e];
+            const A wa = wnode[k];
+            G[k] = wa * (A)rh[e];
+            G[nn + k] = wa * (A)rb[e];
         }
     }
     // Hermite bases He_0..He_{Nm+1} at xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt)
@@ -174,17 +180,14 @@ __global__ __launch_bounds__(NT) void k_update2d(
     // ---- force and implicit BGK update (solver.f90:337-388)
     const A inv_sqrt_rt = (A)(1.0 / sqrt_rt);
     const A two_pi = (A)(2.0 * c.pi);
-    const A lamo = (A)po[3], lamn = (A)pn[3];
-    const A Hco = (A)(po[0] * (po[3] / c.pi)), Hcn = (A)(pn[0] * (pn[3] / c.pi));
-    const A kfo = (A)(c.K / (2.0 * po[3])), kfn = (A)(c.K / (2.0 * pn[3]));
-    const A cso = (A)(c.shk * (1.0 - c.prandtl) * po[3] * po[3] / po[0]);
-    const A csn = (A)(c.shk * (1.0 - c.prandtl) * pn[3] * pn[3] / pn[0]);
-    const A Un = (A)pn[1], Vn = (A)pn[2];
-    const A q0 = (A)qf0, q1 = (A)qf1, K = (A)c.K;
-    const A dtA = (A)dt, half_dt = (A)(0.5 * dt);
-    const A itau = (A)(1.0 / tau), itau_o = (A)(1.0 / tau_old);
-    const A den = (A)(1.0 + 0.5 * dt / tau);
-    const A iarea = (A)(1.0 / area);
+    const double Hco = po[0] * (po[3] / c.pi), Hcn = pn[0] * (pn[3] / c.pi);
+    const double kfo = c.K / (2.0 * po[3]), kfn = c.K / (2.0 * pn[3]);
+    const double cso = c.shk * (1.0 - c.prandtl) * po[3] * po[3] / po[0];
+    const double csn = c.shk * (1.0 - c.prandtl) * pn[3] * pn[3] / pn[0];
+    const double half_dt = 0.5 * dt;
+    const double itau = 1.0 / tau, itau_o = 1.0 / tau_old;
+    const double den = 1.0 + 0.5 * dt / tau;
+    const double iarea = 1.0 / area;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = tid + e * NT;
@@ -198,27 +201,28 @@ __global__ __launch_bounds__(NT) void k_update2d(
                 sh += hx * R[r * nv + jv];
                 sb += hx * R[(m2 + r) * nv + jv];
             }
-            const A Fh = -inv_sqrt_rt * om * sh;
-            const A Fb = -inv_sqrt_rt * om * sb;
+            const double Fh = (double)(-inv_sqrt_rt * om * sh);
+            const double Fb = (double)(-inv_sqrt_rt * om * sb);
             // Maxwellians and Shakhov parts at t^n and t^{n+1}, both with the
             // heat flux of h^n (solver.f90:337-347)
-            const A u = unode[k], v = vnode[k];
-            const A cuo = u - Uo, cvo = v - Vo, c2o = cuo * cuo + cvo * cvo;
-            const A Ho = Hco * fexp(-lamo * c2o), Bo = Ho * kfo;
-            const A cqo = cso * (cuo * q0 + cvo * q1);
-            const A Hpo = Ho + cqo * ((A)2 * lamo * c2o + K - (A)5) * Ho;
-            const A Bpo = Bo + cqo * ((A)2 * lamo * c2o + K - (A)3) * Bo;
-            const A cun = u - Un, cvn = v - Vn, c2n = cun * cun + cvn * cvn;
-            const A Hn = Hcn * fexp(-lamn * c2n), Bn = Hn * kfn;
-            const A cqn = csn * (cun * q0 + cvn * q1);
-            const A Hpn = Hn + cqn * ((A)2 * lamn * c2n + K - (A)5) * Hn;
-            const A Bpn = Bn + cqn * ((A)2 * lamn * c2n + K - (A)3) * Bn;
-            const A dfh = ((A)vfh[(size_t)vl * c.ldv + k] - (A)vfh[(size_t)vr * c.ldv + k] +
-                           (A)hfh[(size_t)hd * c.ldv + k] - (A)hfh[(size_t)hu * c.ldv + k]) * iarea;
-            const A dfb = ((A)vfb[(size_t)vl * c.ldv + k] - (A)vfb[(size_t)vr * c.ldv + k] +
-                           (A)hfb[(size_t)hd * c.ldv + k] - (A)hfb[(size_t)hu * c.ldv + k]) * iarea;
-            const A hn = (rh[e] + dfh + half_dt * (Hpn * itau + (Hpo - rh[e]) * itau_o) - Fh * dtA) / den;
-            const A bn = (rb[e] + dfb + half_dt * (Bpn * itau + (Bpo - rb[e]) * itau_o) - Fb * dtA) / den;
+            const double u = u1[iu], v = v1[jv];
+            const double cuo = u - po[1], cvo = v - po[2], c2o = cuo * cuo + cvo * cvo;
+            const double Ho = Hco * exp(-po[3] * c2o), Bo = Ho * kfo;
+            const double cqo = cso * (cuo * qf0 + cvo * qf1);
+            const double Hpo = Ho + cqo * (2.0 * po[3] * c2o + c.K - 5.0) * Ho;
+            const double Bpo = Bo + cqo * (2.0 * po[3] * c2o + c.K - 3.0) * Bo;
+            const double cun = u - pn[1], cvn = v - pn[2], c2n = cun * cun + cvn * cvn;
+            const double Hn = Hcn * exp(-pn[3] * c2n), Bn = Hn * kfn;
+            const double cqn = csn * (cun * qf0 + cvn * qf1);
+            const double Hpn = Hn + cqn * (2.0 * pn[3] * c2n + c.K - 5.0) * Hn;
+            const double Bpn = Bn + cqn * (2.0 * pn[3] * c2n + c.K - 3.0) * Bn;
+            const double dfh = ((double)vfh[(size_t)vl * c.ldv + k] - (double)vfh[(size_t)vr * c.ldv + k] +
+                                (double)hfh[(size_t)hd * c.ldv + k] - (double)hfh[(size_t)hu * c.ldv + k]) * iarea;
+            const double dfb = ((double)vfb[(size_t)vl * c.ldv + k] - (double)vfb[(size_t)vr * c.ldv + k] +
+                                (double)hfb[(size_t)hd * c.ldv + k] - (double)hfb[(size_t)hu * c.ldv + k]) * iarea;
+            const double ho = (double)rh[e], bo = (double)rb[e];
+            const double hn = (ho + dfh + half_dt * (Hpn * itau + (Hpo - ho) * itau_o) - Fh * dt) / den;
+            const double bn = (bo + dfb + half_dt * (Bpn * itau + (Bpo - bo) * itau_o) - Fb * dt) / den;
             const size_t o = (size_t)cell * c.ldv + k;
             h[o] = (T)hn;
             b[o] = (T)bn;

@@ -23,7 +23,7 @@ static hipError_t step1d_t(const Launch2D& L, hipStream_t s) {
                                                    (const T*)L.shx, (const T*)L.sbx, un, wn,   \
                                                    (T*)L.vfh, (T*)L.vfb, L.vflux);             \
         hipLaunchKernelGGL((k_update1d<T, E>), dim3(c.nx), dim3(64), 0, s, c, L.st, L.W, L.part, (T*)L.h, (T*)L.b,            \
-                                             (const T*)L.vfh, (const T*)L.vfb, L.vflux, un, wn); \
+                                             (const T*)L.vfh, (const T*)L.vfb, L.vflux, un, wn, L.unode_d, L.wnode_d); \
         break;
     switch (ept) {
         HUGKS_1D_CASE(1)

@@ -25,7 +25,7 @@ static hipError_t launch_update_t(const Launch2D& L, hipStream_t s) {
         hipLaunchKernelGGL((k_update2d<T, E, NT>), dim3(ncell), dim3(NT), L.update_lds, s, \
             c, L.st, L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,           \
             (const T*)L.hfh, (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.unode,              \
-            (const A*)L.vnode, (const A*)L.wnode, L.u1, L.v1, L.acc_cell);                      \
+            (const A*)L.vnode, (const A*)L.wnode, L.wnode_d, L.u1, L.v1, L.acc_cell);          \
         break;                                                                                  \
     }
     switch (ept) {

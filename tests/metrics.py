@@ -5,7 +5,11 @@ divided by the field's scale: max |rho| for density, max |E| for energy
 and, for both momentum components, the largest momentum magnitude
 |(rho U, rho V)| (a component that is zero up to round-off, such as the
 wall-normal momentum of Poiseuille flow, is measured against the momentum
-the flow actually carries).
+the flow actually carries).  With thermal=True momentum is measured
+against the thermal momentum sqrt(max rho * max rho E) instead: for a
+state at hydrostatic rest rho U is the small difference of pressure and
+force terms, and a one-ulp perturbation of the initial f moves it by
+1e-10 of its own size within tens of steps.
 
 prim_errors(w, wr): per primitive field (rho, U, V, T) on the same idea:
 velocities against the largest speed, T against max T.
@@ -13,14 +17,16 @@ velocities against the largest speed, T against max T.
 import numpy as np
 
 
-def field_errors(w, wr):
+def field_errors(w, wr, thermal=False):
     w = np.asarray(w)
     wr = np.asarray(wr)
     d = w.shape[1] - 2
     err = np.empty(w.shape[1])
     err[0] = np.max(np.abs(w[:, 0] - wr[:, 0])) / np.max(np.abs(wr[:, 0]))
-    mom = np.sqrt(np.sum(wr[:, 1:1 + d] ** 2, axis=1))
-    ms = max(np.max(mom), 1e-300)
+    if thermal:
+        ms = np.sqrt(np.max(np.abs(wr[:, 0])) * np.max(np.abs(wr[:, -1])))
+    else:
+        ms = max(np.max(np.sqrt(np.sum(wr[:, 1:1 + d] ** 2, axis=1))), 1e-300)
     for q in range(1, 1 + d):
         err[q] = np.max(np.abs(w[:, q] - wr[:, q])) / ms
     err[-1] = np.max(np.abs(w[:, -1] - wr[:, -1])) / np.max(np.abs(wr[:, -1]))

@@ -1,0 +1,114 @@
+"""1D parity of the HIP library (through the C-ABI) with the FP64 oracle:
+the Sod tube under a body force (case 3), the hydrostatic column (case 2)
+and the Fourier channel between diffuse walls (case 4), FP64 at 1e-10 and
+mixed at 1e-5 on the measures of tests/metrics.py, plus the whole 1D Sod
+run of the reference's CPU configuration (nx=200, GH34, Nm=8, phi=1, to
+t=0.15: 577 steps, 1d/HERMITE_VALIDATION_REPORT.md)."""
+import os
+
+import numpy as np
+import pytest
+
+from hugks import capi, cases, namelist
+import oracle_py
+from metrics import field_errors
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cases")
+TOL_FP64 = 1e-10
+TOL_MIXED = 1e-5
+
+
+def sod_c1():
+    p = dict(namelist.DEFAULTS_1D)
+    p.update(nx=200, xlength=1.0, nv=34, Tref=1.0, quad_type="gauss_hermite", force_type="hermite",
+             Nm=8, cfl=0.5, maxtime=0.15, case_id=3, knudsen=0.01, phi=1.0)
+    return p
+
+
+def from_file(name):
+    return namelist.read(os.path.join(GOLDEN, name), 1)[0]
+
+
+CASES = [
+    ("sod40", from_file("p1d_sod_40.namelist"), 40, False),
+    ("hydrostatic30", from_file("p1d_hydrostatic_30.namelist"), 34, True),
+    ("fourier_smoke", from_file("ref1d_fourier_smoke.namelist"), 20, False),
+]
+
+
+def run_gpu(case, steps):
+    print(f"1d case nx={case.desc.nx} nv={case.desc.nv} nm={case.desc.nm} "
+          f"prec={case.desc.precision} steps={steps}", flush=True)
+    with capi.Solver(case) as s:
+        info = s.advance(steps)
+        w, q = s.macro()
+        hf, bf = s.get_f()
+    return info, w, q, hf, bf
+
+
+def run_oracle(case, steps):
+    o = oracle_py.Oracle(case.with_precision(capi.FP64))
+    info = None
+    for _ in range(steps):
+        info = o.step()
+        if info.stop_reason:      # the reference loop ends at maxtime
+            break
+    w, h, b = o.get()
+    return info, w, h, b, o.heat_flux(), o.time()
+
+
+@pytest.mark.parametrize("name,params,steps,thermal", CASES, ids=[c[0] for c in CASES])
+def test_fp64_parity_1d(have_gpu, name, params, steps, thermal):
+    case = cases.build_1d(params, capi.FP64)
+    info, w, q, hf, bf = run_gpu(case, steps)
+    oinfo, wr, hr, br, qr, (t, it, dt) = run_oracle(case, steps)
+    assert info.iter == it
+    assert abs(info.simtime - t) <= 1e-12 * t
+    err = field_errors(w, wr, thermal=thermal)
+    fe = np.max(np.abs(hf - hr)) / np.max(np.abs(hr))
+    print(name, "W err", err, "f err", fe, flush=True)
+    assert np.all(err <= TOL_FP64), err
+    assert fe <= TOL_FP64
+    # steady residual of 1d/src/solver.f90:202-203 is returned by every step
+    sres = np.array(info.steady_res[:3])
+    osres = np.array(oinfo.steady_res[:3])
+    assert np.all(np.abs(sres - osres) <= 1e-6 * np.max(np.abs(osres)) + 1e-300), (sres, osres)
+
+
+@pytest.mark.parametrize("name,params,steps,thermal", CASES, ids=[c[0] for c in CASES])
+def test_mixed_parity_1d(have_gpu, name, params, steps, thermal):
+    case = cases.build_1d(params, capi.MIXED_FP32)
+    info, w, q, hf, bf = run_gpu(case, steps)
+    _, wr, hr, br, qr, _ = run_oracle(case, steps)
+    err = field_errors(w, wr, thermal=thermal)
+    print(name, "mixed W err", err, flush=True)
+    assert np.all(err <= TOL_MIXED), err
+
+
+def test_sod_reference_cpu_configuration_full_run(have_gpu):
+    """Whole run to maxtime with the device-side exit: same step count and
+    final state as the oracle."""
+    case = cases.build_1d(sod_c1(), capi.FP64)
+    with capi.Solver(case) as s:
+        info = s.advance(0)
+        w, _ = s.macro()
+    assert info.stop_reason == capi.STOP_MAXTIME
+    o = oracle_py.Oracle(case)
+    n = 0
+    while True:
+        oi = o.step()
+        n += 1
+        if oi.stop_reason == capi.STOP_MAXTIME:
+            break
+        assert n < 2000
+    wr = o.get()[0]
+    t, it, _ = o.time()
+    print("C1 steps", n, "iter", info.iter, it, "simtime", info.simtime, t, flush=True)
+    assert n == 577 and it == 578
+    assert info.iter == it
+    assert abs(info.simtime - t) <= 1e-12
+    err = field_errors(w, wr)
+    print("C1 W err", err, flush=True)
+    assert np.all(err <= TOL_FP64), err
