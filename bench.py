@@ -2,6 +2,8 @@
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|c2]
                 [--no-cpu] [--no-fp64]
+python bench.py --copy-probe                  (calibration copy for counters)
+python bench.py --rocprof-summary DIR [DIR..] (summarise rocprofv3 CSV output)
 
 A step is one full reference time step (timestep, interpolation,
 evolution, update) over the whole phase space of the workload.  The
@@ -29,7 +31,10 @@ sys.path[:0] = [os.path.join(ROOT, "hermite-ugks_amd"), os.path.join(ROOT, "test
 
 from hugks import capi, cases  # noqa: E402
 
+METRIC = ("phase-space updates/s (cells\u00b7nv\u00b2) mixed vs FP64 at stated err; "
+          "% HBM roofline")
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+COPY_PROBE_BYTES = 1 << 30     # bytes read (and written) by one probe copy
 FP64_PEAK_TFLOPS = 78.6
 
 WORKLOADS = {
@@ -133,7 +138,80 @@ def cpu_baseline(wl, params, budget_rows):
             "sample": f"{p['nx']}x{p['ny']} cells of the workload, 1 step, C restatement"}
 
 
+def copy_probe():
+    """A plain device copy of COPY_PROBE_BYTES, three times: the counter
+    runs of rocprofv3 over this give the FETCH_SIZE / WRITE_SIZE reading of
+    a kernel whose HBM traffic is known, which calibrates the same counters
+    read over the solver's kernels."""
+    import torch
+    n = COPY_PROBE_BYTES // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3}))
+
+
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def rocprof_summary(dirs):
+    """Per-kernel summary of rocprofv3 CSV output directories (kernel
+    trace: calls and average duration; counter collection: average counter
+    value per dispatch).  Prints JSON."""
+    import glob
+    out = {}
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+            for r in _csv_rows(path):
+                k = r.get("Kernel_Name", "?")
+                e = out.setdefault(k, {})
+                dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
+                e["calls"] = e.get("calls", 0) + 1
+                e["total_ms"] = e.get("total_ms", 0.0) + dur
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            acc = {}
+            for r in _csv_rows(path):
+                key = (r.get("Kernel_Name", "?"), r["Counter_Name"])
+                v = acc.setdefault(key, {})
+                disp = r.get("Dispatch_Id", r.get("Correlation_Id", "0"))
+                v[disp] = v.get(disp, 0.0) + float(r["Counter_Value"])
+            for (k, cn), per in acc.items():
+                e = out.setdefault(k, {})
+                e[cn + "_avg"] = sum(per.values()) / len(per)
+                e[cn + "_dispatches"] = len(per)
+    for e in out.values():
+        if "calls" in e:
+            e["avg_ms"] = e["total_ms"] / e["calls"]
+    print(json.dumps(out, indent=1, sort_keys=True))
+
+
+def traffic_from_profiles(workload, kernel):
+    """HBM bytes per launch of kernel from the committed counter summary
+    profiles/traffic_<workload>.json (FETCH_SIZE + WRITE_SIZE, KiB per
+    dispatch, each scaled by its reading over a copy of known size);
+    None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{NAMES[workload]}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return k.get("hbm_bytes_per_launch")
+
+
 def main():
+    if "--copy-probe" in sys.argv:
+        return copy_probe()
+    if "--rocprof-summary" in sys.argv:
+        return rocprof_summary(sys.argv[sys.argv.index("--rocprof-summary") + 1:])
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -188,7 +266,8 @@ def main():
     if ab is not None:
         gbs = ab / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(wl, dom),
+                "kernel": dom,
                 "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": ab}
     cpu = None
     if not a.no_cpu:
@@ -198,7 +277,7 @@ def main():
             cpu = {"error": str(e)}
     ms = 1e3 * rm["seconds"] / a.steps
     line = {
-        "metric": "phase-space updates/s (cells*nv^2), mixed FP32/FP64 path",
+        "metric": METRIC,
         "value": value, "unit": "phase-space updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 storage/flux, f64 moments",

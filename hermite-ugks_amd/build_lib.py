@@ -4,7 +4,9 @@ Usage: python hermite-ugks_amd/build_lib.py [--jobs N]
 
 Every translation unit under csrc/ is compiled separately (in parallel)
 and linked into hermite-ugks_amd/libhugks.so.  Packed FP32 instructions
-are disabled for device code.
+are disabled for device code.  The Fortran drivers under driver/
+(ugks2d_gpu, ugks1d_gpu: the reference's main programs over the C-ABI)
+are then built with amdflang and linked against the library.
 """
 import argparse
 import concurrent.futures as cf
@@ -18,6 +20,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhugks.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+DRIVER = os.path.join(HERE, "driver")
 ARCH = os.environ.get("HUGKS_ARCH", "gfx950")
 
 FLAGS = [
@@ -64,7 +68,36 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    build_drivers()
     return LIB
+
+
+def build_drivers():
+    """ugks2d_gpu and ugks1d_gpu from driver/*.f90 (ISO_C_BINDING module
+    first), linked to libhugks.so with an rpath to this directory."""
+    if not os.path.exists(FC):
+        return []
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    common = ["hugks_capi.f90", "hugks_quadrature.f90"]
+    objs = []
+    for f in common:
+        o = os.path.join(mod, f[:-4] + ".o")
+        r = subprocess.run([FC, "-O2", "-J", mod, "-c", os.path.join(DRIVER, f), "-o", o],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang {f} failed:\n{r.stderr}")
+        objs.append(o)
+    out = []
+    for prog in ("ugks2d_gpu", "ugks1d_gpu"):
+        exe = os.path.join(DRIVER, prog)
+        r = subprocess.run([FC, "-O2", "-J", mod, os.path.join(DRIVER, prog + ".f90"), *objs,
+                            "-L", HERE, "-lhugks", "-Wl,-rpath," + HERE, "-o", exe],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang {prog} failed:\n{r.stderr}")
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

@@ -169,6 +169,7 @@ static int read_state(hugks_handle* h, hugks_step_info* info) {
         info->dt = s.dt;
         for (int q = 0; q < 4; ++q) info->res[q] = s.res[q];
         for (int q = 0; q < 3; ++q) info->steady_res[q] = s.steady[q];
+        info->mass_rel_change = s.mass_rel;
     }
     if (s.flags & 1) return fail(h, HUGKS_E_NUMERIC, "non-finite state or T <= 0 on the device");
     if (s.flags & 2) return fail(h, HUGKS_E_WALL, "zero outgoing wall mass coefficient");
@@ -224,6 +225,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     if (d->force_variant < HUGKS_FORCE_HERMITE_SYMMETRIC ||
         d->force_variant > HUGKS_FORCE_HERMITE_LEGACY)
         return fail(nullptr, HUGKS_E_ARG, "unknown Hermite force variant");
+    if (d->steady_steps < 0 || (d->steady_steps > 0 && (dim != 1 || !(d->steady_tol >= 0) ||
+                                                         !(d->mass_tol >= 0))))
+        return fail(nullptr, HUGKS_E_ARG, "steady test needs dim 1 and tolerances >= 0");
     if (!(d->cfl > 0) || !(d->prandtl > 0) || !(d->muref > 0) || !(d->gamma > 1) ||
         !(d->innerk >= 0) || !(d->pi > 0))
         return fail(nullptr, HUGKS_E_ARG,
@@ -277,6 +281,8 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     c.omega = d->omega; c.cfl = d->cfl; c.maxtime = d->maxtime; c.eps_res = d->eps_res;
     c.umax_dt = d->umax_dt; c.vmax_dt = d->vmax_dt; c.force_rate = d->force_rate;
     c.tref = d->tref; c.phi = d->phi;
+    c.steady_steps = dim == 1 ? d->steady_steps : 0;
+    c.steady_tol = d->steady_tol; c.mass_tol = d->mass_tol;
     Launch2D& L = h->L;
     L.fp64 = h->fp64;
     const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
@@ -396,6 +402,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     hipMemsetAsync(L.vflux, 0, nvf * 4 * sizeof(double), h->stream);
     DevState s0{};
     s0.iter = 1;
+    s0.t_stop = INFINITY;
+    // initial mass: sum of rho * cell size in cell order (2d/src/main.f90:63-68)
+    for (size_t q = 0; q < ncell; ++q) s0.mass0 += d->prim0[(size_t)(dim + 2) * q] * (c.dx * c.dy);
     hipMemcpyAsync(L.st, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream);
     e = launch_init_f(L, dim, prim0_d, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -418,15 +427,21 @@ int hugks_destroy(hugks_handle* h) {
 
 int hugks_step(hugks_handle* h, hugks_step_info* info) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
-    hipError_t e = launch_begin(h->L.st, h->stream);
+    hipError_t e = launch_begin(h->L.st, INFINITY, h->stream);
     if (e == hipSuccess) e = enqueue_step(h);
     if (e != hipSuccess) return hip_fail(h, e, "step launch");
     return read_state(h, info);
 }
 
 int hugks_advance(hugks_handle* h, int32_t max_steps, hugks_step_info* info) {
+    return hugks_advance_until(h, max_steps, INFINITY, info);
+}
+
+int hugks_advance_until(hugks_handle* h, int32_t max_steps, double t_stop,
+                        hugks_step_info* info) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
-    hipError_t e = launch_begin(h->L.st, h->stream);
+    if (std::isnan(t_stop)) return fail(h, HUGKS_E_ARG, "t_stop is NaN");
+    hipError_t e = launch_begin(h->L.st, t_stop, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "advance launch");
     if (max_steps > 0) {
         for (int32_t s = 0; s < max_steps; ++s) {
@@ -505,9 +520,17 @@ int hugks_set_f(hugks_handle* h, const double* hf, const double* bf, const doubl
             e = launch_convert_in(tmp, q ? h->L.b : h->L.h, h->fp64, h->ncell, c.nn, c.ldv, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
-    if (e == hipSuccess && w)
+    if (e == hipSuccess && w) {
         e = hipMemcpyAsync(h->L.W, w, h->ncell * (h->dim + 2) * sizeof(double),
                            hipMemcpyHostToDevice, h->stream);
+        // the new W is the reference of the relative mass change
+        double m0 = 0.0;
+        for (size_t q = 0; q < h->ncell; ++q) m0 += w[(size_t)(h->dim + 2) * q] * (c.dx * c.dy);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&h->L.st->mass0, &m0, sizeof(double), hipMemcpyHostToDevice,
+                               h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
     if (tmp) hipFreeAsync(tmp, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "set_f");

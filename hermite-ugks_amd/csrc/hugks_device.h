@@ -23,8 +23,9 @@ constexpr int WAVE = 64;
 struct Consts {
     int nx, ny, nv, nn, ldv, nm;
     int bc[4];
-    int tau_model, dt_rule, clamp_dt, stop_on_residual;
+    int tau_model, dt_rule, clamp_dt, stop_on_residual, steady_steps;
     double dx, dy, pi, smv, K, gamma, prandtl, shk, muref, omega;
+    double steady_tol, mass_tol;
     double cfl, maxtime, eps_res, umax_dt, vmax_dt, force_rate, tref, phi;
     double tw[4];
 };
@@ -36,12 +37,15 @@ struct DevState {
     double simtime;
     double res[4];
     double steady[3];
+    double t_stop;    // stop time of the current host call
+    double mass0;     // reference mass of the relative mass change
+    double mass_rel;  // (mass - mass0) / (|mass0| + smv) after the last step
     int iter;
     int stopped;      // set once an exit condition held; later steps no-op
     int stop_reason;
     int flags;        // bit 0: NaN/T<=0, bit 1: zero wall coefficient
     int steps;        // steps executed since the last host reset
-    int pad_;
+    int steady_count; // consecutive steps the 1D steady test has held
 };
 
 // ---------------------------------------------------------------- scalars

@@ -44,7 +44,7 @@ struct HookScope {
     }
 };
 
-hipError_t launch_begin(DevState* st, hipStream_t s);
+hipError_t launch_begin(DevState* st, double t_stop, hipStream_t s);
 hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s);
 hipError_t launch_flux2d(const Launch2D& L, hipStream_t s);
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s);

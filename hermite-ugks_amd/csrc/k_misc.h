@@ -33,11 +33,12 @@ __device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
 
 // Marks the start of a host call: later kernels run again even if a
 // previous call ended on an exit condition.
-__global__ void k_begin(DevState* st) {
+__global__ void k_begin(DevState* st, double t_stop) {
     if (threadIdx.x == 0) {
         st->stopped = 0;
         st->stop_reason = 0;
         st->steps = 0;
+        st->t_stop = t_stop;
     }
 }
 
@@ -129,12 +130,32 @@ __global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState
             all_small = false;
         }
         st->steps += 1;
+        // mass = sum of rho * cell size (rho > 0 is enforced by the update)
+        const double mass = (dim == 2 ? s[4] * (c.dx * c.dy) : s[3] * c.dx);
+        st->mass_rel = (mass - st->mass0) / (fabs(st->mass0) + c.smv);
         if (dim == 2 && st->simtime >= c.maxtime) {
             st->stopped = 1;
             st->stop_reason = HUGKS_STOP_MAXTIME;
         } else if (dim == 2 && c.stop_on_residual && all_small) {
             st->stopped = 1;
             st->stop_reason = HUGKS_STOP_RESIDUAL;
+        }
+        // Fourier steady test (fourier_is_steady, 1d/src/solver.f90:266-274;
+        // counting and exit, 1d/src/main.f90:108-126)
+        if (dim == 1 && c.steady_steps > 0) {
+            const double m = fmax(fmax(fabs(st->steady[0]), fabs(st->steady[1])), fabs(st->steady[2]));
+            if (m <= c.steady_tol && fabs(st->mass_rel) <= c.mass_tol)
+                st->steady_count += 1;
+            else
+                st->steady_count = 0;
+            if (st->steady_count >= c.steady_steps && !st->stopped) {
+                st->stopped = 1;
+                st->stop_reason = HUGKS_STOP_STEADY;
+            }
+        }
+        if (!st->stopped && st->simtime >= st->t_stop) {
+            st->stopped = 1;
+            st->stop_reason = HUGKS_STOP_TIME;
         }
     }
 }

@@ -9,8 +9,8 @@ hipError_t launch_finalize(const Launch2D& L, int dim, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_begin(DevState* st, hipStream_t s) {
-    hipLaunchKernelGGL((k_begin), dim3(1), dim3(64), 0, s, st);
+hipError_t launch_begin(DevState* st, double t_stop, hipStream_t s) {
+    hipLaunchKernelGGL((k_begin), dim3(1), dim3(64), 0, s, st, t_stop);
     return hipGetLastError();
 }
 

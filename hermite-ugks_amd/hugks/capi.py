@@ -20,7 +20,7 @@ BC_PERIODIC, BC_SPECULAR, BC_DIFFUSE = 0, 1, 2
 TAU_MU_OVER_RHO, TAU_VHS = 0, 1
 DT_2D_GENERAL, DT_2D_WALL_NORMAL, DT_1D = 0, 1, 2
 FORCE_HERMITE_SYMMETRIC, FORCE_HERMITE_SYMMETRIC_B, FORCE_HERMITE_LEGACY = 0, 1, 2
-STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL = 0, 1, 2
+STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL, STOP_TIME, STOP_STEADY = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 
@@ -31,14 +31,14 @@ class Desc(C.Structure):
         ("nv", C.c_int32), ("nm", C.c_int32), ("precision", C.c_int32),
         ("tau_model", C.c_int32), ("dt_rule", C.c_int32), ("clamp_dt", C.c_int32),
         ("stop_on_residual", C.c_int32), ("bc", C.c_int32 * 4), ("force_variant", C.c_int32),
-        ("reserved_i", C.c_int32 * 2),
+        ("steady_steps", C.c_int32), ("reserved_i", C.c_int32 * 1),
         ("dx", C.c_double), ("dy", C.c_double), ("pi", C.c_double), ("smv", C.c_double),
         ("innerk", C.c_double), ("gamma", C.c_double), ("prandtl", C.c_double),
         ("shakhov_coef", C.c_double), ("muref", C.c_double), ("omega", C.c_double),
         ("cfl", C.c_double), ("maxtime", C.c_double), ("eps_res", C.c_double),
         ("umax_dt", C.c_double), ("vmax_dt", C.c_double), ("force_rate", C.c_double),
         ("tw", C.c_double * 4), ("tref", C.c_double), ("phi", C.c_double),
-        ("reserved_d", C.c_double * 4),
+        ("steady_tol", C.c_double), ("mass_tol", C.c_double), ("reserved_d", C.c_double * 2),
         ("u", _dp), ("v", _dp), ("weight", _dp), ("prim0", _dp), ("accel_cell", _dp),
         ("accel_vface", _dp), ("accel_hface", _dp),
     ]
@@ -48,17 +48,19 @@ class StepInfo(C.Structure):
     _fields_ = [
         ("steps", C.c_int32), ("iter", C.c_int32), ("stop_reason", C.c_int32),
         ("device_flags", C.c_int32), ("simtime", C.c_double), ("dt", C.c_double),
-        ("res", C.c_double * 4), ("steady_res", C.c_double * 3),
+        ("res", C.c_double * 4), ("steady_res", C.c_double * 3), ("mass_rel_change", C.c_double),
     ]
 
     def as_dict(self):
         return {"steps": self.steps, "iter": self.iter, "stop_reason": self.stop_reason,
                 "device_flags": self.device_flags, "simtime": self.simtime, "dt": self.dt,
-                "res": list(self.res), "steady_res": list(self.steady_res)}
+                "res": list(self.res), "steady_res": list(self.steady_res),
+                "mass_rel_change": self.mass_rel_change}
 
 
 EXPORTS = [
     "hugks_create", "hugks_destroy", "hugks_last_error", "hugks_step", "hugks_advance",
+    "hugks_advance_until",
     "hugks_get_macro", "hugks_get_f", "hugks_set_f", "hugks_get_time", "hugks_kernel_times",
     "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
 ]
@@ -85,6 +87,7 @@ def load(path=LIBPATH):
     lib.hugks_last_error.restype = C.c_char_p
     lib.hugks_step.argtypes = [H, C.POINTER(StepInfo)]
     lib.hugks_advance.argtypes = [H, C.c_int32, C.POINTER(StepInfo)]
+    lib.hugks_advance_until.argtypes = [H, C.c_int32, C.c_double, C.POINTER(StepInfo)]
     lib.hugks_get_macro.argtypes = [H, _dp, _dp]
     lib.hugks_get_f.argtypes = [H, _dp, _dp]
     lib.hugks_set_f.argtypes = [H, _dp, _dp, _dp]
@@ -186,6 +189,12 @@ class Solver:
     def advance(self, max_steps):
         info = StepInfo()
         self._check(self.lib.hugks_advance(self.h, int(max_steps), C.byref(info)))
+        return info
+
+    def advance_until(self, max_steps, t_stop):
+        info = StepInfo()
+        self._check(self.lib.hugks_advance_until(self.h, int(max_steps), float(t_stop),
+                                                 C.byref(info)))
         return info
 
     def macro(self, with_q=True):

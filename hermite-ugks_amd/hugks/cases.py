@@ -244,6 +244,9 @@ def build_1d(p, precision=capi.FP64):
         prim[:, 0] = 1.0
         prim[:, 2] = 1.0 / (float(p["TwL"]) + x * (float(p["TwR"]) - float(p["TwL"])))
         d.bc[:] = [capi.BC_DIFFUSE, capi.BC_DIFFUSE, 0, 0]
+        # steady-state exit of the Fourier case, 1d/src/main.f90:108-126
+        d.steady_steps = int(p["steady_steps"])
+        d.steady_tol, d.mass_tol = float(p["steady_tol"]), float(p["mass_tol"])
     else:
         raise ValueError(f"1D case_id {case_id} is not one of the reference's cases (2, 3, 4)")
     d.tw[:] = [float(p["TwL"]), float(p["TwR"]), 1.0, 1.0]

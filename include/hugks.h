@@ -83,6 +83,8 @@ extern "C" {
 #define HUGKS_STOP_NONE      0  /* step cap reached                         */
 #define HUGKS_STOP_MAXTIME   1  /* simtime >= maxtime                       */
 #define HUGKS_STOP_RESIDUAL  2  /* all(res < eps_res) (2D case 5)           */
+#define HUGKS_STOP_TIME      3  /* simtime >= the t_stop of the call        */
+#define HUGKS_STOP_STEADY    4  /* 1D steady test held steady_steps times   */
 
 typedef struct hugks_desc {
     int32_t abi_version;      /* = HUGKS_ABI_VERSION                          */
@@ -99,7 +101,10 @@ typedef struct hugks_desc {
     int32_t force_variant;    /* HUGKS_FORCE_*: the reference operator named
                                  by force_type; the library evaluates all of
                                  them with one separable Hermite operator    */
-    int32_t reserved_i[2];
+    int32_t steady_steps;     /* 1D: stop after this many consecutive steady
+                                 steps (case 4, 1d/src/main.f90:109-126);
+                                 0 disables the test                         */
+    int32_t reserved_i[1];
     double dx, dy;            /* cell sizes (uniform grid; area = dx*dy)      */
     double pi;                /* the caller's PI constant (3.1415926535d0)    */
     double smv;               /* limiter/residual guard (2D 1e-8, 1D tiny)    */
@@ -116,7 +121,9 @@ typedef struct hugks_desc {
     double tw[4];             /* wall temperature per side (diffuse walls)    */
     double tref;              /* temperature scale of the 1D steady residual  */
     double phi;               /* 1D acceleration (2D uses the arrays below)   */
-    double reserved_d[4];
+    double steady_tol;        /* 1D steady test: max steady_res <= steady_tol */
+    double mass_tol;          /*   and |relative mass change| <= mass_tol     */
+    double reserved_d[2];
     const double *u;          /* [nv] u nodes                                 */
     const double *v;          /* [nv] v nodes (2D)                            */
     const double *weight;     /* [nv*nv] (2D, u fastest) or [nv] (1D)         */
@@ -135,6 +142,9 @@ typedef struct hugks_step_info {
     double dt;                /* dt of the last step                          */
     double res[4];            /* residual per conserved component             */
     double steady_res[3];     /* 1D primitive-change residual                 */
+    double mass_rel_change;   /* (mass - mass0) / (|mass0| + smv), mass =
+                                 sum of rho * cell size; mass0 at create or
+                                 at the last hugks_set_f with W              */
 } hugks_step_info;
 
 typedef struct hugks_handle hugks_handle;
@@ -159,6 +169,12 @@ int hugks_step(hugks_handle *h, hugks_step_info *info);
  * program evaluated on the device (2d/src/main.f90:103-104; 1d:84).
  * max_steps <= 0 runs until an exit condition. */
 int hugks_advance(hugks_handle *h, int32_t max_steps, hugks_step_info *info);
+
+/* As hugks_advance, and also stops after the first step that ends with
+ * simtime >= t_stop: the driver's output points (2d/src/main.f90:88-99,
+ * write_macro at time2/time3/time4) without a per-step host round trip. */
+int hugks_advance_until(hugks_handle *h, int32_t max_steps, double t_stop,
+                        hugks_step_info *info);
 
 /* Output fields of write_macro (2d/src/output2d.f90:156-161;
  * 1d/src/output.f90:309-321): w = conserved variables

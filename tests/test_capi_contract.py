@@ -47,7 +47,8 @@ def test_header_constants_match_binding():
                  "E_NOHIP", "FP64", "MIXED_FP32", "MIXED_BF16X3", "BC_PERIODIC", "BC_SPECULAR",
                  "BC_DIFFUSE", "TAU_MU_OVER_RHO", "TAU_VHS", "DT_2D_GENERAL", "DT_1D",
                  "FORCE_HERMITE_SYMMETRIC", "FORCE_HERMITE_SYMMETRIC_B", "FORCE_HERMITE_LEGACY",
-                 "STOP_NONE", "STOP_MAXTIME", "STOP_RESIDUAL", "ARR_SHX", "ARR_HFLUX"):
+                 "STOP_NONE", "STOP_MAXTIME", "STOP_RESIDUAL", "STOP_TIME", "STOP_STEADY",
+                 "ARR_SHX", "ARR_HFLUX"):
         assert consts["HUGKS_" + name] == getattr(capi, name), name
     assert consts["HUGKS_DT_2D_WALL_NORMAL"] == capi.DT_2D_WALL_NORMAL
 

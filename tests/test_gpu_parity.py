@@ -78,3 +78,24 @@ def test_mixed_parity(have_gpu, name, params, variant, steps):
     err = field_err(r["w"], r["wr"])
     print(name, "mixed W err", err)
     assert np.all(err <= TOL_MIXED), err
+
+
+def test_advance_until_output_time(have_gpu):
+    """advance_until stops after the first step ending at or past t_stop,
+    the reference main loop's write_macro points (2d/src/main.f90:88-99);
+    the state there equals the oracle's after the same number of steps."""
+    case = cases.build_2d(rt(), capi.FP64, rt_variant="cartesian", seed=3)
+    o = oracle_py.Oracle(case)
+    ts = []
+    for _ in range(12):
+        ts.append(o.step().simtime)
+    t_stop = 0.5 * (ts[6] + ts[7])      # reached by step 8
+    with capi.Solver(case) as s:
+        info = s.advance_until(0, t_stop)
+        assert info.stop_reason == capi.STOP_TIME and info.steps == 8
+        w = s.macro(with_q=False)
+        info2 = s.advance_until(0, ts[11])
+        assert info2.stop_reason == capi.STOP_TIME and info2.steps == 4
+    o2 = oracle_py.Oracle(case)
+    o2.step(8)
+    assert np.all(field_err(w, o2.get()[0]) <= TOL_FP64)
