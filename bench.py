@@ -79,7 +79,8 @@ def algorithmic_bytes(case, kernel):
 def run_gpu(case, warmup, steps, timing=True):
     import torch
     s = capi.Solver(case)
-    s.advance(warmup)
+    if warmup > 0:               # advance(0) would run to the loop exit
+        s.advance(warmup)
     torch.cuda.synchronize()
     s.set_timing(timing)
     s.reset_kernel_times()
@@ -88,6 +89,8 @@ def run_gpu(case, warmup, steps, timing=True):
     info = s.advance(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if info.steps != steps:
+        raise RuntimeError(f"the timed region ran {info.steps} steps, not {steps}")
     kt = s.kernel_times()
     w = s.macro(with_q=False)
     s.close()
