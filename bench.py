@@ -73,6 +73,13 @@ def algorithmic_bytes(case, kernel):
         return 6 * s * N + 2 * s * nhf
     if kernel == "update2d":         # read h, b, 4 face arrays; write h, b
         return 4 * s * N + 2 * s * (nvf + nhf)
+    # interior-face pipeline: both orientations per launch
+    xper, yper = d.bc[0] == capi.BC_PERIODIC, d.bc[2] == capi.BC_PERIODIC
+    nint = (d.nx + 1 if xper else d.nx - 1) * d.ny + d.nx * (d.ny + 1 if yper else d.ny - 1)
+    if kernel == "flux_sum":         # read 6 upwind arrays per face node; 37 sums per face
+        return 6 * s * nint * nn + 8 * 37 * nint
+    if kernel == "flux_node":        # read 6 upwind arrays, write 2 face arrays per face node
+        return 8 * s * nint * nn
     return None
 
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -376,6 +377,16 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         for (int s = 0; s < 4; ++s)
             if (d->bc[s] != HUGKS_BC_PERIODIC) nwall += s < 2 ? d->ny : d->nx;
         L.nwall = nwall;
+        // interior-face pipeline scratch (per-face sums and coefficients)
+        const bool xper = d->bc[0] == HUGKS_BC_PERIODIC, yper = d->bc[2] == HUGKS_BC_PERIODIC;
+        const size_t nint = (size_t)(xper ? d->nx + 1 : d->nx - 1) * d->ny +
+                            (size_t)d->nx * (yper ? d->ny + 1 : d->ny - 1);
+        L.flux_mode = 1;
+        if (const char* fm = getenv("HUGKS_FLUX_KERNEL"))
+            if (strcmp(fm, "fused") == 0) L.flux_mode = 0;
+        ALLOC(L.fsum, (nint ? nint : 1) * FACE_SUM_LD * sizeof(double));
+        ALLOC(L.fcoef, (nint ? nint : 1) * FACE_COEF_LD * sizeof(double));
+        ALLOC(L.fmaxw, (nint ? nint : 1) * FACE_MAXW_LD * sizeof(double));
         L.update_lds = update2d_lds_bytes(c, h->fp64);
         hipDeviceProp_t prop;
         hipGetDeviceProperties(&prop, h->device);

@@ -26,11 +26,19 @@ struct Launch2D {
     const double *u1, *v1;                   // 1D node lists [nv] (double)
     const double *acc_cell, *acc_v, *acc_h;
     unsigned nwall;      // number of wall faces (non-periodic sides)
+    int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline
+    double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
     int update_nt;       // threads per update workgroup
     size_t update_lds;   // dynamic LDS bytes of the update kernel
     LaunchHook hook;     // may be null
     void* hook_ctx;
 };
+
+// Leading dimensions of the per-face scratch of the interior-face pipeline
+// (k_flux_split.h).
+constexpr int FACE_SUM_LD = 40;
+constexpr int FACE_COEF_LD = 40;
+constexpr int FACE_MAXW_LD = 16;
 
 // Brackets one launch with the hook.
 struct HookScope {

@@ -1,0 +1,562 @@
+// Interior-face UGKS flux as a five-kernel pipeline (gfx950):
+// calc_flux_mtforcev / calc_flux_mtforceh, 2d/src/flux.f90:28-472.
+//
+// Both face orientations go through each launch; the global interior face
+// index g < nvi is a vertical face, g >= nvi a horizontal one (the branch
+// is uniform per wavefront).
+//
+//   K1 k_fsum      one wavefront per face, one read of the upwind inputs
+//                  (h, b and their normal/tangential slopes): the 37
+//                  velocity sums that do not depend on the face state, in
+//                  FP64 (flux.f90:68-102,116-120,184-202, and the raw
+//                  moments the heat flux is expanded into)
+//   K2 k_fstate    one lane per face: state at the face, tau, the time
+//                  coefficients Mt, aL, aR, the tangential slope, the
+//                  f0 part of the macro flux and the heat flux
+//                  (flux.f90:93-147, get_heat_flux)
+//   K3 k_fmaxw     one wavefront per face, no distribution reads: the
+//                  Maxwellian sums of the force compatibility
+//                  (flux.f90:691-705) and of the Shakhov terms
+//   K4 k_fslope_t  one lane per face: time slope aT (flux.f90:126-135) and
+//                  the conserved flux (flux.f90:152-246)
+//   K5 k_fnode     one wavefront per face, second read of the inputs: the
+//                  distribution fluxes per node (flux.f90:207-236)
+//
+// The scalar moment algebra runs once per face on one lane instead of on
+// every lane of the face's wavefront, and the distribution functions are
+// read twice instead of three times.  Velocity sums are FP64 with a fixed
+// butterfly order (bitwise reproducible).
+#pragma once
+#include "hugks_device.h"
+#include "hugks_kernels.h"
+
+namespace hugks {
+
+constexpr int FSUM_N = 37;      // K1 sums per face
+constexpr int FSUM_LD = FACE_SUM_LD;
+constexpr int FCOEF_LD = FACE_COEF_LD;  // K2/K4 coefficients per face
+constexpr int FMAXW_N = 13;     // K3 sums per face
+constexpr int FMAXW_LD = FACE_MAXW_LD;
+constexpr int SPLIT_FPW = 4;    // faces (wavefronts) per workgroup of K1/K3/K5
+
+struct FaceGeo {
+    int dir, face, cL, cR;
+};
+
+// Interior face g -> orientation, face slot and its two cells (the face
+// numbering of the vertical and horizontal face arrays).
+__device__ __forceinline__ FaceGeo face_geo(const Consts& c, int g, int nvi) {
+    FaceGeo r;
+    if (g < nvi) {
+        const bool per = c.bc[0] == HUGKS_BC_PERIODIC;
+        const int nf = per ? c.nx + 1 : c.nx - 1;
+        const int i = (per ? 0 : 1) + g % nf, j = g / nf;
+        r.dir = 0;
+        r.face = i + (c.nx + 1) * j;
+        r.cL = (i - 1 + c.nx) % c.nx + c.nx * j;
+        r.cR = i % c.nx + c.nx * j;
+    } else {
+        g -= nvi;
+        const bool per = c.bc[2] == HUGKS_BC_PERIODIC;
+        const int i = g % c.nx, j = (per ? 0 : 1) + g / c.nx;
+        r.dir = 1;
+        r.face = i + c.nx * j;
+        r.cL = i + c.nx * ((j - 1 + c.ny) % c.ny);
+        r.cR = i + c.nx * (j % c.ny);
+    }
+    return r;
+}
+
+template <int NV>
+__device__ __forceinline__ void wave_sum_bcast(double (&v)[NV]) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double x = v[i];
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+        v[i] = __shfl(x, 0, WAVE);
+    }
+}
+
+// ------------------------------------------------------------------ K1
+template <int DIR, typename T>
+__device__ __forceinline__ void fsum_face(const Consts& c, const FaceGeo& fg, double vshift,
+                                          const T* __restrict__ h, const T* __restrict__ b,
+                                          const T* __restrict__ shx, const T* __restrict__ shy,
+                                          const T* __restrict__ sbx, const T* __restrict__ sby,
+                                          const typename Arith<T>::A* __restrict__ unode,
+                                          const typename Arith<T>::A* __restrict__ vnode,
+                                          const double* __restrict__ und,
+                                          const double* __restrict__ vnd,
+                                          const double* __restrict__ wnd, double* __restrict__ out) {
+    using A = typename Arith<T>::A;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const T* sN = DIR == 0 ? shx : shy;
+    const T* sT = DIR == 0 ? shy : shx;
+    const T* bN = DIR == 0 ? sbx : sby;
+    const T* bT = DIR == 0 ? sby : sbx;
+    const double len = DIR == 0 ? c.dx : c.dy;
+    const double hl = 0.5 * len;
+    const double tsign = DIR == 0 ? 1.0 : -1.0;
+    const A vs = (A)vshift;
+    const size_t baseL = (size_t)fg.cL * c.ldv, baseR = (size_t)fg.cR * c.ldv;
+    double s[FSUM_N];
+#pragma unroll
+    for (int q = 0; q < FSUM_N; ++q) s[q] = 0.0;
+#pragma unroll 2
+    for (int k = lane; k < c.nn; k += WAVE) {
+        // upwind choice in the node arithmetic, as the fused kernel makes it
+        const int d = upwind_delta((DIR == 0 ? unode[k] : vnode[k]) - vs);
+        const size_t o = (d ? baseL : baseR) + k;
+        const double sgn = d ? hl : -hl;
+        const double sn = (double)sN[o], bn = (double)bN[o];
+        const double hf = (double)h[o] + sgn * sn;
+        const double bf = (double)b[o] + sgn * bn;
+        const double st = tsign * (double)sT[o], bt = tsign * (double)bT[o];
+        const double u = und[k], v = vnd[k], w = wnd[k];
+        const double vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
+        const double a = w * hf, e = w * sn, f = w * st, g = w * bf, p = w * bn, r = w * bt;
+        const double vn2 = vn * vn, vt2 = vt * vt, c2 = vn2 + vt2, vnvt = vn * vt;
+        const double vn3 = vn2 * vn, vn2vt = vn2 * vt, vt2vn = vt2 * vn, vnc2 = vn * c2;
+        s[0] += a;
+        s[1] += vn * a;
+        s[2] += vt * a;
+        s[3] += c2 * a;
+        s[4] += g;
+        s[5] += f;
+        s[6] += vn * f;
+        s[7] += vt * f;
+        s[8] += c2 * f;
+        s[9] += r;
+        s[10] += vn2 * e;
+        s[11] += vnvt * f;
+        s[12] += vn2 * a;
+        s[13] += vn3 * e;
+        s[14] += vn2vt * f;
+        s[15] += vnvt * a;
+        s[16] += vn2vt * e;
+        s[17] += vt2vn * f;
+        s[18] += vnc2 * a;
+        s[19] += vn * g;
+        s[20] += (vn2 * c2) * e;
+        s[21] += vn2 * p;
+        s[22] += (vnvt * c2) * f;
+        s[23] += vnvt * r;
+        s[24] += vn * e;
+        s[25] += vnvt * e;
+        s[26] += vnc2 * e;
+        s[27] += vn * p;
+        s[28] += vn2 * f;
+        s[29] += vnc2 * f;
+        s[30] += vn * r;
+        // raw moments of the heat flux expansion
+        s[31] += vt2 * a;
+        s[32] += vn3 * a;
+        s[33] += vn2vt * a;
+        s[34] += vt2vn * a;
+        s[35] += (vt2 * vt) * a;
+        s[36] += vt * g;
+    }
+    wave_sum_bcast<FSUM_N>(s);
+    if (lane < FSUM_N) {
+        double x = s[0];
+#pragma unroll
+        for (int q = 1; q < FSUM_N; ++q)
+            if (lane == q) x = s[q];
+        out[lane] = x;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
+    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ und,
+    const double* __restrict__ vnd, const double* __restrict__ wnd,
+    const double* __restrict__ acc_v, const double* __restrict__ acc_h,
+    double* __restrict__ fsum, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    const double dt = st->dt;
+    if (fg.dir == 0) {
+        const double vshift = 0.5 * acc_v[2 * fg.face] * dt;
+        fsum_face<0, T>(c, fg, vshift, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd,
+                        fsum + (size_t)g * FSUM_LD);
+    } else {
+        const double vshift = 0.5 * acc_h[2 * fg.face] * dt;
+        fsum_face<1, T>(c, fg, vshift, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd,
+                        fsum + (size_t)g * FSUM_LD);
+    }
+}
+
+// ------------------------------------------------------------------ K2
+// Coefficient slots of one face.
+enum : int {
+    FC_P = 0,      // prim at the face, 4
+    FC_TAU = 4,
+    FC_MT = 5,     // Mt(1..6) at 5..10
+    FC_AL = 11,    // 4
+    FC_AR = 15,    // 4
+    FC_AB = 19,    // 4
+    FC_FL = 23,    // 4: f0 and force part of the macro flux
+    FC_Q = 27,     // heat flux, 2
+    FC_AT = 29,    // 4 (K4)
+};
+
+__global__ __launch_bounds__(256) void k_fstate(Consts c, const DevState* __restrict__ st,
+                                                const double* __restrict__ W,
+                                                const double* __restrict__ acc_v,
+                                                const double* __restrict__ acc_h,
+                                                const double* __restrict__ fsum,
+                                                double* __restrict__ fcoef, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    const double* s1 = fsum + (size_t)g * FSUM_LD;
+    double* fc = fcoef + (size_t)g * FCOEF_LD;
+    const double dt = st->dt;
+    const double* acc = fg.dir == 0 ? acc_v : acc_h;
+    const double fn = acc[2 * fg.face], ft = acc[2 * fg.face + 1];
+    const double len = fg.dir == 0 ? c.dx : c.dy;
+    double p[4];
+    const double w[4] = {s1[0], s1[1], s1[2], 0.5 * (s1[3] + s1[4])};
+    prim_from_w(c, w, p);
+    const double tau = tau_of(c, p[0], p[3]);
+    double Mt[7];
+    {
+        const double ex = exp(-dt / tau);
+        Mt[4] = tau * (1.0 - ex);
+        Mt[5] = -tau * dt * ex + tau * Mt[4];
+        Mt[1] = dt - Mt[4];
+        Mt[2] = -tau * Mt[1] + Mt[5];
+        Mt[3] = dt * dt / 2.0 - tau * Mt[1];
+        Mt[6] = tau * tau * tau - 0.5 * ex * tau * (dt * dt + 2.0 * dt * tau + 2.0 * tau * tau);
+    }
+    double aL[4], aR[4], ab[4], fl[4];
+    {
+        const double* wl = W + 4 * (size_t)fg.cL;
+        const double* wr = W + 4 * (size_t)fg.cR;
+        double locL[4], locR[4], sw[4];
+        if (fg.dir == 0) {
+            for (int q = 0; q < 4; ++q) { locL[q] = wl[q]; locR[q] = wr[q]; }
+        } else {
+            locL[0] = wl[0]; locL[1] = wl[2]; locL[2] = -wl[1]; locL[3] = wl[3];
+            locR[0] = wr[0]; locR[1] = wr[2]; locR[2] = -wr[1]; locR[3] = wr[3];
+        }
+        for (int q = 0; q < 4; ++q) sw[q] = (w[q] - locL[q]) / (0.5 * len);
+        micro_slope(c, p, sw, aL);
+        for (int q = 0; q < 4; ++q) sw[q] = (locR[q] - w[q]) / (0.5 * len);
+        micro_slope(c, p, sw, aR);
+        sw[0] = s1[5]; sw[1] = s1[6]; sw[2] = s1[7]; sw[3] = 0.5 * (s1[8] + s1[9]);
+        micro_slope(c, p, sw, ab);
+        fl[0] = Mt[4] * s1[1] - Mt[5] * s1[10] - Mt[5] * s1[11];
+        fl[1] = Mt[4] * s1[12] - Mt[5] * s1[13] - Mt[5] * s1[14];
+        fl[2] = Mt[4] * s1[15] - Mt[5] * s1[16] - Mt[5] * s1[17];
+        fl[3] = Mt[4] * 0.5 * (s1[18] + s1[19]) - Mt[5] * 0.5 * (s1[20] + s1[21]) -
+                Mt[5] * 0.5 * (s1[22] + s1[23]);
+        const double gg[4] = {fn * Mt[6] * s1[24] + ft * Mt[6] * s1[6],
+                              fn * Mt[6] * s1[10] + ft * Mt[6] * s1[28],
+                              fn * Mt[6] * s1[25] + ft * Mt[6] * s1[11],
+                              fn * Mt[6] * 0.5 * (s1[26] + s1[27]) +
+                                  ft * Mt[6] * 0.5 * (s1[29] + s1[30])};
+        for (int q = 0; q < 4; ++q) fl[q] += gg[q];
+    }
+    // heat flux (get_heat_flux, 2d/src/gastheory2d.f90:115-128) from the raw
+    // moments M_ab = sum w vn^a vt^b h and B_ab of b, expanded about (U, V)
+    {
+        const double U = p[1], V = p[2];
+        const double M00 = s1[0], M10 = s1[1], M01 = s1[2], M20 = s1[12], M11 = s1[15];
+        const double M02 = s1[31], M30 = s1[32], M21 = s1[33], M12 = s1[34], M03 = s1[35];
+        const double B00 = s1[4], B10 = s1[19], B01 = s1[36];
+        const double hx = M30 - 3.0 * U * M20 + 3.0 * U * U * M10 - U * U * U * M00 + M12 -
+                          2.0 * V * M11 + V * V * M10 - U * M02 + 2.0 * U * V * M01 -
+                          U * V * V * M00;
+        const double hy = M03 - 3.0 * V * M02 + 3.0 * V * V * M01 - V * V * V * M00 + M21 -
+                          2.0 * U * M11 + U * U * M01 - V * M20 + 2.0 * U * V * M10 -
+                          U * U * V * M00;
+        fc[FC_Q] = 0.5 * (hx + (B10 - U * B00));
+        fc[FC_Q + 1] = 0.5 * (hy + (B01 - V * B00));
+    }
+    for (int q = 0; q < 4; ++q) {
+        fc[FC_P + q] = p[q];
+        fc[FC_AL + q] = aL[q];
+        fc[FC_AR + q] = aR[q];
+        fc[FC_AB + q] = ab[q];
+        fc[FC_FL + q] = fl[q];
+    }
+    fc[FC_TAU] = tau;
+    for (int q = 1; q <= 6; ++q) fc[FC_MT + q - 1] = Mt[q];
+}
+
+// ------------------------------------------------------------------ K3
+template <int DIR, typename T>
+__device__ __forceinline__ void fmaxw_face(const Consts& c, const double* __restrict__ fc,
+                                           double fn, double ft,
+                                           const double* __restrict__ und,
+                                           const double* __restrict__ vnd,
+                                           const double* __restrict__ wnd,
+                                           double* __restrict__ out) {
+    using A = typename Arith<T>::A;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const double U = fc[FC_P + 1], V = fc[FC_P + 2], lam = fc[FC_P + 3];
+    const A lamA = (A)lam;
+    const A rho_l_pi = (A)(fc[FC_P] * (lam / c.pi));
+    const double kfac = c.K / (2.0 * lam);
+    double s[FMAXW_N];
+#pragma unroll
+    for (int q = 0; q < FMAXW_N; ++q) s[q] = 0.0;
+#pragma unroll 2
+    for (int k = lane; k < c.nn; k += WAVE) {
+        const double u = und[k], v = vnd[k], w = wnd[k];
+        const double vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
+        const double cu = vn - U, cv = vt - V;
+        const double cp2 = cu * cu + cv * cv;
+        const double c2 = vn * vn + vt * vt;
+        // the Maxwellian in the node arithmetic (FP32 on the mixed path)
+        const double H0 = (double)(rho_l_pi * fexp(-lamA * (A)cp2));
+        const double B0 = H0 * kfac;
+        const double wg = w * lam * (fn * cu + ft * cv);
+        s[0] += vn * wg * H0;
+        s[1] += vt * wg * H0;
+        s[2] += wg * (c2 * H0 + B0);
+        const double lam2 = 2.0 * lam * cp2;
+        const double wvn = w * vn;
+        const double P5H = (lam2 + c.K - 5.0) * H0 * wvn;
+        const double P3B = (lam2 + c.K - 3.0) * B0 * wvn;
+        s[3] += cu * P5H;
+        s[4] += cv * P5H;
+        s[5] += vn * cu * P5H;
+        s[6] += vn * cv * P5H;
+        s[7] += vt * cu * P5H;
+        s[8] += vt * cv * P5H;
+        s[9] += c2 * cu * P5H;
+        s[10] += c2 * cv * P5H;
+        s[11] += cu * P3B;
+        s[12] += cv * P3B;
+    }
+    wave_sum_bcast<FMAXW_N>(s);
+    if (lane < FMAXW_N) {
+        double x = s[0];
+#pragma unroll
+        for (int q = 1; q < FMAXW_N; ++q)
+            if (lane == q) x = s[q];
+        out[lane] = x;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fmaxw(
+    Consts c, const DevState* __restrict__ st, const double* __restrict__ und,
+    const double* __restrict__ vnd, const double* __restrict__ wnd,
+    const double* __restrict__ acc_v, const double* __restrict__ acc_h,
+    const double* __restrict__ fcoef, double* __restrict__ fmaxw, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    const double* acc = fg.dir == 0 ? acc_v : acc_h;
+    const double fn = acc[2 * fg.face], ft = acc[2 * fg.face + 1];
+    const double* fc = fcoef + (size_t)g * FCOEF_LD;
+    double* out = fmaxw + (size_t)g * FMAXW_LD;
+    if (fg.dir == 0)
+        fmaxw_face<0, T>(c, fc, fn, ft, und, vnd, wnd, out);
+    else
+        fmaxw_face<1, T>(c, fc, fn, ft, und, vnd, wnd, out);
+}
+
+// ------------------------------------------------------------------ K4
+__global__ __launch_bounds__(256) void k_fslope_t(Consts c, const DevState* __restrict__ st,
+                                                  const double* __restrict__ acc_v,
+                                                  const double* __restrict__ acc_h,
+                                                  const double* __restrict__ fmaxw,
+                                                  double* __restrict__ fcoef,
+                                                  double* __restrict__ vflux,
+                                                  double* __restrict__ hflux, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    double* fc = fcoef + (size_t)g * FCOEF_LD;
+    const double* s2 = fmaxw + (size_t)g * FMAXW_LD;   // s2[q] = K3 sum q
+    const double* acc = fg.dir == 0 ? acc_v : acc_h;
+    const double fn = acc[2 * fg.face], ft = acc[2 * fg.face + 1];
+    double p[4], aL[4], aR[4], ab[4], fl[4], Mt[7];
+    for (int q = 0; q < 4; ++q) {
+        p[q] = fc[FC_P + q];
+        aL[q] = fc[FC_AL + q];
+        aR[q] = fc[FC_AR + q];
+        ab[q] = fc[FC_AB + q];
+        fl[q] = fc[FC_FL + q];
+    }
+    for (int q = 1; q <= 6; ++q) Mt[q] = fc[FC_MT + q - 1];
+    const double tau = fc[FC_TAU], qf0 = fc[FC_Q], qf1 = fc[FC_Q + 1];
+    Moments m;
+    calc_moment(c, p, m);
+    double aT[4];
+    {
+        // flux.f90:126-135
+        double MauL[4], MauR[4], Mbu[4], sw[4];
+        moment_au(aL, m.MuL, m.Mv, m.Mxi, 1, 0, MauL);
+        moment_au(aR, m.MuR, m.Mv, m.Mxi, 1, 0, MauR);
+        moment_au(ab, m.Mu, m.Mv, m.Mxi, 0, 1, Mbu);
+        for (int q = 0; q < 4; ++q) sw[q] = -p[0] * (MauL[q] + MauR[q] + Mbu[q]);
+        sw[1] = sw[1] + 2.0 * s2[0];
+        sw[2] = sw[2] + 2.0 * s2[1];
+        sw[3] = sw[3] + s2[2];
+        micro_slope(c, p, sw, aT);
+    }
+    for (int q = 0; q < 4; ++q) fc[FC_AT + q] = aT[q];
+    double MauL[4], MauR[4], Mbu[4], MauT[4], Mau0[4], g0[4];
+    // g0 part (flux.f90:152-172)
+    moment_uv(m.Mu, m.Mv, m.Mxi, 1, 0, 0, Mau0);
+    moment_au(aL, m.MuL, m.Mv, m.Mxi, 2, 0, MauL);
+    moment_au(aR, m.MuR, m.Mv, m.Mxi, 2, 0, MauR);
+    moment_au(ab, m.Mu, m.Mv, m.Mxi, 1, 1, Mbu);
+    moment_au(aT, m.Mu, m.Mv, m.Mxi, 1, 0, MauT);
+    for (int q = 0; q < 4; ++q)
+        g0[q] = Mt[1] * p[0] * Mau0[q] + Mt[2] * p[0] * (MauL[q] + MauR[q] + Mbu[q]) +
+                Mt[3] * p[0] * MauT[q];
+    moment_au(aL, m.MuL, m.Mv, m.Mxi, 1, 0, MauL);
+    moment_au(aR, m.MuR, m.Mv, m.Mxi, 1, 0, MauR);
+    for (int q = 0; q < 4; ++q)
+        g0[q] = g0[q] - tau * Mt[2] * p[0] * fn * (MauL[q] + MauR[q]) -
+                Mt[6] * p[0] * fn * (MauL[q] + MauR[q]);
+    moment_au(ab, m.Mu, m.Mv, m.Mxi, 0, 1, Mbu);
+    for (int q = 0; q < 4; ++q)
+        g0[q] = g0[q] - tau * Mt[2] * p[0] * ft * Mbu[q] - Mt[6] * p[0] * ft * Mbu[q];
+    // Shakhov moments from their heat-flux split (flux.f90:184-191)
+    const double cs = c.shk * (1.0 - c.prandtl) * p[3] * p[3] / p[0];
+    const double X0 = cs * (qf0 * s2[3] + qf1 * s2[4]);
+    const double X1 = cs * (qf0 * s2[5] + qf1 * s2[6]);
+    const double X2 = cs * (qf0 * s2[7] + qf1 * s2[8]);
+    const double X3 = cs * (qf0 * s2[9] + qf1 * s2[10]);
+    const double X4 = cs * (qf0 * s2[11] + qf1 * s2[12]);
+    const double out4[4] = {g0[0] + Mt[1] * X0 + fl[0], g0[1] + Mt[1] * X1 + fl[1],
+                            g0[2] + Mt[1] * X2 + fl[2], g0[3] + Mt[1] * 0.5 * (X3 + X4) + fl[3]};
+    // global frame (flux.f90:242) and face length
+    const double flen = fg.dir == 0 ? c.dy : c.dx;
+    double g1 = out4[1], g2 = out4[2];
+    if (fg.dir == 1) { g1 = -out4[2]; g2 = out4[1]; }
+    double* out = (fg.dir == 0 ? vflux : hflux) + 4 * (size_t)fg.face;
+    out[0] = flen * out4[0];
+    out[1] = flen * g1;
+    out[2] = flen * g2;
+    out[3] = flen * out4[3];
+}
+
+// ------------------------------------------------------------------ K5
+template <int DIR, typename T>
+__device__ __forceinline__ void fnode_face(const Consts& c, const FaceGeo& fg,
+                                           const double* __restrict__ fc, double fn, double ft,
+                                           const T* __restrict__ h, const T* __restrict__ b,
+                                           const T* __restrict__ shx, const T* __restrict__ shy,
+                                           const T* __restrict__ sbx, const T* __restrict__ sby,
+                                           const typename Arith<T>::A* __restrict__ unode,
+                                           const typename Arith<T>::A* __restrict__ vnode,
+                                           double dt, T* __restrict__ ofh, T* __restrict__ ofb) {
+    using A = typename Arith<T>::A;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const T* sN = DIR == 0 ? shx : shy;
+    const T* sT = DIR == 0 ? shy : shx;
+    const T* bN = DIR == 0 ? sbx : sby;
+    const T* bT = DIR == 0 ? sby : sbx;
+    const double len = DIR == 0 ? c.dx : c.dy;
+    const double flen = DIR == 0 ? c.dy : c.dx;
+    const A tsign = DIR == 0 ? (A)1 : (A)-1;
+    const A hlen = (A)(0.5 * len);
+    const A vshift = (A)(0.5 * fn * dt);
+    const size_t baseL = (size_t)fg.cL * c.ldv, baseR = (size_t)fg.cR * c.ldv;
+    const double p0 = fc[FC_P], p1 = fc[FC_P + 1], p2 = fc[FC_P + 2], p3 = fc[FC_P + 3];
+    const double tau = fc[FC_TAU];
+    const A U = (A)p1, V = (A)p2, lam = (A)p3;
+    const A rho_l_pi = (A)(p0 * (p3 / c.pi));
+    const A kfac = (A)(c.K / (2.0 * p3));
+    const A fnA = (A)fn, ftA = (A)ft;
+    const A K = (A)c.K;
+    const A mt1 = (A)fc[FC_MT + 0], mt2 = (A)fc[FC_MT + 1], mt3 = (A)fc[FC_MT + 2],
+            mt4 = (A)fc[FC_MT + 3], mt5 = (A)fc[FC_MT + 4], mt6 = (A)fc[FC_MT + 5];
+    const A tm2fn = (A)(tau * fc[FC_MT + 1] * fn), tm2ft = (A)(tau * fc[FC_MT + 1] * ft);
+    const A mxi2 = (A)((c.K * c.K + 2.0 * c.K) / (4.0 * p3 * p3));
+    const A csA = (A)(c.shk * (1.0 - c.prandtl) * p3 * p3 / p0);
+    const A q0 = (A)fc[FC_Q], q1 = (A)fc[FC_Q + 1];
+    const A flenA = (A)flen;
+    A a_L[4], a_R[4], a_b[4], a_T[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        a_L[q] = (A)fc[FC_AL + q];
+        a_R[q] = (A)fc[FC_AR + q];
+        a_b[q] = (A)fc[FC_AB + q];
+        a_T[q] = (A)fc[FC_AT + q];
+    }
+#pragma unroll 2
+    for (int k = lane; k < c.nn; k += WAVE) {
+        const A u = unode[k], v = vnode[k];
+        const A vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
+        const int d = upwind_delta(vn - vshift);
+        const size_t o = (d ? baseL : baseR) + k;
+        const A sgn = d ? hlen : -hlen;
+        const A sn = (A)sN[o], bn = (A)bN[o];
+        const A hr = (A)h[o] + sgn * sn;
+        const A br = (A)b[o] + sgn * bn;
+        const A st = tsign * (A)sT[o], bt = tsign * (A)bT[o];
+        const A cu = vn - U, cv = vt - V;
+        const A cp2 = cu * cu + cv * cv;
+        const A c2 = vn * vn + vt * vt;
+        const A H = rho_l_pi * fexp(-lam * cp2);
+        const A Bm = H * kfac;
+        const A cq = csA * (cu * q0 + cv * q1);
+        const A lam2 = (A)2 * lam * cp2;
+        const A Hp = cq * (lam2 + K - (A)5) * H;
+        const A Bp = cq * (lam2 + K - (A)3) * Bm;
+        const A eh = c2 * H + Bm, eb = c2 * Bm + mxi2 * H;
+        const A u0 = d ? a_L[0] : a_R[0], u1 = d ? a_L[1] : a_R[1];
+        const A u2 = d ? a_L[2] : a_R[2], u3 = d ? a_L[3] : a_R[3];
+        const A gh = u0 * H + u1 * vn * H + u2 * vt * H + (A)0.5 * u3 * eh;
+        const A gb = u0 * Bm + u1 * vn * Bm + u2 * vt * Bm + (A)0.5 * u3 * eb;
+        const A gbh = a_b[0] * H + a_b[1] * vn * H + a_b[2] * vt * H + (A)0.5 * a_b[3] * eh;
+        const A gbb = a_b[0] * Bm + a_b[1] * vn * Bm + a_b[2] * vt * Bm + (A)0.5 * a_b[3] * eb;
+        const A gTh = a_T[0] * H + a_T[1] * vn * H + a_T[2] * vt * H + (A)0.5 * a_T[3] * eh;
+        const A gTb = a_T[0] * Bm + a_T[1] * vn * Bm + a_T[2] * vt * Bm + (A)0.5 * a_T[3] * eb;
+        A xh = mt1 * vn * (H + Hp) + mt2 * vn * vn * gh + mt2 * vn * vt * gbh + mt3 * vn * gTh +
+               mt4 * vn * hr - mt5 * vn * vn * sn - mt5 * vn * vt * st;
+        A xb = mt1 * vn * (Bm + Bp) + mt2 * vn * vn * gb + mt2 * vn * vt * gbb + mt3 * vn * gTb +
+               mt4 * vn * br - mt5 * vn * vn * bn - mt5 * vn * vt * bt;
+        xh = xh - tm2fn * vn * gh + mt6 * fnA * vn * sn;
+        xb = xb - tm2fn * vn * gb + mt6 * fnA * vn * bn;
+        xh = xh - tm2ft * vn * gbh + mt6 * ftA * vn * st;
+        xb = xb - tm2ft * vn * gbb + mt6 * ftA * vn * bt;
+        ofh[k] = (T)(flenA * xh);
+        ofb[k] = (T)(flenA * xb);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
+    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ acc_v,
+    const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
+    T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    const double dt = st->dt;
+    const double* fc = fcoef + (size_t)g * FCOEF_LD;
+    if (fg.dir == 0) {
+        const double fn = acc_v[2 * fg.face], ft = acc_v[2 * fg.face + 1];
+        fnode_face<0, T>(c, fg, fc, fn, ft, h, b, shx, shy, sbx, sby, unode, vnode, dt,
+                         vfh + (size_t)fg.face * c.ldv, vfb + (size_t)fg.face * c.ldv);
+    } else {
+        const double fn = acc_h[2 * fg.face], ft = acc_h[2 * fg.face + 1];
+        fnode_face<1, T>(c, fg, fc, fn, ft, h, b, shx, shy, sbx, sby, unode, vnode, dt,
+                         hfh + (size_t)fg.face * c.ldv, hfb + (size_t)fg.face * c.ldv);
+    }
+}
+
+}  // namespace hugks

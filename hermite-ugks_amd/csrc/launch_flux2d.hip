@@ -1,6 +1,7 @@
 // Host launchers of the 2D slope, interior-face and wall-face kernels.
 #include "k_flux2d.h"
 #include "k_flux_face.h"
+#include "k_flux_split.h"
 
 namespace hugks {
 
@@ -23,14 +24,48 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     const A* vn = (const A*)L.vnode;
     const A* wn = (const A*)L.wnode;
     const int fpw = FACES_PER_WG;
-    if (nvf > 0) {
+    const int nint = nvf + nhf;
+    if (L.flux_mode == 1 && nint > 0) {
+        const unsigned gw = (unsigned)((nint + SPLIT_FPW - 1) / SPLIT_FPW);
+        const unsigned gl = (unsigned)((nint + 255) / 256);
+        {
+            HookScope hs(L, "flux_sum");
+            hipLaunchKernelGGL((k_fsum<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+                               (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
+                               (const T*)L.sbx, (const T*)L.sby, un, vn, L.unode_d, L.vnode_d,
+                               L.wnode_d, L.acc_v, L.acc_h, L.fsum, nvf, nint);
+        }
+        {
+            HookScope hs(L, "flux_state");
+            hipLaunchKernelGGL((k_fstate), dim3(gl), dim3(256), 0, s, c, L.st, L.W, L.acc_v,
+                               L.acc_h, L.fsum, L.fcoef, nvf, nint);
+        }
+        {
+            HookScope hs(L, "flux_maxw");
+            hipLaunchKernelGGL((k_fmaxw<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+                               L.unode_d, L.vnode_d, L.wnode_d, L.acc_v, L.acc_h, L.fcoef,
+                               L.fmaxw, nvf, nint);
+        }
+        {
+            HookScope hs(L, "flux_slope_t");
+            hipLaunchKernelGGL((k_fslope_t), dim3(gl), dim3(256), 0, s, c, L.st, L.acc_v,
+                               L.acc_h, L.fmaxw, L.fcoef, L.vflux, L.hflux, nvf, nint);
+        }
+        {
+            HookScope hs(L, "flux_node");
+            hipLaunchKernelGGL((k_fnode<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+                               (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
+                               (const T*)L.sbx, (const T*)L.sby, un, vn, L.acc_v, L.acc_h,
+                               L.fcoef, (T*)L.vfh, (T*)L.vfb, (T*)L.hfh, (T*)L.hfb, nvf, nint);
+        }
+    } else if (nvf > 0) {
         HookScope hs(L, "flux_vface");
         hipLaunchKernelGGL((k_flux_interior<0, T>), dim3((unsigned)((nvf + fpw - 1) / fpw)),
                            dim3(WAVE * fpw), 0, s, c, L.st, L.W, (const T*)L.h, (const T*)L.b,
                            (const T*)L.shx, (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby, un,
                            vn, wn, L.acc_v, (T*)L.vfh, (T*)L.vfb, L.vflux, nvf);
     }
-    if (nhf > 0) {
+    if (L.flux_mode != 1 && nhf > 0) {
         HookScope hs(L, "flux_hface");
         hipLaunchKernelGGL((k_flux_interior<1, T>), dim3((unsigned)((nhf + fpw - 1) / fpw)),
                            dim3(WAVE * fpw), 0, s, c, L.st, L.W, (const T*)L.h, (const T*)L.b,

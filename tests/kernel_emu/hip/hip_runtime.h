@@ -28,6 +28,8 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(::emu::dyn_smem);
+// wave-uniform values only: the emulation has no scalar register file
+#define __builtin_amdgcn_readfirstlane(x) (x)
 
 struct dim3 {
     unsigned x, y, z;

@@ -99,3 +99,16 @@ def test_advance_until_output_time(have_gpu):
     o2 = oracle_py.Oracle(case)
     o2.step(8)
     assert np.all(field_err(w, o2.get()[0]) <= TOL_FP64)
+
+
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_flux_kernels_agree(have_gpu, monkeypatch, mode):
+    """Both interior-face flux implementations (one fused kernel; the
+    five-kernel pipeline) against the oracle, FP64 and mixed."""
+    monkeypatch.setenv("HUGKS_FLUX_KERNEL", mode)
+    for prec, tol in ((capi.FP64, TOL_FP64), (capi.MIXED_FP32, TOL_MIXED)):
+        case = cases.build_2d(rt(nx=10, ny=12, nv=16, nm=8), prec, rt_variant="cartesian", seed=5)
+        r = run_pair(case, 6)
+        err = field_err(r["w"], r["wr"])
+        print(mode, prec, "W err", err, flush=True)
+        assert np.all(err <= tol), err

@@ -34,7 +34,7 @@ def from_file(name):
 CASES = [
     ("sod40", from_file("p1d_sod_40.namelist"), 40, False),
     ("hydrostatic30", from_file("p1d_hydrostatic_30.namelist"), 34, True),
-    ("fourier_smoke", from_file("ref1d_fourier_smoke.namelist"), 20, False),
+    ("fourier_smoke", from_file("ref1d_fourier_smoke.namelist"), 2, False),
 ]
 
 
