@@ -388,6 +388,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         ALLOC(L.fcoef, (nint ? nint : 1) * FACE_COEF_LD * sizeof(double));
         ALLOC(L.fmaxw, (nint ? nint : 1) * FACE_MAXW_LD * sizeof(double));
         L.update_lds = update2d_lds_bytes(c, h->fp64);
+        L.update_mode = update2d_mfma_supported(c) ? 1 : 0;
+        if (const char* um = getenv("HUGKS_UPDATE_KERNEL"))
+            if (strcmp(um, "lds") == 0) L.update_mode = 0;
         hipDeviceProp_t prop;
         hipGetDeviceProperties(&prop, h->device);
         if (L.update_lds > (size_t)prop.sharedMemPerBlockOptin && prop.sharedMemPerBlockOptin > 0) {

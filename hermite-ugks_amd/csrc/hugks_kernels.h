@@ -27,6 +27,7 @@ struct Launch2D {
     const double *acc_cell, *acc_v, *acc_h;
     unsigned nwall;      // number of wall faces (non-periodic sides)
     int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline
+    int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles
     double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
     int update_nt;       // threads per update workgroup
     size_t update_lds;   // dynamic LDS bytes of the update kernel
@@ -68,5 +69,7 @@ hipError_t launch_convert_in(const double* src, void* dst, bool dst_fp64, size_t
 hipError_t launch_step1d(const Launch2D& L, hipStream_t s);
 
 size_t update2d_lds_bytes(const Consts& c, bool fp64);
+bool update2d_mfma_supported(const Consts& c);
+size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64);
 
 }  // namespace hugks

@@ -1,5 +1,6 @@
 // Host launchers for the kernels of k_update2d.h.
 #include "k_update2d.h"
+#include "k_update2d_mfma.h"
 
 namespace hugks {
 
@@ -43,7 +44,48 @@ static hipError_t launch_update_t(const Launch2D& L, hipStream_t s) {
     return hipGetLastError();
 }
 
+bool update2d_mfma_supported(const Consts& c) { return c.nv <= 48 && c.nm <= MF_NB - 2; }
+
+size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64) {
+    return mf_lds_elems(c.nv) * (fp64 ? sizeof(double) : sizeof(float));
+}
+
+template <typename T>
+static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s) {
+    const Consts& c = L.c;
+    const int ept = (c.nn + MF_NT - 1) / MF_NT;
+    const unsigned ncell = (unsigned)(c.nx * c.ny);
+    const size_t lds = update2d_mfma_lds_bytes(c, sizeof(T) == 8);
+    using A = typename Arith<T>::A;
+#define HUGKS_UPDM_CASE(E)                                                                      \
+    case E: {                                                                                   \
+        HookScope hs(L, "update2d");                                                            \
+        (void)hipFuncSetAttribute((const void*)k_update2d_mfma<T, E>,                           \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);        \
+        hipLaunchKernelGGL((k_update2d_mfma<T, E>), dim3(ncell), dim3(MF_NT), lds, s, c, L.st,  \
+            L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,   \
+            (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.wnode, L.wnode_d, L.u1, L.v1,        \
+            L.acc_cell);                                                                        \
+        break;                                                                                  \
+    }
+    switch (ept) {
+        HUGKS_UPDM_CASE(1)
+        HUGKS_UPDM_CASE(2)
+        HUGKS_UPDM_CASE(3)
+        HUGKS_UPDM_CASE(4)
+        HUGKS_UPDM_CASE(5)
+        HUGKS_UPDM_CASE(6)
+        HUGKS_UPDM_CASE(7)
+        HUGKS_UPDM_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef HUGKS_UPDM_CASE
+    return hipGetLastError();
+}
+
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
+    if (L.update_mode == 1)
+        return L.fp64 ? launch_update_mfma_t<double>(L, s) : launch_update_mfma_t<float>(L, s);
     return L.fp64 ? launch_update_t<double>(L, s) : launch_update_t<float>(L, s);
 }
 

@@ -9,6 +9,11 @@
 
 namespace emu {
 dim3 tid, bid, bdim;
+Fiber* fibers_base = nullptr;
+unsigned nthreads = 0;
+Bar block_bar = {0, 0};
+Bar wave_bar[16] = {};
+double mfa[1024], mfb[1024];
 unsigned char* dyn_smem = nullptr;
 unsigned char xbuf[1024 * 8];
 long launches = 0;
@@ -43,6 +48,31 @@ void yield_to_sched() {
 #endif
 }
 
+static unsigned live_in(unsigned first, unsigned count) {
+    unsigned n = 0;
+    for (unsigned t = first; t < first + count && t < nthreads; ++t) n += !fibers_base[t].done;
+    return n;
+}
+
+void barrier(Bar& b, unsigned first, unsigned count) {
+    const unsigned g = b.gen;
+    b.arrived += 1;
+    for (;;) {
+        if (b.gen != g) return;
+        if (b.arrived >= live_in(first, count)) {
+            b.arrived = 0;
+            b.gen += 1;
+            return;
+        }
+        yield_to_sched();
+    }
+}
+
+void wave_barrier() {
+    const unsigned w = tid.x / 64;
+    barrier(wave_bar[w], 64 * w, 64);
+}
+
 void fiber_entry() {
 #ifdef EMU_ASAN
     __sanitizer_finish_switch_fiber(nullptr, nullptr, nullptr);
@@ -56,6 +86,8 @@ void fiber_entry() {
 }
 
 void run_block(Fiber* fibers, unsigned nt) {
+    block_bar = {0, 0};
+    for (auto& wb : wave_bar) wb = {0, 0};
     for (unsigned t = 0; t < nt; ++t) {
         Fiber& f = fibers[t];
         f.done = false;

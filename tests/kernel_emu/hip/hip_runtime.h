@@ -49,6 +49,19 @@ struct Fiber {
     dim3 t;
 };
 extern dim3 tid, bid, bdim;
+extern Fiber* fibers_base;
+extern unsigned nthreads;
+// Counting barrier over the live fibers [first, first + count): a fiber
+// that arrived waits (yielding) until every live fiber of the range has
+// arrived; fibers that returned no longer count.
+struct Bar {
+    unsigned arrived;
+    unsigned gen;
+};
+extern Bar block_bar;
+extern Bar wave_bar[16];
+void barrier(Bar& b, unsigned first, unsigned count);
+void wave_barrier();
 extern unsigned char* dyn_smem;
 extern unsigned char xbuf[1024 * 8];
 extern long launches;
@@ -93,6 +106,8 @@ void launch(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
                    (unsigned)(blk / ((unsigned long)g.x * g.y)));
         for (unsigned t = 0; t < nt; ++t)
             fibers[t].t = dim3(t % b.x, (t / b.x) % b.y, t / (b.x * b.y));
+        fibers_base = fibers.data();
+        nthreads = nt;
         run_block(fibers.data(), nt);
     }
     for (auto& f : fibers) free(f.stack);
@@ -104,30 +119,64 @@ void launch(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
 #define blockIdx (::emu::bid)
 #define blockDim (::emu::bdim)
 
-inline void __syncthreads() { emu::yield_to_sched(); }
+inline void __syncthreads() { emu::barrier(emu::block_bar, 0, emu::nthreads); }
 
+// Wave-level operations exchange through xbuf between two wave barriers
+// (every lane of the wavefront takes part, as on the hardware).
 template <class T>
 inline T __shfl_xor(T v, int mask, int width = 64) {
     static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
     const unsigned t = threadIdx.x;
     std::memcpy(emu::xbuf + 8 * t, &v, sizeof(T));
-    __syncthreads();
+    emu::wave_barrier();
     const unsigned src = (t & ~(unsigned)(width - 1)) | ((t & (unsigned)(width - 1)) ^ (unsigned)mask);
     T r;
     std::memcpy(&r, emu::xbuf + 8 * src, sizeof(T));
-    __syncthreads();
+    emu::wave_barrier();
     return r;
 }
 template <class T>
 inline T __shfl(T v, int lane, int width = 64) {
     const unsigned t = threadIdx.x;
     std::memcpy(emu::xbuf + 8 * t, &v, sizeof(T));
-    __syncthreads();
+    emu::wave_barrier();
     const unsigned src = (t & ~(unsigned)(width - 1)) | ((unsigned)lane & (unsigned)(width - 1));
     T r;
     std::memcpy(&r, emu::xbuf + 8 * src, sizeof(T));
-    __syncthreads();
+    emu::wave_barrier();
     return r;
+}
+
+// 16x16x4 matrix tiles, lane layouts as on gfx950: A(i,k) from lane
+// 16k+i, B(k,j) from lane 16k+j; result element v of lane l is
+// C(4*(l/16)+v, l%16) for f32 and C(l/16+4*v, l%16) for f64.
+namespace emu {
+extern double mfa[1024], mfb[1024];
+template <class V, class S>
+inline V mfma16x16x4(S a, S b, V c, bool f64) {
+    const unsigned t = threadIdx.x, w0 = t & ~63u, l = t & 63u;
+    mfa[t] = (double)a;
+    mfb[t] = (double)b;
+    wave_barrier();
+    V r = c;
+    for (int v = 0; v < 4; ++v) {
+        const unsigned i = f64 ? (l >> 4) + 4 * v : 4 * (l >> 4) + v, j = l & 15;
+        S acc = c[v];
+        for (unsigned k = 0; k < 4; ++k)
+            acc = (S)std::fma((S)mfa[w0 + 16 * k + i], (S)mfb[w0 + 16 * k + j], acc);
+        r[v] = acc;
+    }
+    wave_barrier();
+    return r;
+}
+}  // namespace emu
+typedef float emu_f4 __attribute__((vector_size(4 * sizeof(float))));
+typedef double emu_d4 __attribute__((vector_size(4 * sizeof(double))));
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, emu_f4 c, int, int, int) {
+    return emu::mfma16x16x4<emu_f4, float>(a, b, c, false);
+}
+inline emu_d4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, emu_d4 c, int, int, int) {
+    return emu::mfma16x16x4<emu_d4, double>(a, b, c, true);
 }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 
