@@ -170,6 +170,28 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
     }
 }
 
+// ------------------------------------------------- XCD-aware cell order
+// The workgroups of one launch are dealt to the eight XCDs round-robin
+// (workgroup b to XCD b % 8), and each XCD has its own L2.  xcd_block
+// renumbers workgroups so that every XCD runs one contiguous range of the
+// logical order (a bijection on [0, nb)); tiled_cell walks the cells in
+// strips of TILE_W columns, row by row inside a strip, so the neighbour
+// cells a kernel reads (i +- 1 and j +- 1) were touched a few dozen cells
+// earlier on the same XCD and are still in its L2.
+constexpr int NXCD = 8;
+constexpr int TILE_W = 32;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    const unsigned q = nb / NXCD, r = nb % NXCD, x = b % NXCD;
+    return x * q + (x < r ? x : r) + b / NXCD;
+}
+__device__ __forceinline__ int tiled_cell(int L, int nx, int ny) {
+    const int s = L / (TILE_W * ny);
+    const int rem = L - s * TILE_W * ny;
+    const int ws = nx - s * TILE_W < TILE_W ? nx - s * TILE_W : TILE_W;
+    const int jj = rem / ws, ii = rem - jj * ws;
+    return s * TILE_W + ii + nx * jj;
+}
+
 template <typename T> struct Arith;           // per-node arithmetic type
 template <> struct Arith<double> { using A = double; };
 template <> struct Arith<float> { using A = float; };

@@ -18,11 +18,12 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
                                                  T* __restrict__ sbx, T* __restrict__ sby) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
-    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t idx = (size_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const size_t ncell = (size_t)c.nx * c.ny;
     if (idx >= ncell * c.ldv) return;
-    const int cell = (int)(idx / c.ldv);
-    const int k = (int)(idx - (size_t)cell * c.ldv);
+    const int lc = (int)(idx / c.ldv);
+    const int k = (int)(idx - (size_t)lc * c.ldv);
+    const int cell = tiled_cell(lc, c.nx, c.ny);
     const int i = cell % c.nx, j = cell / c.nx;
     const A smv = (A)c.smv;
 #pragma unroll

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(NT) void k_update2d(
     __shared__ double red[4 * (NT / WAVE)];
     if (st->stopped) return;
     const int tid = threadIdx.x;
-    const int cell = blockIdx.x;
+    const int cell = tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
     const int nx = c.nx, nv = c.nv, nm = c.nm, nn = c.nn;
     const int i = cell % nx, j = cell / nx;
     const double dt = st->dt;

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     __shared__ double red[4 * (MF_NT / WAVE)];
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    const int cell = blockIdx.x;
+    const int cell = tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
     const int nx = c.nx, nv = c.nv, nm = c.nm, nn = c.nn;
     const int np = mf_np(nv);
     const int i = cell % nx, j = cell / nx;

@@ -8,7 +8,7 @@
 #endif
 
 namespace emu {
-dim3 tid, bid, bdim;
+dim3 tid, bid, bdim, gdim;
 Fiber* fibers_base = nullptr;
 unsigned nthreads = 0;
 Bar block_bar = {0, 0};

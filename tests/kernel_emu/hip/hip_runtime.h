@@ -48,7 +48,7 @@ struct Fiber {
     bool done;
     dim3 t;
 };
-extern dim3 tid, bid, bdim;
+extern dim3 tid, bid, bdim, gdim;
 extern Fiber* fibers_base;
 extern unsigned nthreads;
 // Counting barrier over the live fibers [first, first + count): a fiber
@@ -94,6 +94,7 @@ void launch(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
     unsigned char* d = (unsigned char*)malloc(shmem ? shmem : 1);
     dyn_smem = d;
     bdim = b;
+    gdim = g;
     std::vector<Fiber> fibers(nt);
     const size_t ss = 256 * 1024;
     for (auto& f : fibers) {
@@ -118,6 +119,7 @@ void launch(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
 #define threadIdx (::emu::tid)
 #define blockIdx (::emu::bid)
 #define blockDim (::emu::bdim)
+#define gridDim (::emu::gdim)
 
 inline void __syncthreads() { emu::barrier(emu::block_bar, 0, emu::nthreads); }
 
