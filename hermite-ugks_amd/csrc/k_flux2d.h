@@ -18,54 +18,62 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
                                                  T* __restrict__ sbx, T* __restrict__ sby) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
-    const size_t idx = (size_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const size_t ncell = (size_t)c.nx * c.ny;
-    if (idx >= ncell * c.ldv) return;
-    const int lc = (int)(idx / c.ldv);
-    const int k = (int)(idx - (size_t)lc * c.ldv);
-    const int cell = tiled_cell(lc, c.nx, c.ny);
+    // one workgroup per cell, cells in XCD-aware strip order; the neighbour
+    // cells of both directions are fixed per workgroup
+    const int cell = tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
     const int i = cell % c.nx, j = cell / c.nx;
     const A smv = (A)c.smv;
+    int cl[2], cr[2];
+    bool inner[2];
+    A inv_den[2];
 #pragma unroll
     for (int dir = 0; dir < 2; ++dir) {
         const bool per = c.bc[2 * dir] == HUGKS_BC_PERIODIC;
         const int n = dir == 0 ? c.nx : c.ny;
         const int p = dir == 0 ? i : j;
         const A len = (A)(dir == 0 ? c.dx : c.dy);
-        int cl, cr;
-        bool inner = true;
+        inner[dir] = true;
         if (per) {
             const int pl = (p - 1 + n) % n, pr = (p + 1) % n;
-            cl = dir == 0 ? pl + c.nx * j : i + c.nx * pl;
-            cr = dir == 0 ? pr + c.nx * j : i + c.nx * pr;
+            cl[dir] = dir == 0 ? pl + c.nx * j : i + c.nx * pl;
+            cr[dir] = dir == 0 ? pr + c.nx * j : i + c.nx * pr;
         } else if (p == 0) {
-            cl = cell;
-            cr = dir == 0 ? cell + 1 : cell + c.nx;
-            inner = false;
+            cl[dir] = cell;
+            cr[dir] = dir == 0 ? cell + 1 : cell + c.nx;
+            inner[dir] = false;
         } else if (p == n - 1) {
-            cl = dir == 0 ? cell - 1 : cell - c.nx;
-            cr = cell;
-            inner = false;
+            cl[dir] = dir == 0 ? cell - 1 : cell - c.nx;
+            cr[dir] = cell;
+            inner[dir] = false;
         } else {
-            cl = dir == 0 ? cell - 1 : cell - c.nx;
-            cr = dir == 0 ? cell + 1 : cell + c.nx;
+            cl[dir] = dir == 0 ? cell - 1 : cell - c.nx;
+            cr[dir] = dir == 0 ? cell + 1 : cell + c.nx;
         }
-        const A den = (A)0.5 * len + (A)0.5 * len;
+        inv_den[dir] = (A)1 / ((A)0.5 * len + (A)0.5 * len);
+    }
+    const size_t o = (size_t)cell * c.ldv;
+    const size_t oL[2] = {(size_t)cl[0] * c.ldv, (size_t)cl[1] * c.ldv};
+    const size_t oR[2] = {(size_t)cr[0] * c.ldv, (size_t)cr[1] * c.ldv};
+    for (int k = threadIdx.x; k < c.nn; k += 256) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const T* f = q == 0 ? h : b;
-            const A fl = (A)f[(size_t)cl * c.ldv + k], fr = (A)f[(size_t)cr * c.ldv + k];
-            A s;
-            if (inner) {
-                const A fn = (A)f[(size_t)cell * c.ldv + k];
-                const A sL = (fn - fl) / den, sR = (fr - fn) / den;
-                s = (copysign((A)1, sR) + copysign((A)1, sL)) * fabs(sR) * fabs(sL) /
-                    (fabs(sR) + fabs(sL) + smv);
-            } else {
-                s = (fr - fl) / den;
+            const A fn = (A)f[o + k];
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {
+                const A fl = (A)f[oL[dir] + k], fr = (A)f[oR[dir] + k];
+                A s;
+                if (inner[dir]) {
+                    // van Leer limiter (2d/src/solver.f90:146-151)
+                    const A sL = (fn - fl) * inv_den[dir], sR = (fr - fn) * inv_den[dir];
+                    s = (copysign((A)1, sR) + copysign((A)1, sL)) * fabs(sR) * fabs(sL) /
+                        (fabs(sR) + fabs(sL) + smv);
+                } else {
+                    s = (fr - fl) * inv_den[dir];
+                }
+                T* out = dir == 0 ? (q == 0 ? shx : sbx) : (q == 0 ? shy : sby);
+                out[o + k] = (T)s;
             }
-            T* out = dir == 0 ? (q == 0 ? shx : sbx) : (q == 0 ? shy : sby);
-            out[(size_t)cell * c.ldv + k] = (T)s;
         }
     }
 }

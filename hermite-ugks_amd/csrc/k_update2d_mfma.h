@@ -62,7 +62,8 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb, const double* __restrict__ vflux,
     const double* __restrict__ hflux, const typename Arith<T>::A* __restrict__ wnode,
-    const double* __restrict__ wnode_d, const double* __restrict__ u1,
+    const double* __restrict__ wnode_d, const double* __restrict__ und,
+    const double* __restrict__ vnd, const double* __restrict__ u1,
     const double* __restrict__ v1, const double* __restrict__ acc_cell) {
     using A = typename Arith<T>::A;
     using M = Mfma<T>;
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             rh[e] = h[o];
             rb[e] = b[o];
             const double wt = wnode_d[k];
-            const double cu = u1[k % nv] - po[1], cv = v1[k / nv] - po[2];
+            const double cu = und[k] - po[1], cv = vnd[k] - po[2];
             const double c2 = cu * cu + cv * cv;
             q4[0] += wt * cu * c2 * (double)rh[e];
             q4[1] += wt * cu * (double)rb[e];
@@ -248,18 +249,18 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     const double csn = c.shk * (1.0 - c.prandtl) * pn[3] * pn[3] / pn[0];
     const double half_dt = 0.5 * dt;
     const double itau = 1.0 / tau, itau_o = 1.0 / tau_old;
-    const double den = 1.0 + 0.5 * dt / tau;
+    const double iden = 1.0 / (1.0 + 0.5 * dt / tau);
     const double iarea = 1.0 / area;
+    const A Uo = (A)po[1], Vo = (A)po[2];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = tid + e * MF_NT;
         if (k < nn) {
-            const int iu = k % nv, jv = k / nv;
-            const A xi = (A)Hx[np + iu], eta = (A)Hy[np + jv];
+            const double u = und[k], v = vnd[k];
+            const A xi = ((A)u - Uo) * inv_sqrt_rt, eta = ((A)v - Vo) * inv_sqrt_rt;
             const A om = fexp((A)-0.5 * (xi * xi + eta * eta)) / two_pi;
             const double Fh = (double)(-inv_sqrt_rt * om * (A)F[k]);
             const double Fb = (double)(-inv_sqrt_rt * om * (A)F[nn + k]);
-            const double u = u1[iu], v = v1[jv];
             const double cuo = u - po[1], cvo = v - po[2], c2o = cuo * cuo + cvo * cvo;
             const double Ho = Hco * exp(-po[3] * c2o), Bo = Ho * kfo;
             const double cqo = cso * (cuo * qf0 + cvo * qf1);
@@ -275,8 +276,8 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             const double dfb = ((double)vfb[(size_t)vl * c.ldv + k] - (double)vfb[(size_t)vr * c.ldv + k] +
                                 (double)hfb[(size_t)hd * c.ldv + k] - (double)hfb[(size_t)hu * c.ldv + k]) * iarea;
             const double ho = (double)rh[e], bo = (double)rb[e];
-            const double hn = (ho + dfh + half_dt * (Hpn * itau + (Hpo - ho) * itau_o) - Fh * dt) / den;
-            const double bn = (bo + dfb + half_dt * (Bpn * itau + (Bpo - bo) * itau_o) - Fb * dt) / den;
+            const double hn = (ho + dfh + half_dt * (Hpn * itau + (Hpo - ho) * itau_o) - Fh * dt) * iden;
+            const double bn = (bo + dfb + half_dt * (Bpn * itau + (Bpo - bo) * itau_o) - Fb * dt) * iden;
             const size_t o = (size_t)cell * c.ldv + k;
             h[o] = (T)hn;
             b[o] = (T)bn;

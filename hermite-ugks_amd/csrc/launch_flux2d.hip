@@ -11,8 +11,7 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     const size_t ncell = (size_t)c.nx * c.ny;
     {
         HookScope hs(L, "slope2d");
-        const size_t n = ncell * c.ldv;
-        hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c,
+        hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)ncell), dim3(256), 0, s, c,
                            L.st, (const T*)L.h, (const T*)L.b, (T*)L.shx, (T*)L.shy, (T*)L.sbx,
                            (T*)L.sby);
     }

@@ -64,8 +64,8 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);        \
         hipLaunchKernelGGL((k_update2d_mfma<T, E>), dim3(ncell), dim3(MF_NT), lds, s, c, L.st,  \
             L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,   \
-            (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.wnode, L.wnode_d, L.u1, L.v1,        \
-            L.acc_cell);                                                                        \
+            (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.wnode, L.wnode_d, L.unode_d,         \
+            L.vnode_d, L.u1, L.v1, L.acc_cell);                                                 \
         break;                                                                                  \
     }
     switch (ept) {
