@@ -8,6 +8,9 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <tuple>
+#include <mutex>
+#include <map>
 
 #include "hugks.h"
 #include "hugks_device.h"
@@ -136,24 +139,28 @@ static void timing_hook(void* ctx, int phase, const char* name) {
     }
 }
 
-static hipError_t enqueue_step(hugks_handle* h) {
+// One reference step of h on stream s (the handle's own stream, or a
+// stream being captured into a batch graph: then without timing events).
+static hipError_t enqueue_step_on(hugks_handle* h, hipStream_t s, bool timed) {
     hipError_t e;
     Launch2D& L = h->L;
-    L.hook = h->timing ? timing_hook : nullptr;
+    L.hook = timed && h->timing ? timing_hook : nullptr;
     L.hook_ctx = h;
-    e = launch_timestep(L, h->dim, h->stream);
+    e = launch_timestep(L, h->dim, s);
     if (e != hipSuccess) return e;
     if (h->dim == 2) {
-        e = launch_flux2d(L, h->stream);
+        e = launch_flux2d(L, s);
         if (e != hipSuccess) return e;
-        e = launch_update2d(L, h->stream);
+        e = launch_update2d(L, s);
         if (e != hipSuccess) return e;
     } else {
-        e = launch_step1d(L, h->stream);
+        e = launch_step1d(L, s);
         if (e != hipSuccess) return e;
     }
-    return launch_finalize(L, h->dim, h->stream);
+    return launch_finalize(L, h->dim, s);
 }
+
+static hipError_t enqueue_step(hugks_handle* h) { return enqueue_step_on(h, h->stream, true); }
 
 static int read_state(hugks_handle* h, hugks_step_info* info) {
     DevState s;
@@ -398,6 +405,13 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
             delete h;
             return fail(nullptr, HUGKS_E_UNSUPPORTED, "Hermite working set exceeds LDS");
         }
+        e = prepare_update2d(L);
+        if (e != hipSuccess) {
+            int rc = hip_fail(nullptr, e, "update kernel attributes");
+            release(h);
+            delete h;
+            return rc;
+        }
     }
 #undef ALLOC
     // zero everything that pads or is read before written, then the
@@ -633,6 +647,269 @@ int hugks_set_timing(hugks_handle* h, int32_t on) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
     h->timing = on != 0;
     return HUGKS_OK;
+}
+
+
+// ----------------------------------------------------------- schedules
+
+int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kernel) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_SPLIT ||
+        update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA)
+        return fail(h, HUGKS_E_ARG, "unknown kernel choice");
+    if (h->dim != 2) return HUGKS_OK;   // 1D has one kernel of each kind
+    if (update_kernel == HUGKS_UPDATE_MFMA && !update2d_mfma_supported(h->L.c))
+        return fail(h, HUGKS_E_UNSUPPORTED, "MFMA update needs nv <= 48 and Nm <= 14");
+    h->L.flux_mode = flux_kernel;
+    h->L.update_mode = update_kernel;
+    return HUGKS_OK;
+}
+
+int hugks_get_schedule(hugks_handle* h, int32_t* flux_kernel, int32_t* update_kernel) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    if (flux_kernel) *flux_kernel = h->dim == 2 ? h->L.flux_mode : HUGKS_FLUX_FUSED;
+    if (update_kernel) *update_kernel = h->dim == 2 ? h->L.update_mode : HUGKS_UPDATE_LDS;
+    return HUGKS_OK;
+}
+
+}  // extern "C"
+
+// Autotuner: for one case shape, time every kernel schedule on a scratch
+// handle of that shape (a few steps after a warm-up, HIP events on the
+// handle's stream) and keep the fastest.  Results are cached per shape for
+// the life of the process.
+namespace {
+struct TuneKey {
+    int dim, nx, ny, nv, nm, precision, bc0, bc2;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(dim, nx, ny, nv, nm, precision, bc0, bc2) <
+               std::tie(o.dim, o.nx, o.ny, o.nv, o.nm, o.precision, o.bc0, o.bc2);
+    }
+};
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+std::mutex g_tune_mu;
+}  // namespace
+
+static int tune_case(const hugks_desc* d, int* flux_kernel, int* update_kernel, double* best_ms) {
+    const TuneKey key{d->dim, d->nx, d->ny, d->nv, d->nm, d->precision, d->bc[0], d->bc[2]};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            *flux_kernel = it->second.first;
+            *update_kernel = it->second.second;
+            if (best_ms) *best_ms = -1.0;
+            return HUGKS_OK;
+        }
+    }
+    hugks_handle* h = nullptr;
+    int rc = hugks_create(d, &h);
+    if (rc != HUGKS_OK) return rc;
+    int bf = h->L.flux_mode, bu = h->L.update_mode;
+    double best = 1e300;
+    if (d->dim == 2) {
+        for (int fk = HUGKS_FLUX_FUSED; fk <= HUGKS_FLUX_SPLIT; ++fk)
+            for (int uk = HUGKS_UPDATE_LDS; uk <= HUGKS_UPDATE_MFMA; ++uk) {
+                if (hugks_set_schedule(h, fk, uk) != HUGKS_OK) continue;
+                hipError_t e = launch_begin(h->L.st, INFINITY, h->stream);
+                for (int s = 0; s < 2 && e == hipSuccess; ++s) e = enqueue_step_on(h, h->stream, false);
+                if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+                const int nrep = 4;
+                for (int s = 0; s < nrep && e == hipSuccess; ++s)
+                    e = enqueue_step_on(h, h->stream, false);
+                if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+                if (e != hipSuccess) {
+                    rc = hip_fail(nullptr, e, "autotune");
+                    hugks_destroy(h);
+                    return rc;
+                }
+                if (ms < best) {
+                    best = ms;
+                    bf = fk;
+                    bu = uk;
+                }
+            }
+    }
+    hugks_destroy(h);
+    *flux_kernel = bf;
+    *update_kernel = bu;
+    if (best_ms) *best_ms = best < 1e300 ? best / 4 : -1.0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = {bf, bu};
+    return HUGKS_OK;
+}
+
+extern "C" {
+
+int hugks_tune(const hugks_desc* d, int32_t* flux_kernel, int32_t* update_kernel,
+               double* ms_per_step) {
+    if (!d || !flux_kernel || !update_kernel)
+        return fail(nullptr, HUGKS_E_ARG, "null descriptor or output");
+    int fk = 0, uk = 0;
+    int rc = tune_case(d, &fk, &uk, ms_per_step);
+    if (rc != HUGKS_OK) return rc;
+    *flux_kernel = fk;
+    *update_kernel = uk;
+    return HUGKS_OK;
+}
+
+// ------------------------------------------------------------ batches
+
+}  // extern "C"
+
+struct hugks_batch {
+    std::vector<hugks_handle*> cases;
+    hipStream_t main = nullptr;
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string err;
+};
+
+static void batch_release(hugks_batch* b) {
+    if (!b) return;
+    if (b->main) hipStreamSynchronize(b->main);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (hipEvent_t e : b->joins) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
+    for (hipStream_t s : b->lanes) hipStreamDestroy(s);
+    if (b->main) hipStreamDestroy(b->main);
+    for (hugks_handle* h : b->cases) hugks_destroy(h);
+    delete b;
+}
+
+static int batch_fail(hugks_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+extern "C" {
+
+int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_batch** out) {
+    if (n <= 0 || !descs || !out) return batch_fail(nullptr, HUGKS_E_ARG, "empty batch or null pointer");
+    *out = nullptr;
+    hugks_batch* b = new hugks_batch();
+    for (int32_t i = 0; i < n; ++i) {
+        int fk = -1, uk = -1;
+        if (flags & HUGKS_BATCH_AUTOTUNE) {
+            int rc = tune_case(&descs[i], &fk, &uk, nullptr);
+            if (rc != HUGKS_OK) {
+                std::string m = "case " + std::to_string(i) + ": " + g_create_error;
+                batch_release(b);
+                return batch_fail(nullptr, rc, m);
+            }
+        }
+        hugks_handle* h = nullptr;
+        int rc = hugks_create(&descs[i], &h);
+        if (rc != HUGKS_OK) {
+            std::string m = "case " + std::to_string(i) + ": " + g_create_error;
+            batch_release(b);
+            return batch_fail(nullptr, rc, m);
+        }
+        b->cases.push_back(h);
+        if (fk >= 0) hugks_set_schedule(h, fk, uk);
+    }
+    // one step of every case, fanned out over a few streams and captured
+    // into one graph: a replay advances the whole batch by one step
+    hipError_t e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking);
+    const int nl = n < 4 ? n : 4;
+    for (int l = 0; l < nl && e == hipSuccess; ++l) {
+        hipStream_t s = nullptr;
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess) b->lanes.push_back(s);
+        hipEvent_t ev = nullptr;
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->joins.push_back(ev);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    // cases finish their own set-up on their streams first
+    for (hugks_handle* h : b->cases)
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = hipEventRecord(b->fork, b->main);
+    for (int l = 0; l < nl && e == hipSuccess; ++l) e = hipStreamWaitEvent(b->lanes[l], b->fork, 0);
+    for (int32_t i = 0; i < n && e == hipSuccess; ++i)
+        e = enqueue_step_on(b->cases[i], b->lanes[i % nl], false);
+    for (int l = 0; l < nl && e == hipSuccess; ++l) {
+        e = hipEventRecord(b->joins[l], b->lanes[l]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->joins[l], 0);
+    }
+    if (e == hipSuccess) {
+        e = hipStreamEndCapture(b->main, &b->graph);
+    } else {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(b->main, &g);
+        if (g) hipGraphDestroy(g);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        std::string m = std::string("batch graph: ") + hipGetErrorString(e);
+        batch_release(b);
+        return batch_fail(nullptr, e == hipErrorOutOfMemory ? HUGKS_E_NOMEM : HUGKS_E_DEVICE, m);
+    }
+    *out = b;
+    return HUGKS_OK;
+}
+
+int hugks_batch_destroy(hugks_batch* b) {
+    batch_release(b);
+    return HUGKS_OK;
+}
+
+const char* hugks_batch_last_error(const hugks_batch* b) {
+    return b ? b->err.c_str() : g_create_error.c_str();
+}
+
+int32_t hugks_batch_size(const hugks_batch* b) { return b ? (int32_t)b->cases.size() : 0; }
+
+int hugks_batch_handle(hugks_batch* b, int32_t i, hugks_handle** h) {
+    if (!b || !h || i < 0 || i >= (int32_t)b->cases.size())
+        return batch_fail(b, HUGKS_E_ARG, "case index out of range");
+    *h = b->cases[i];
+    return HUGKS_OK;
+}
+
+int hugks_batch_advance(hugks_batch* b, int32_t max_steps, hugks_step_info* infos) {
+    if (!b) return batch_fail(nullptr, HUGKS_E_ARG, "null batch");
+    const size_t n = b->cases.size();
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < n && e == hipSuccess; ++i)
+        e = launch_begin(b->cases[i]->L.st, INFINITY, b->main);
+    if (max_steps > 0) {
+        for (int32_t s = 0; s < max_steps && e == hipSuccess; ++s) e = hipGraphLaunch(b->exec, b->main);
+    } else {
+        // no cap: replay until every case met an exit condition
+        std::vector<DevState> st(n);
+        for (;;) {
+            for (int s = 0; s < 32 && e == hipSuccess; ++s) e = hipGraphLaunch(b->exec, b->main);
+            for (size_t i = 0; i < n && e == hipSuccess; ++i)
+                e = hipMemcpyAsync(&st[i], b->cases[i]->L.st, sizeof(DevState),
+                                   hipMemcpyDeviceToHost, b->main);
+            if (e == hipSuccess) e = hipStreamSynchronize(b->main);
+            if (e != hipSuccess) break;
+            bool all = true;
+            for (size_t i = 0; i < n; ++i) all = all && (st[i].stopped || st[i].flags);
+            if (all) break;
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->main);
+    if (e != hipSuccess) return batch_fail(b, HUGKS_E_DEVICE, std::string("batch advance: ") + hipGetErrorString(e));
+    int first = HUGKS_OK;
+    for (size_t i = 0; i < n; ++i) {
+        int rc = read_state(b->cases[i], infos ? &infos[i] : nullptr);
+        if (rc != HUGKS_OK && first == HUGKS_OK) {
+            first = rc;
+            b->err = "case " + std::to_string(i) + ": " + b->cases[i]->err;
+        }
+    }
+    return first;
 }
 
 }  // extern "C"

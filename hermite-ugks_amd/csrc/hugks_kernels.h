@@ -57,6 +57,7 @@ hipError_t launch_begin(DevState* st, double t_stop, hipStream_t s);
 hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s);
 hipError_t launch_flux2d(const Launch2D& L, hipStream_t s);
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s);
+hipError_t prepare_update2d(const Launch2D& L);
 hipError_t launch_finalize(const Launch2D& L, int dim, hipStream_t s);
 hipError_t launch_init_f(const Launch2D& L, int dim, const double* prim0, hipStream_t s);
 hipError_t launch_macro(const Launch2D& L, int dim, double* q, hipStream_t s);

@@ -12,7 +12,7 @@ size_t update2d_lds_bytes(const Consts& c, bool fp64) {
 }
 
 template <typename T>
-static hipError_t launch_update_t(const Launch2D& L, hipStream_t s) {
+static hipError_t launch_update_t(const Launch2D& L, hipStream_t s, bool prepare) {
     const Consts& c = L.c;
     constexpr int NT = 256;
     const int ept = (c.ldv + NT - 1) / NT;
@@ -20,9 +20,11 @@ static hipError_t launch_update_t(const Launch2D& L, hipStream_t s) {
     using A = typename Arith<T>::A;
 #define HUGKS_UPD_CASE(E)                                                                       \
     case E: {                                                                                   \
+        if (prepare)                                                                            \
+            return hipFuncSetAttribute((const void*)k_update2d<T, E, NT>,                       \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,              \
+                                       (int)L.update_lds);                                      \
         HookScope hs(L, "update2d");                                                            \
-        (void)hipFuncSetAttribute((const void*)k_update2d<T, E, NT>,                            \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.update_lds);     \
         hipLaunchKernelGGL((k_update2d<T, E, NT>), dim3(ncell), dim3(NT), L.update_lds, s, \
             c, L.st, L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,           \
             (const T*)L.hfh, (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.unode,              \
@@ -51,7 +53,7 @@ size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64) {
 }
 
 template <typename T>
-static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s) {
+static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool prepare) {
     const Consts& c = L.c;
     const int ept = (c.nn + MF_NT - 1) / MF_NT;
     const unsigned ncell = (unsigned)(c.nx * c.ny);
@@ -59,9 +61,10 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s) {
     using A = typename Arith<T>::A;
 #define HUGKS_UPDM_CASE(E)                                                                      \
     case E: {                                                                                   \
+        if (prepare)                                                                            \
+            return hipFuncSetAttribute((const void*)k_update2d_mfma<T, E>,                      \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
         HookScope hs(L, "update2d");                                                            \
-        (void)hipFuncSetAttribute((const void*)k_update2d_mfma<T, E>,                           \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);        \
         hipLaunchKernelGGL((k_update2d_mfma<T, E>), dim3(ncell), dim3(MF_NT), lds, s, c, L.st,  \
             L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,   \
             (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.wnode, L.wnode_d, L.unode_d,         \
@@ -85,8 +88,20 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s) {
 
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
     if (L.update_mode == 1)
-        return L.fp64 ? launch_update_mfma_t<double>(L, s) : launch_update_mfma_t<float>(L, s);
-    return L.fp64 ? launch_update_t<double>(L, s) : launch_update_t<float>(L, s);
+        return L.fp64 ? launch_update_mfma_t<double>(L, s, false)
+                      : launch_update_mfma_t<float>(L, s, false);
+    return L.fp64 ? launch_update_t<double>(L, s, false) : launch_update_t<float>(L, s, false);
+}
+
+// Dynamic-LDS limits of both update kernels of this case, set once before
+// any launch (so that no launch, captured into a graph or not, needs to).
+hipError_t prepare_update2d(const Launch2D& L) {
+    hipError_t e = L.fp64 ? launch_update_t<double>(L, nullptr, true)
+                          : launch_update_t<float>(L, nullptr, true);
+    if (e == hipSuccess && update2d_mfma_supported(L.c))
+        e = L.fp64 ? launch_update_mfma_t<double>(L, nullptr, true)
+                   : launch_update_mfma_t<float>(L, nullptr, true);
+    return e;
 }
 
 }  // namespace hugks

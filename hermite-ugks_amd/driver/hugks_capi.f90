@@ -27,6 +27,8 @@ module hugks_capi
        HUGKS_FORCE_HERMITE_SYMMETRIC_B = 1, HUGKS_FORCE_HERMITE_LEGACY = 2
   integer(c_int32_t), parameter :: HUGKS_STOP_NONE = 0, HUGKS_STOP_MAXTIME = 1, &
        HUGKS_STOP_RESIDUAL = 2, HUGKS_STOP_TIME = 3, HUGKS_STOP_STEADY = 4
+  integer(c_int32_t), parameter :: HUGKS_FLUX_FUSED = 0, HUGKS_FLUX_SPLIT = 1, &
+       HUGKS_UPDATE_LDS = 0, HUGKS_UPDATE_MFMA = 1, HUGKS_BATCH_AUTOTUNE = 1
 
   type, bind(c) :: hugks_desc
     integer(c_int32_t) :: abi_version = HUGKS_ABI_VERSION
@@ -131,6 +133,50 @@ module hugks_capi
     type(c_ptr) function hugks_device_info_c() bind(c, name='hugks_device_info')
       import :: c_ptr
     end function hugks_device_info_c
+
+    integer(c_int) function hugks_set_schedule(handle, flux_kernel, update_kernel) &
+         bind(c, name='hugks_set_schedule')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int32_t), value :: flux_kernel, update_kernel
+    end function hugks_set_schedule
+
+    integer(c_int) function hugks_tune(desc, flux_kernel, update_kernel, ms_per_step) &
+         bind(c, name='hugks_tune')
+      import :: c_int, c_int32_t, c_double, hugks_desc
+      type(hugks_desc), intent(in) :: desc
+      integer(c_int32_t), intent(out) :: flux_kernel, update_kernel
+      real(c_double), intent(out) :: ms_per_step
+    end function hugks_tune
+
+    integer(c_int) function hugks_batch_create(n, descs, flags, batch) &
+         bind(c, name='hugks_batch_create')
+      import :: c_int, c_int32_t, c_ptr, hugks_desc
+      integer(c_int32_t), value :: n
+      type(hugks_desc), intent(in) :: descs(*)
+      integer(c_int32_t), value :: flags
+      type(c_ptr), intent(out) :: batch
+    end function hugks_batch_create
+
+    integer(c_int) function hugks_batch_destroy(batch) bind(c, name='hugks_batch_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function hugks_batch_destroy
+
+    integer(c_int) function hugks_batch_handle(batch, i, handle) bind(c, name='hugks_batch_handle')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: i
+      type(c_ptr), intent(out) :: handle
+    end function hugks_batch_handle
+
+    integer(c_int) function hugks_batch_advance(batch, max_steps, infos) &
+         bind(c, name='hugks_batch_advance')
+      import :: c_int, c_int32_t, c_ptr, hugks_step_info
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: max_steps
+      type(hugks_step_info), intent(out) :: infos(*)
+    end function hugks_batch_advance
   end interface
 
 contains

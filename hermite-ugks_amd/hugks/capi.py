@@ -21,6 +21,8 @@ TAU_MU_OVER_RHO, TAU_VHS = 0, 1
 DT_2D_GENERAL, DT_2D_WALL_NORMAL, DT_1D = 0, 1, 2
 FORCE_HERMITE_SYMMETRIC, FORCE_HERMITE_SYMMETRIC_B, FORCE_HERMITE_LEGACY = 0, 1, 2
 STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL, STOP_TIME, STOP_STEADY = 0, 1, 2, 3, 4
+FLUX_FUSED, FLUX_SPLIT, UPDATE_LDS, UPDATE_MFMA = 0, 1, 0, 1
+BATCH_AUTOTUNE = 1
 
 _dp = C.POINTER(C.c_double)
 
@@ -63,6 +65,9 @@ EXPORTS = [
     "hugks_advance_until",
     "hugks_get_macro", "hugks_get_f", "hugks_set_f", "hugks_get_time", "hugks_kernel_times",
     "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
+    "hugks_set_schedule", "hugks_get_schedule", "hugks_tune", "hugks_batch_create",
+    "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_handle",
+    "hugks_batch_advance",
 ]
 ARR_SHX, ARR_SHY, ARR_SBX, ARR_SBY, ARR_VFH, ARR_VFB, ARR_HFH, ARR_HFB, ARR_VFLUX, ARR_HFLUX = range(10)
 
@@ -98,6 +103,18 @@ def load(path=LIBPATH):
     lib.hugks_set_timing.argtypes = [H, C.c_int32]
     lib.hugks_device_info.restype = C.c_char_p
     lib.hugks_debug_fetch.argtypes = [H, C.c_int32, _dp]
+    I = C.POINTER(C.c_int32)
+    lib.hugks_set_schedule.argtypes = [H, C.c_int32, C.c_int32]
+    lib.hugks_get_schedule.argtypes = [H, I, I]
+    lib.hugks_tune.argtypes = [C.POINTER(Desc), I, I, _dp]
+    lib.hugks_batch_create.argtypes = [C.c_int32, C.POINTER(Desc), C.c_int32, C.POINTER(H)]
+    lib.hugks_batch_destroy.argtypes = [H]
+    lib.hugks_batch_last_error.argtypes = [H]
+    lib.hugks_batch_last_error.restype = C.c_char_p
+    lib.hugks_batch_size.argtypes = [H]
+    lib.hugks_batch_size.restype = C.c_int32
+    lib.hugks_batch_handle.argtypes = [H, C.c_int32, C.POINTER(H)]
+    lib.hugks_batch_advance.argtypes = [H, C.c_int32, C.POINTER(StepInfo)]
     _lib = lib
     return lib
 
@@ -151,9 +168,13 @@ class Case:
 class Solver:
     """One library handle (one case on the current HIP device)."""
 
-    def __init__(self, case):
+    def __init__(self, case, handle=None):
         self.lib = load()
         self.case = case
+        self.owned = handle is None
+        if handle is not None:          # a case handle owned by a Batch
+            self.h = handle
+            return
         self.h = C.c_void_p()
         d = case.bind()
         rc = self.lib.hugks_create(C.byref(d), C.byref(self.h))
@@ -165,9 +186,9 @@ class Solver:
             raise HugksError(rc, self.lib.hugks_last_error(self.h).decode())
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             self.lib.hugks_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -252,6 +273,77 @@ class Solver:
         n = C.c_int32()
         self._check(self.lib.hugks_kernel_times(self.h, cap, names, ms, calls, C.byref(n)))
         return {names[i].decode(): (ms[i], calls[i]) for i in range(min(n.value, cap))}
+
+
+    def set_schedule(self, flux_kernel, update_kernel):
+        self._check(self.lib.hugks_set_schedule(self.h, int(flux_kernel), int(update_kernel)))
+
+    def schedule(self):
+        f, u = C.c_int32(), C.c_int32()
+        self._check(self.lib.hugks_get_schedule(self.h, C.byref(f), C.byref(u)))
+        return f.value, u.value
+
+
+def tune(case):
+    """(flux_kernel, update_kernel, ms_per_step) the autotuner picks for
+    the case's shape."""
+    lib = load()
+    f, u, ms = C.c_int32(), C.c_int32(), C.c_double()
+    d = case.bind()
+    rc = lib.hugks_tune(C.byref(d), C.byref(f), C.byref(u), C.byref(ms))
+    if rc != OK:
+        raise HugksError(rc, lib.hugks_last_error(None).decode())
+    return f.value, u.value, ms.value
+
+
+class Batch:
+    """Independent cases advanced together by one graph replay per step."""
+
+    def __init__(self, case_list, autotune=False):
+        self.lib = load()
+        self.cases = list(case_list)
+        descs = (Desc * len(self.cases))()
+        for i, c in enumerate(self.cases):
+            descs[i] = c.bind()
+        self._descs = descs
+        self.b = C.c_void_p()
+        rc = self.lib.hugks_batch_create(len(self.cases), descs,
+                                         BATCH_AUTOTUNE if autotune else 0, C.byref(self.b))
+        if rc != OK:
+            raise HugksError(rc, self.lib.hugks_batch_last_error(None).decode())
+        self.solvers = []
+        for i, c in enumerate(self.cases):
+            h = C.c_void_p()
+            self._check(self.lib.hugks_batch_handle(self.b, i, C.byref(h)))
+            self.solvers.append(Solver(c, handle=h))
+
+    def _check(self, rc):
+        if rc != OK:
+            raise HugksError(rc, self.lib.hugks_batch_last_error(self.b).decode())
+
+    def advance(self, max_steps):
+        infos = (StepInfo * len(self.cases))()
+        self._check(self.lib.hugks_batch_advance(self.b, int(max_steps), infos))
+        return list(infos)
+
+    def close(self):
+        if self.b:
+            for s in self.solvers:
+                s.h = C.c_void_p()
+            self.lib.hugks_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def device_info():

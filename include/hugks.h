@@ -206,6 +206,46 @@ int hugks_set_f(hugks_handle *h, const double *hf, const double *bf,
 #define HUGKS_ARR_HFLUX  9
 int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
 
+/* Kernel schedule of the 2D step (ignored in 1D).  Every schedule computes
+ * the same step; they differ in speed by case shape.
+ *   flux:   HUGKS_FLUX_FUSED  one kernel per face orientation
+ *           HUGKS_FLUX_SPLIT  five-kernel pipeline (default)
+ *   update: HUGKS_UPDATE_LDS  Hermite contractions as LDS dot products
+ *           HUGKS_UPDATE_MFMA Hermite contractions as matrix tiles (default
+ *                             when nv <= 48 and Nm <= 14) */
+#define HUGKS_FLUX_FUSED   0
+#define HUGKS_FLUX_SPLIT   1
+#define HUGKS_UPDATE_LDS   0
+#define HUGKS_UPDATE_MFMA  1
+int hugks_set_schedule(hugks_handle *h, int32_t flux_kernel, int32_t update_kernel);
+int hugks_get_schedule(hugks_handle *h, int32_t *flux_kernel, int32_t *update_kernel);
+
+/* Autotuner: times every schedule on a scratch handle of the descriptor's
+ * shape and returns the fastest (cached per shape for the process);
+ * *ms_per_step (may be NULL) receives its time, or -1 from the cache. */
+int hugks_tune(const hugks_desc *d, int32_t *flux_kernel, int32_t *update_kernel,
+               double *ms_per_step);
+
+/* Batches of independent cases (a Kn / force sweep of uneven grid and
+ * velocity sizes).  The reference runs each case as its own process; here
+ * one step of every case is captured into one HIP graph, whose replay
+ * advances the whole batch by a step, each case stopping on its own exit
+ * conditions.  With HUGKS_BATCH_AUTOTUNE each case gets the schedule
+ * hugks_tune picks for its shape.  hugks_batch_handle gives the case's
+ * handle for hugks_get_macro and the other per-case calls (not for
+ * hugks_step/advance while the batch is in use). */
+#define HUGKS_BATCH_AUTOTUNE 1
+typedef struct hugks_batch hugks_batch;
+int hugks_batch_create(int32_t n, const hugks_desc *descs, int32_t flags,
+                       hugks_batch **out);
+int hugks_batch_destroy(hugks_batch *b);
+const char *hugks_batch_last_error(const hugks_batch *b);
+int32_t hugks_batch_size(const hugks_batch *b);
+int hugks_batch_handle(hugks_batch *b, int32_t i, hugks_handle **h);
+/* max_steps replays (<= 0: until every case stopped); infos[n] receives
+ * each case's loop state. */
+int hugks_batch_advance(hugks_batch *b, int32_t max_steps, hugks_step_info *infos);
+
 /* Loop state (simtime, iter) of the handle. */
 int hugks_get_time(hugks_handle *h, double *simtime, int32_t *iter);
 

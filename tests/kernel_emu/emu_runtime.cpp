@@ -14,6 +14,8 @@ unsigned nthreads = 0;
 Bar block_bar = {0, 0};
 Bar wave_bar[16] = {};
 double mfa[1024], mfb[1024];
+bool capturing = false;
+std::vector<std::function<void()>>* capture_list = nullptr;
 unsigned char* dyn_smem = nullptr;
 unsigned char xbuf[1024 * 8];
 long launches = 0;

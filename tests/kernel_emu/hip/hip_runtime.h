@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+#include <functional>
 #include <vector>
 
 #define __global__
@@ -83,8 +84,26 @@ struct Call {
     }
 };
 
+// Stream capture: while a capture is open, launches are recorded (by
+// value) instead of run; a graph replays them in recording order, which is
+// one valid order of the captured dependencies.
+extern bool capturing;
+extern std::vector<std::function<void()>>* capture_list;
+
+template <class K, class... A>
+void launch_now(K kernel, dim3 g, dim3 b, size_t shmem, A... args);
+
 template <class K, class... A>
 void launch(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
+    if (capturing) {
+        capture_list->push_back([=]() { launch_now(kernel, g, b, shmem, args...); });
+        return;
+    }
+    launch_now(kernel, g, b, shmem, args...);
+}
+
+template <class K, class... A>
+void launch_now(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
     const unsigned nt = b.x * b.y * b.z;
     const unsigned long nblk = (unsigned long)g.x * g.y * g.z;
     ++launches;
@@ -246,3 +265,33 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
 }
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
+#define hipEventDisableTiming 2
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal,
+                            hipStreamCaptureModeRelaxed };
+struct emu_graph { std::vector<std::function<void()>> ops; };
+typedef emu_graph* hipGraph_t;
+typedef emu_graph* hipGraphExec_t;
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    emu::capture_list = new std::vector<std::function<void()>>();
+    emu::capturing = true;
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    emu::capturing = false;
+    *g = new emu_graph{std::move(*emu::capture_list)};
+    delete emu::capture_list;
+    emu::capture_list = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, void*, void*, size_t) {
+    *x = new emu_graph{g->ops};
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t) {
+    for (auto& op : x->ops) op();
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t x) { delete x; return hipSuccess; }

@@ -1,0 +1,94 @@
+"""Batches of independent cases (hugks_batch_*): one graph replay per step
+for the whole batch must give, case by case, exactly what the same case
+gives when stepped on its own with the same kernel schedule, on every
+replay, with ordinary eager work and synchronisations between replays;
+the autotuned schedules must still agree with the oracle; each case stops
+on its own exit conditions."""
+import numpy as np
+import pytest
+
+from hugks import capi, cases
+import oracle_py
+from metrics import field_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def poiseuille(nx=4, ny=12, nv=12, nm=6, kn=0.1, phi=0.05, tw=(1.0, 1.0), maxtime=200.0):
+    return dict(nx=nx, ny=ny, xlength=0.1, ylength=1.0, nv=nv, vmax=5.0, Tref=1.0,
+                quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=nm, cfl=0.8,
+                maxtime=maxtime, case_id=5, knudsen=kn, phi=phi, TwL=tw[0], TwR=tw[1])
+
+
+def rt(nx=6, ny=8, nv=12, nm=6):
+    return dict(nx=nx, ny=ny, xlength=0.5, ylength=1.0, nv=nv, vmax=5.0, Tref=1.0,
+                quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=nm, cfl=0.5,
+                maxtime=0.14, case_id=6, knudsen=1e-3, phi=1.0, TwL=1.0, TwR=1.0)
+
+
+def batch_cases():
+    return [
+        cases.build_2d(poiseuille(nv=12, ny=10), capi.FP64),
+        cases.build_2d(poiseuille(nv=16, ny=14, kn=0.5, phi=0.1, tw=(1.0, 1.2)), capi.MIXED_FP32),
+        cases.build_2d(rt(nv=14), capi.FP64, rt_variant="cartesian", seed=4),
+        cases.build_2d(poiseuille(nv=20, ny=8, nm=8), capi.MIXED_FP32),
+        cases.build_2d(rt(nv=16, nm=8), capi.MIXED_FP32, rt_variant="cartesian", seed=9),
+    ]
+
+
+@pytest.mark.parametrize("autotune", [False, True])
+def test_batch_equals_cases_stepped_alone(have_gpu, autotune):
+    cs = batch_cases()
+    print(f"batch of {len(cs)} autotune={autotune}", flush=True)
+    with capi.Batch(cs, autotune=autotune) as b:
+        scheds = [s.schedule() for s in b.solvers]
+        print("schedules", scheds, flush=True)
+        singles = []
+        for c, sc in zip(cs, scheds):
+            s = capi.Solver(c)
+            s.set_schedule(*sc)
+            singles.append(s)
+        for rep, n in enumerate((3, 1, 4)):
+            infos = b.advance(n)
+            assert [i.steps for i in infos] == [n] * len(cs)
+            for k, s in enumerate(singles):
+                s.advance(n)
+                # eager work on the case's own stream between replays
+                wa = s.macro(with_q=False)
+                wb = b.solvers[k].macro(with_q=False)
+                assert np.array_equal(wa, wb), (rep, k, np.max(np.abs(wa - wb)))
+                ha, _ = s.get_f()
+                hb, _ = b.solvers[k].get_f()
+                assert np.array_equal(ha, hb), (rep, k)
+        for s in singles:
+            s.close()
+
+
+def test_batch_autotuned_parity_with_oracle(have_gpu):
+    cs = batch_cases()
+    with capi.Batch(cs, autotune=True) as b:
+        b.advance(6)
+        ws = [s.macro(with_q=False) for s in b.solvers]
+    for c, w in zip(cs, ws):
+        o = oracle_py.Oracle(c.with_precision(capi.FP64))
+        o.step(6)
+        err = field_errors(w, o.get()[0])
+        tol = 1e-10 if c.desc.precision == capi.FP64 else 1e-5
+        print("case nv", c.desc.nv, "prec", c.desc.precision, "err", err, flush=True)
+        assert np.all(err <= tol), err
+
+
+def test_batch_cases_stop_on_their_own(have_gpu):
+    """Cases with different maxtime: the uncapped advance runs until every
+    case has stopped; each stops at its own maxtime after the same number
+    of steps as when run alone."""
+    cs = [cases.build_2d(poiseuille(nv=10, ny=8, maxtime=t), capi.FP64) for t in (0.02, 0.05, 0.035)]
+    with capi.Batch(cs) as b:
+        infos = b.advance(0)
+    for c, inf in zip(cs, infos):
+        with capi.Solver(c) as s:
+            alone = s.advance(0)
+        print("maxtime", c.desc.maxtime, "batch steps", inf.steps, "alone", alone.steps, flush=True)
+        assert inf.stop_reason == capi.STOP_MAXTIME
+        assert inf.steps == alone.steps and inf.iter == alone.iter
+        assert inf.simtime == alone.simtime
