@@ -289,9 +289,10 @@ def bench_c4(a):
             e[1] += calls
             e[2] += ab or 0
         s.close()
-    dom = max(kt_all, key=lambda k: kt_all[k][0])
+    with_bytes = [k for k in kt_all if kt_all[k][2]]
+    dom = max(with_bytes, key=lambda k: kt_all[k][0]) if with_bytes else None
     roof = None
-    if kt_all[dom][2]:
+    if dom:
         gbs = kt_all[dom][2] / (kt_all[dom][0] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
@@ -301,10 +302,16 @@ def bench_c4(a):
         f64 = [c.with_precision(capi.FP64) for c in mixed]
         tf, wf, _ = run_batch(f64)
         errs = np.array([field_errors(x, y) for x, y in zip(wm, wf)])
+        errs_th = np.array([field_errors(x, y, thermal=True) for x, y in zip(wm, wf)])
         out["fp64"] = {"value": N * a.steps / tf, "ms_per_step": 1e3 * tf / a.steps}
-        out["mixed_vs_fp64"] = {"speedup": value / out["fp64"]["value"],
-                                "max_rel_err_conserved": errs.max(axis=0).tolist(),
-                                "steps_compared": a.warmup + a.steps}
+        out["mixed_vs_fp64"] = {
+            "speedup": value / out["fp64"]["value"],
+            "max_rel_err_conserved": errs.max(axis=0).tolist(),
+            "max_rel_err_conserved_thermal_momentum_scale": errs_th.max(axis=0).tolist(),
+            "steps_compared": a.warmup + a.steps,
+            "note": "channel flows started from rest: momentum is small for the first "
+                    "hundreds of steps, so errors relative to the momentum magnitude are "
+                    "large; the second figure scales momentum by sqrt(rho * rho E)"}
     from collections import Counter
     line = {
         "metric": METRIC, "value": value, "unit": "phase-space updates/s", "n_gpus": 1,

@@ -819,7 +819,9 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     // one step of every case, fanned out over a few streams and captured
     // into one graph: a replay advances the whole batch by one step
     hipError_t e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking);
-    const int nl = n < 4 ? n : 4;
+    int lanes = 8;
+    if (const char* ls = getenv("HUGKS_BATCH_LANES")) lanes = atoi(ls) > 0 ? atoi(ls) : lanes;
+    const int nl = n < lanes ? n : lanes;
     for (int l = 0; l < nl && e == hipSuccess; ++l) {
         hipStream_t s = nullptr;
         e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
