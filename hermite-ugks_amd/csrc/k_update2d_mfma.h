@@ -50,10 +50,13 @@ constexpr int MF_NB = 16;    // Hermite basis rows He_0..He_15
 __host__ __device__ constexpr int mf_np(int nv) { return (nv + 15) / 16 * 16; }
 
 // LDS elements (of T) of one cell: G/F [2][nn], Hx, Hy [16][NP],
-// Tm [2][16][NP], D [2][16][16], R [2][16][NP].
+// Tm [2][16][NP], D [2][16][16], R [2][16][NP]; then, as doubles, the
+// per-axis factors of the two Maxwellians and of the force weight,
+// [6][NP] (counted here in T elements).
 __host__ __device__ inline size_t mf_lds_elems(int nv) {
     const size_t np = mf_np(nv), nn = (size_t)nv * nv;
-    return 2 * nn + 2 * MF_NB * np + 2 * MF_NB * np + 2 * MF_NB * MF_NB + 2 * MF_NB * np;
+    return 2 * nn + 2 * MF_NB * np + 2 * MF_NB * np + 2 * MF_NB * MF_NB + 2 * MF_NB * np + 1 +
+           6 * np * 2;   // doubles take at most two T elements; +1 aligns them
 }
 
 template <typename T, int EPT>
@@ -105,6 +108,17 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     T* Tm = Hy + MF_NB * np;                      // [2][16][np]
     T* D = Tm + 2 * MF_NB * np;                   // [2][16][16]
     T* R = D + 2 * MF_NB * MF_NB;                 // [2][16][np]
+    // per-axis factors (FP64): the Maxwellians at t^n and t^{n+1} and the
+    // force weight are products of a u factor and a v factor,
+    // exp(-lam (u-U)^2 - lam (v-V)^2) = ex(u) ey(v)
+    double* Ex = reinterpret_cast<double*>(
+        (reinterpret_cast<uintptr_t>(R + 2 * MF_NB * np) + 7) & ~(uintptr_t)7);
+    double* Eo_x = Ex;                            // [np] old Maxwellian, u
+    double* Eo_y = Ex + np;                       //      v
+    double* En_x = Ex + 2 * np;                   // new Maxwellian
+    double* En_y = Ex + 3 * np;
+    double* Om_x = Ex + 4 * np;                   // force weight exp(-xi^2/2)
+    double* Om_y = Ex + 5 * np;
 
     // ---- load f^n, weighted copies, heat flux at t^n (solver.f90:337),
     // FP64 whatever the storage type
@@ -140,7 +154,12 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
         const int idx = isx ? t : t - np;
         T* Hb = isx ? Hx : Hy;
         if (idx < nv) {
-            const A x = (A)((isx ? (u1[idx] - po[1]) : (v1[idx] - po[2])) / sqrt_rt);
+            const double xv = isx ? u1[idx] : v1[idx];
+            const double co = xv - (isx ? po[1] : po[2]), cn = xv - (isx ? pn[1] : pn[2]);
+            (isx ? Eo_x : Eo_y)[idx] = exp(-po[3] * co * co);
+            (isx ? En_x : En_y)[idx] = exp(-pn[3] * cn * cn);
+            const A x = (A)(co / sqrt_rt);
+            (isx ? Om_x : Om_y)[idx] = (double)fexp((A)-0.5 * x * x);
             A p0 = (A)1, p1 = x;
             Hb[idx] = (T)p0;
             Hb[np + idx] = (T)p1;
@@ -251,23 +270,25 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     const double itau = 1.0 / tau, itau_o = 1.0 / tau_old;
     const double iden = 1.0 / (1.0 + 0.5 * dt / tau);
     const double iarea = 1.0 / area;
-    const A Uo = (A)po[1], Vo = (A)po[2];
+    const float rnv = 1.0f / (float)nv;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = tid + e * MF_NT;
         if (k < nn) {
+            // node (iu, jv) of k = iu + nv*jv without an integer division:
+            // (k + 1/2)/nv stays at least 1/(2 nv) away from an integer
+            const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
             const double u = und[k], v = vnd[k];
-            const A xi = ((A)u - Uo) * inv_sqrt_rt, eta = ((A)v - Vo) * inv_sqrt_rt;
-            const A om = fexp((A)-0.5 * (xi * xi + eta * eta)) / two_pi;
+            const A om = (A)(Om_x[iu] * Om_y[jv]) / two_pi;
             const double Fh = (double)(-inv_sqrt_rt * om * (A)F[k]);
             const double Fb = (double)(-inv_sqrt_rt * om * (A)F[nn + k]);
             const double cuo = u - po[1], cvo = v - po[2], c2o = cuo * cuo + cvo * cvo;
-            const double Ho = Hco * exp(-po[3] * c2o), Bo = Ho * kfo;
+            const double Ho = Hco * (Eo_x[iu] * Eo_y[jv]), Bo = Ho * kfo;
             const double cqo = cso * (cuo * qf0 + cvo * qf1);
             const double Hpo = Ho + cqo * (2.0 * po[3] * c2o + c.K - 5.0) * Ho;
             const double Bpo = Bo + cqo * (2.0 * po[3] * c2o + c.K - 3.0) * Bo;
             const double cun = u - pn[1], cvn = v - pn[2], c2n = cun * cun + cvn * cvn;
-            const double Hn = Hcn * exp(-pn[3] * c2n), Bn = Hn * kfn;
+            const double Hn = Hcn * (En_x[iu] * En_y[jv]), Bn = Hn * kfn;
             const double cqn = csn * (cun * qf0 + cvn * qf1);
             const double Hpn = Hn + cqn * (2.0 * pn[3] * c2n + c.K - 5.0) * Hn;
             const double Bpn = Bn + cqn * (2.0 * pn[3] * c2n + c.K - 3.0) * Bn;
