@@ -394,6 +394,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         ALLOC(L.fsum, (nint ? nint : 1) * FACE_SUM_LD * sizeof(double));
         ALLOC(L.fcoef, (nint ? nint : 1) * FACE_COEF_LD * sizeof(double));
         ALLOC(L.fmaxw, (nint ? nint : 1) * FACE_MAXW_LD * sizeof(double));
+        ALLOC(L.cellpre, ncell * 24 * sizeof(double));
         L.update_lds = update2d_lds_bytes(c, h->fp64);
         L.update_mode = update2d_mfma_supported(c) ? 1 : 0;
         if (const char* um = getenv("HUGKS_UPDATE_KERNEL"))

@@ -29,6 +29,7 @@ struct Launch2D {
     int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline
     int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles
     double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
+    double* cellpre;     // per-cell scalars of the MFMA update [ncell][24]
     int update_nt;       // threads per update workgroup
     size_t update_lds;   // dynamic LDS bytes of the update kernel
     LaunchHook hook;     // may be null

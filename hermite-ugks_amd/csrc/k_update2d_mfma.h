@@ -59,15 +59,92 @@ __host__ __device__ inline size_t mf_lds_elems(int nv) {
            6 * np * 2;   // doubles take at most two T elements; +1 aligns them
 }
 
+// Per-cell scalars of the update, one lane per cell, computed once instead
+// of by every thread of the cell's workgroup: the macroscopic predictor
+// (solver.f90:302-321), both collision times and the coefficients of the
+// node-wise update; also the new W, the residual partials and the
+// NaN/T<=0 flag.  Record of CP_LD doubles per cell (slots CP_*).
+constexpr int CP_LD = 24;
+enum : int {
+    CP_PO = 0, CP_PN = 4, CP_TAUO = 8, CP_TAU = 9, CP_AX = 10, CP_AY = 11, CP_RT = 12,
+    CP_HCO = 13, CP_HCN = 14, CP_KFO = 15, CP_KFN = 16, CP_CSO = 17, CP_CSN = 18,
+    CP_IDEN = 19, CP_ITAU = 20, CP_ITAUO = 21,
+};
+
+__global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict__ st,
+                                                  double* __restrict__ W,
+                                                  double* __restrict__ part,
+                                                  const double* __restrict__ vflux,
+                                                  const double* __restrict__ hflux,
+                                                  const double* __restrict__ acc_cell,
+                                                  double* __restrict__ cp) {
+    if (st->stopped) return;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= c.nx * c.ny) return;
+    const int nx = c.nx;
+    const int i = cell % nx, j = cell / nx;
+    const double dt = st->dt;
+    const double area = c.dx * c.dy;
+    const int vl = i + (nx + 1) * j, vr = vl + 1;
+    const int hd = i + nx * j, hu = i + nx * (j + 1);
+    const double ax = acc_cell[2 * cell], ay = acc_cell[2 * cell + 1];
+
+    double wo[4], po[4], wn[4], pn[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wo[q] = W[4 * (size_t)cell + q];
+    prim_from_w(c, wo, po);
+    const double tau_old = tau_of(c, po[0], po[3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        wn[q] = wo[q] + (vflux[4 * (size_t)vl + q] - vflux[4 * (size_t)vr + q] +
+                         hflux[4 * (size_t)hd + q] - hflux[4 * (size_t)hu + q]) / area;
+    prim_from_w(c, wn, pn);
+    pn[1] = pn[1] + ax * dt;
+    pn[2] = pn[2] + ay * dt;
+    w_from_prim(c, pn, wn);
+    const double tau = tau_of(c, pn[0], pn[3]);
+
+    double* P = cp + (size_t)CP_LD * cell;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        P[CP_PO + q] = po[q];
+        P[CP_PN + q] = pn[q];
+    }
+    P[CP_TAUO] = tau_old;
+    P[CP_TAU] = tau;
+    P[CP_AX] = ax;
+    P[CP_AY] = ay;
+    P[CP_RT] = RC / po[3];
+    P[CP_HCO] = po[0] * (po[3] / c.pi);
+    P[CP_HCN] = pn[0] * (pn[3] / c.pi);
+    P[CP_KFO] = c.K / (2.0 * po[3]);
+    P[CP_KFN] = c.K / (2.0 * pn[3]);
+    P[CP_CSO] = c.shk * (1.0 - c.prandtl) * po[3] * po[3] / po[0];
+    P[CP_CSN] = c.shk * (1.0 - c.prandtl) * pn[3] * pn[3] / pn[0];
+    P[CP_IDEN] = 1.0 / (1.0 + 0.5 * dt / tau);
+    P[CP_ITAU] = 1.0 / tau;
+    P[CP_ITAUO] = 1.0 / tau_old;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        W[4 * (size_t)cell + q] = wn[q];
+        part[5 * (size_t)cell + q] = (wo[q] - wn[q]) * (wo[q] - wn[q]);
+    }
+    part[5 * (size_t)cell + 4] = fabs(wn[0]);
+    if (!(pn[3] > 0.0) || !isfinite(pn[0]) || !isfinite(pn[1]) || !isfinite(pn[2]) ||
+        !isfinite(pn[3]) || !(pn[0] > 0.0))
+        atomicOr(&st->flags, 1);
+}
+
+// Shakhov-BGK update of one cell per workgroup with the Hermite force on
+// MFMA; the cell's scalars come from k_cell_pre.
 template <typename T, int EPT>
 __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
-    Consts c, DevState* __restrict__ st, double* __restrict__ W, double* __restrict__ part,
+    Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
-    const T* __restrict__ hfh, const T* __restrict__ hfb, const double* __restrict__ vflux,
-    const double* __restrict__ hflux, const typename Arith<T>::A* __restrict__ wnode,
-    const double* __restrict__ wnode_d, const double* __restrict__ und,
-    const double* __restrict__ vnd, const double* __restrict__ u1,
-    const double* __restrict__ v1, const double* __restrict__ acc_cell) {
+    const T* __restrict__ hfh, const T* __restrict__ hfb,
+    const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ wnode_d,
+    const double* __restrict__ und, const double* __restrict__ vnd,
+    const double* __restrict__ u1, const double* __restrict__ v1) {
     using A = typename Arith<T>::A;
     using M = Mfma<T>;
     using V = typename M::V;
@@ -83,23 +160,14 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     const double area = c.dx * c.dy;
     const int vl = i + (nx + 1) * j, vr = vl + 1;
     const int hd = i + nx * j, hu = i + nx * (j + 1);
-    const double ax = acc_cell[2 * cell], ay = acc_cell[2 * cell + 1];
-
-    // ---- macroscopic predictor (solver.f90:302-321)
-    double wo[4], po[4], wn[4], pn[4];
+    const double* P = cp + (size_t)CP_LD * cell;
+    double po[4], pn[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) wo[q] = W[4 * (size_t)cell + q];
-    prim_from_w(c, wo, po);
-    const double tau_old = tau_of(c, po[0], po[3]);
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        wn[q] = wo[q] + (vflux[4 * (size_t)vl + q] - vflux[4 * (size_t)vr + q] +
-                         hflux[4 * (size_t)hd + q] - hflux[4 * (size_t)hu + q]) / area;
-    prim_from_w(c, wn, pn);
-    pn[1] = pn[1] + ax * dt;
-    pn[2] = pn[2] + ay * dt;
-    w_from_prim(c, pn, wn);
-    const double tau = tau_of(c, pn[0], pn[3]);
+    for (int q = 0; q < 4; ++q) {
+        po[q] = P[CP_PO + q];
+        pn[q] = P[CP_PN + q];
+    }
+    const double ax = P[CP_AX], ay = P[CP_AY];
 
     // ---- LDS carve-up
     T* G = reinterpret_cast<T*>(smem_raw);        // [2][nn]; later F
@@ -122,7 +190,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
 
     // ---- load f^n, weighted copies, heat flux at t^n (solver.f90:337),
     // FP64 whatever the storage type
-    const double rt = RC / po[3];
+    const double rt = P[CP_RT];
     const double sqrt_rt = sqrt(rt);
     T rh[EPT], rb[EPT];
     double q4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -262,13 +330,12 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     // ---- force and implicit BGK update (solver.f90:337-388)
     const A inv_sqrt_rt = (A)(1.0 / sqrt_rt);
     const A two_pi = (A)(2.0 * c.pi);
-    const double Hco = po[0] * (po[3] / c.pi), Hcn = pn[0] * (pn[3] / c.pi);
-    const double kfo = c.K / (2.0 * po[3]), kfn = c.K / (2.0 * pn[3]);
-    const double cso = c.shk * (1.0 - c.prandtl) * po[3] * po[3] / po[0];
-    const double csn = c.shk * (1.0 - c.prandtl) * pn[3] * pn[3] / pn[0];
+    const double Hco = P[CP_HCO], Hcn = P[CP_HCN];
+    const double kfo = P[CP_KFO], kfn = P[CP_KFN];
+    const double cso = P[CP_CSO], csn = P[CP_CSN];
     const double half_dt = 0.5 * dt;
-    const double itau = 1.0 / tau, itau_o = 1.0 / tau_old;
-    const double iden = 1.0 / (1.0 + 0.5 * dt / tau);
+    const double itau = P[CP_ITAU], itau_o = P[CP_ITAUO];
+    const double iden = P[CP_IDEN];
     const double iarea = 1.0 / area;
     const float rnv = 1.0f / (float)nv;
 #pragma unroll
@@ -303,17 +370,6 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             h[o] = (T)hn;
             b[o] = (T)bn;
         }
-    }
-    if (tid == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            W[4 * (size_t)cell + q] = wn[q];
-            part[5 * (size_t)cell + q] = (wo[q] - wn[q]) * (wo[q] - wn[q]);
-        }
-        part[5 * (size_t)cell + 4] = fabs(wn[0]);
-        if (!(pn[3] > 0.0) || !isfinite(pn[0]) || !isfinite(pn[1]) || !isfinite(pn[2]) ||
-            !isfinite(pn[3]) || !(pn[0] > 0.0))
-            atomicOr(&st->flags, 1);
     }
 }
 

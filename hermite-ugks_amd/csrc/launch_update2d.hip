@@ -66,10 +66,14 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool pr
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
         HookScope hs(L, "update2d");                                                            \
         hipLaunchKernelGGL((k_update2d_mfma<T, E>), dim3(ncell), dim3(MF_NT), lds, s, c, L.st,  \
-            L.W, L.part, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,   \
-            (const T*)L.hfb, L.vflux, L.hflux, (const A*)L.wnode, L.wnode_d, L.unode_d,         \
-            L.vnode_d, L.u1, L.v1, L.acc_cell);                                                 \
+            L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,     \
+            (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.unode_d, L.vnode_d, L.u1, L.v1);   \
         break;                                                                                  \
+    }
+    if (!prepare) {
+        HookScope hs(L, "cell_pre");
+        hipLaunchKernelGGL((k_cell_pre), dim3((ncell + 255) / 256), dim3(256), 0, s, c, L.st, L.W,
+                           L.part, L.vflux, L.hflux, L.acc_cell, L.cellpre);
     }
     switch (ept) {
         HUGKS_UPDM_CASE(1)
