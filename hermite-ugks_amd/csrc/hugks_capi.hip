@@ -389,8 +389,41 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         const size_t nint = (size_t)(xper ? d->nx + 1 : d->nx - 1) * d->ny +
                             (size_t)d->nx * (yper ? d->ny + 1 : d->ny - 1);
         L.flux_mode = 1;
-        if (const char* fm = getenv("HUGKS_FLUX_KERNEL"))
+        bool maxw_nodes = false;
+        if (const char* fm = getenv("HUGKS_FLUX_KERNEL")) {
             if (strcmp(fm, "fused") == 0) L.flux_mode = 0;
+            if (strcmp(fm, "split-nodes") == 0) maxw_nodes = true;
+        }
+        // Factors of a tensor-product quadrature, w[i + nv j] = wx[i] wy[j]
+        // (both quadratures of velocity.f90:43,128 are such products); the
+        // Maxwellian sums of the flux then reduce to 1D moments.  Taken
+        // from the row and column of the largest weight and accepted when
+        // they reproduce every weight to 1e-13.
+        {
+            const int nv = d->nv;
+            int km = 0;
+            for (int k = 1; k < nn; ++k)
+                if (d->weight[k] > d->weight[km]) km = k;
+            const int im = km % nv, jm = km / nv;
+            const double wmax = d->weight[km];
+            std::vector<double> fx(nv), fy(nv);
+            bool sep = wmax > 0 && !maxw_nodes;
+            for (int i = 0; i < nv; ++i) fx[i] = d->weight[i + nv * jm];
+            for (int j = 0; j < nv; ++j) fy[j] = d->weight[im + nv * j] / wmax;
+            for (int k = 0; sep && k < nn; ++k) {
+                const double w = d->weight[k], pr = fx[k % nv] * fy[k / nv];
+                if (!(fabs(pr - w) <= 1e-13 * w)) sep = false;
+            }
+            if (sep) {
+                double *wxd, *wyd;
+                ALLOC(wxd, nv * sizeof(double));
+                ALLOC(wyd, nv * sizeof(double));
+                hipMemcpy(wxd, fx.data(), nv * sizeof(double), hipMemcpyHostToDevice);
+                hipMemcpy(wyd, fy.data(), nv * sizeof(double), hipMemcpyHostToDevice);
+                L.wx = wxd;
+                L.wy = wyd;
+            }
+        }
         ALLOC(L.fsum, (nint ? nint : 1) * FACE_SUM_LD * sizeof(double));
         ALLOC(L.fcoef, (nint ? nint : 1) * FACE_COEF_LD * sizeof(double));
         ALLOC(L.fmaxw, (nint ? nint : 1) * FACE_MAXW_LD * sizeof(double));

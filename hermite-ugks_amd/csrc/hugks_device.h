@@ -170,6 +170,48 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
     }
 }
 
+// Stage of wave_reduce_scatter: the M values of each lane are halved to
+// ceil(M/2) by exchanging with the lane O apart; the lower lane of the pair
+// keeps the first half, the upper lane the second (zero-padded when M is
+// odd).  On the way back pos becomes the value index the lane ends up
+// holding, or -1 where its path ran through padding.
+template <int M, int O>
+__device__ __forceinline__ double wave_rs_stage(double* x, int lane, int& pos) {
+    constexpr int M2 = (M + 1) / 2;
+    const bool hi = (lane & O) != 0;
+#pragma unroll
+    for (int t = 0; t < M2; ++t) {
+        const double lo_v = x[t];
+        const double hi_v = (t + M2 < M) ? x[t + M2] : 0.0;
+        const double keep = hi ? hi_v : lo_v;
+        const double send = hi ? lo_v : hi_v;
+        x[t] = keep + __shfl_xor(send, O, WAVE);
+    }
+    double r;
+    if constexpr (O > 1) {
+        r = wave_rs_stage<M2, O / 2>(x, lane, pos);
+    } else {
+        r = x[0];
+        pos = 0;
+    }
+    if (pos >= 0) {
+        pos += hi ? M2 : 0;
+        if (pos >= M) pos = -1;
+    }
+    return r;
+}
+
+// Sum of N doubles over the wavefront, scattered over the lanes: returns
+// the total of value q on exactly one lane, with q in *q, and q = -1 on
+// the other lanes.  About N shuffles instead of the 6N of a butterfly per
+// value; fixed order, so bitwise reproducible.  v is clobbered.
+template <int N>
+__device__ __forceinline__ double wave_reduce_scatter(double (&v)[N], int& q) {
+    static_assert(N >= 1 && N <= WAVE, "one value per lane at most");
+    q = 0;
+    return wave_rs_stage<N, WAVE / 2>(v, (int)(threadIdx.x & (WAVE - 1)), q);
+}
+
 // ------------------------------------------------- XCD-aware cell order
 // The workgroups of one launch are dealt to the eight XCDs round-robin
 // (workgroup b to XCD b % 8), and each XCD has its own L2.  xcd_block

@@ -24,6 +24,8 @@ struct Launch2D {
     const void *unode, *vnode, *wnode;       // per-node tables in the arithmetic type
     const double *unode_d, *vnode_d, *wnode_d;  // FP64 tables (moments)
     const double *u1, *v1;                   // 1D node lists [nv] (double)
+    const double *wx, *wy;   // weight factors w[i + nv j] = wx[i] wy[j]; null if the
+                             // weights do not factor
     const double *acc_cell, *acc_v, *acc_h;
     unsigned nwall;      // number of wall faces (non-periodic sides)
     int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline

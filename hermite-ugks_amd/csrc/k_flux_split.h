@@ -16,7 +16,9 @@
 //                  (flux.f90:93-147, get_heat_flux)
 //   K3 k_fmaxw     one wavefront per face, no distribution reads: the
 //                  Maxwellian sums of the force compatibility
-//                  (flux.f90:691-705) and of the Shakhov terms
+//                  (flux.f90:691-705) and of the Shakhov terms;
+//                  k_fmaxw_sep computes the same sums from 1D moments
+//                  when the quadrature is a tensor product
 //   K4 k_fslope_t  one lane per face: time slope aT (flux.f90:126-135) and
 //                  the conserved flux (flux.f90:152-246)
 //   K5 k_fnode     one wavefront per face, second read of the inputs: the
@@ -24,8 +26,9 @@
 //
 // The scalar moment algebra runs once per face on one lane instead of on
 // every lane of the face's wavefront, and the distribution functions are
-// read twice instead of three times.  Velocity sums are FP64 with a fixed
-// butterfly order (bitwise reproducible).
+// read twice instead of three times.  Velocity sums are FP64 in a fixed
+// order (bitwise reproducible), reduced over the wavefront by a
+// reduce-scatter.
 #pragma once
 #include "hugks_device.h"
 #include "hugks_kernels.h"
@@ -65,17 +68,6 @@ __device__ __forceinline__ FaceGeo face_geo(const Consts& c, int g, int nvi) {
         r.cR = i + c.nx * (j % c.ny);
     }
     return r;
-}
-
-template <int NV>
-__device__ __forceinline__ void wave_sum_bcast(double (&v)[NV]) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double x = v[i];
-#pragma unroll
-        for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
-        v[i] = __shfl(x, 0, WAVE);
-    }
 }
 
 // ------------------------------------------------------------------ K1
@@ -157,14 +149,9 @@ __device__ __forceinline__ void fsum_face(const Consts& c, const FaceGeo& fg, do
         s[35] += (vt2 * vt) * a;
         s[36] += vt * g;
     }
-    wave_sum_bcast<FSUM_N>(s);
-    if (lane < FSUM_N) {
-        double x = s[0];
-#pragma unroll
-        for (int q = 1; q < FSUM_N; ++q)
-            if (lane == q) x = s[q];
-        out[lane] = x;
-    }
+    int q;
+    const double x = wave_reduce_scatter<FSUM_N>(s, q);
+    if (q >= 0) out[q] = x;
 }
 
 template <typename T>
@@ -338,14 +325,9 @@ __device__ __forceinline__ void fmaxw_face(const Consts& c, const double* __rest
         s[11] += cu * P3B;
         s[12] += cv * P3B;
     }
-    wave_sum_bcast<FMAXW_N>(s);
-    if (lane < FMAXW_N) {
-        double x = s[0];
-#pragma unroll
-        for (int q = 1; q < FMAXW_N; ++q)
-            if (lane == q) x = s[q];
-        out[lane] = x;
-    }
+    int q;
+    const double x = wave_reduce_scatter<FMAXW_N>(s, q);
+    if (q >= 0) out[q] = x;
 }
 
 template <typename T>
@@ -366,6 +348,106 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fmaxw(
         fmaxw_face<0, T>(c, fc, fn, ft, und, vnd, wnd, out);
     else
         fmaxw_face<1, T>(c, fc, fn, ft, und, vnd, wnd, out);
+}
+
+// K3 for a tensor-product quadrature, one lane per face.  With nodes
+// (u_i, v_j), weights wx_i wy_j and the Maxwellian
+// exp(-lam (cu^2 + cv^2)) = exp(-lam cu^2) exp(-lam cv^2), every sum of
+// fmaxw_face is a short sum of products <f>_n <g>_t of 1D moments along
+// the face normal (cu = vn - U) and tangent (cv = vt - V):
+//   n[]: <1> <cu> <vn> <cu vn> <cu vn^2> <vn^2> <cu vn q5> <vn q5>
+//        <cu vn^2 q5> <vn^2 q5> <cu vn^3 q5> <cu vn^3> <vn^3 q5> <vn^3>
+//        <cu vn q3> <vn q3>,   q5 = 2 lam cu^2 + K - 5, q3 = 2 lam cu^2 + K - 3
+//   t[]: <1> <cv> <vt> <cv vt> <vt^2> <cv vt^2> <r> <cv r> <vt r> <cv vt r>
+//        <vt^2 r> <cv vt^2 r>,  r = 2 lam cv^2
+// (2 lam cp^2 + K - 5 = q5 + r, c^2 = vn^2 + vt^2).  2 nv exponentials and
+// about 30 nv multiply-adds per face instead of nv^2 of each.
+__global__ __launch_bounds__(256) void k_fmaxw_sep(Consts c, const DevState* __restrict__ st,
+                                                   const double* __restrict__ u1,
+                                                   const double* __restrict__ v1,
+                                                   const double* __restrict__ wx,
+                                                   const double* __restrict__ wy,
+                                                   const double* __restrict__ acc_v,
+                                                   const double* __restrict__ acc_h,
+                                                   const double* __restrict__ fcoef,
+                                                   double* __restrict__ fmaxw, int nvi,
+                                                   int nint) {
+    if (st->stopped) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    const double* acc = fg.dir == 0 ? acc_v : acc_h;
+    const double fn = acc[2 * fg.face], ft = acc[2 * fg.face + 1];
+    const double* fc = fcoef + (size_t)g * FCOEF_LD;
+    const double rho = fc[FC_P], U = fc[FC_P + 1], V = fc[FC_P + 2], lam = fc[FC_P + 3];
+    // vertical face: normal u, tangent v; horizontal face: normal v, tangent -u
+    const double* xn = fg.dir == 0 ? u1 : v1;
+    const double* wn = fg.dir == 0 ? wx : wy;
+    const double* xt = fg.dir == 0 ? v1 : u1;
+    const double* wt = fg.dir == 0 ? wy : wx;
+    const double tsgn = fg.dir == 0 ? 1.0 : -1.0;
+    const double tl = 2.0 * lam, a5 = c.K - 5.0, a3 = c.K - 3.0;
+    double n[16], t[12];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) n[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) t[q] = 0.0;
+    for (int i = 0; i < c.nv; ++i) {
+        const double vn = xn[i], cu = vn - U, cu2 = cu * cu;
+        const double e = wn[i] * exp(-lam * cu2);
+        const double vn2 = vn * vn, vn3 = vn2 * vn;
+        const double q5 = tl * cu2 + a5, q3 = tl * cu2 + a3;
+        const double ecu = e * cu;
+        n[0] += e;
+        n[1] += ecu;
+        n[2] += e * vn;
+        n[3] += ecu * vn;
+        n[4] += ecu * vn2;
+        n[5] += e * vn2;
+        n[6] += ecu * vn * q5;
+        n[7] += e * vn * q5;
+        n[8] += ecu * vn2 * q5;
+        n[9] += e * vn2 * q5;
+        n[10] += ecu * vn3 * q5;
+        n[11] += ecu * vn3;
+        n[12] += e * vn3 * q5;
+        n[13] += e * vn3;
+        n[14] += ecu * vn * q3;
+        n[15] += e * vn * q3;
+    }
+    for (int j = 0; j < c.nv; ++j) {
+        const double vt = tsgn * xt[j], cv = vt - V, cv2 = cv * cv;
+        const double e = wt[j] * exp(-lam * cv2);
+        const double r = tl * cv2, vt2 = vt * vt, ecv = e * cv;
+        t[0] += e;
+        t[1] += ecv;
+        t[2] += e * vt;
+        t[3] += ecv * vt;
+        t[4] += e * vt2;
+        t[5] += ecv * vt2;
+        t[6] += e * r;
+        t[7] += ecv * r;
+        t[8] += e * vt * r;
+        t[9] += ecv * vt * r;
+        t[10] += e * vt2 * r;
+        t[11] += ecv * vt2 * r;
+    }
+    const double R = rho * (lam / c.pi), lR = lam * R, kfac = c.K / (2.0 * lam);
+    double* out = fmaxw + (size_t)g * FMAXW_LD;
+    out[0] = lR * (fn * n[3] * t[0] + ft * n[2] * t[1]);
+    out[1] = lR * (fn * n[1] * t[2] + ft * n[0] * t[3]);
+    out[2] = lR * (fn * (n[4] * t[0] + n[1] * t[4] + kfac * n[1] * t[0]) +
+                   ft * (n[5] * t[1] + n[0] * t[5] + kfac * n[0] * t[1]));
+    out[3] = R * (n[6] * t[0] + n[3] * t[6]);
+    out[4] = R * (n[7] * t[1] + n[2] * t[7]);
+    out[5] = R * (n[8] * t[0] + n[4] * t[6]);
+    out[6] = R * (n[9] * t[1] + n[5] * t[7]);
+    out[7] = R * (n[6] * t[2] + n[3] * t[8]);
+    out[8] = R * (n[7] * t[3] + n[2] * t[9]);
+    out[9] = R * (n[10] * t[0] + n[11] * t[6] + n[6] * t[4] + n[3] * t[10]);
+    out[10] = R * (n[12] * t[1] + n[13] * t[7] + n[7] * t[5] + n[2] * t[11]);
+    out[11] = kfac * R * (n[14] * t[0] + n[3] * t[6]);
+    out[12] = kfac * R * (n[15] * t[1] + n[2] * t[7]);
 }
 
 // ------------------------------------------------------------------ K4

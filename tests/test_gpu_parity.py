@@ -101,10 +101,12 @@ def test_advance_until_output_time(have_gpu):
     assert np.all(field_err(w, o2.get()[0]) <= TOL_FP64)
 
 
-@pytest.mark.parametrize("mode", ["fused", "split"])
+@pytest.mark.parametrize("mode", ["fused", "split", "split-nodes"])
 def test_flux_kernels_agree(have_gpu, monkeypatch, mode):
-    """Both interior-face flux implementations (one fused kernel; the
-    five-kernel pipeline) against the oracle, FP64 and mixed."""
+    """The interior-face flux implementations (one fused kernel; the
+    five-kernel pipeline with the Maxwellian sums from 1D moments of the
+    tensor-product quadrature, or node by node) against the oracle, FP64
+    and mixed."""
     monkeypatch.setenv("HUGKS_FLUX_KERNEL", mode)
     for prec, tol in ((capi.FP64, TOL_FP64), (capi.MIXED_FP32, TOL_MIXED)):
         case = cases.build_2d(rt(nx=10, ny=12, nv=16, nm=8), prec, rt_variant="cartesian", seed=5)
@@ -126,3 +128,16 @@ def test_update_kernels_agree(have_gpu, monkeypatch, mode):
         err = field_err(r["w"], r["wr"])
         print(mode, prec, "W err", err, flush=True)
         assert np.all(err <= tol), err
+
+
+def test_flux_weights_not_a_tensor_product(have_gpu):
+    """Weights perturbed by 1e-3 so that they no longer factor as
+    wx[i] wy[j]: the pipeline must take the node-by-node Maxwellian sums
+    (the 1D-moment form would be off by about 1e-4)."""
+    case = cases.build_2d(rt(nx=8, ny=10, nv=14, nm=8), capi.FP64, rt_variant="cartesian", seed=6)
+    w = case.arrays["weight"]
+    w *= 1.0 + 1e-3 * np.random.default_rng(3).uniform(-1.0, 1.0, w.shape)
+    r = run_pair(case, 6)
+    err = field_err(r["w"], r["wr"])
+    print("perturbed weights W err", err, flush=True)
+    assert np.all(err <= TOL_FP64), err
