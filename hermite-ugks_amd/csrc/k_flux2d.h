@@ -51,28 +51,46 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
         }
         inv_den[dir] = (A)1 / ((A)0.5 * len + (A)0.5 * len);
     }
+    // 16-byte vectors of VW nodes: the padded row length (a multiple of 64)
+    // keeps every vector aligned and inside the row; the pad nodes are
+    // computed too and never read
+    constexpr int VW = 16 / (int)sizeof(T);
+    struct alignas(16) Vec { T v[VW]; };
+    const int ng = (c.nn + VW - 1) / VW;
     const size_t o = (size_t)cell * c.ldv;
     const size_t oL[2] = {(size_t)cl[0] * c.ldv, (size_t)cl[1] * c.ldv};
     const size_t oR[2] = {(size_t)cr[0] * c.ldv, (size_t)cr[1] * c.ldv};
-    for (int k = threadIdx.x; k < c.nn; k += 256) {
+    for (int g = threadIdx.x; g < ng; g += 256) {
+        const size_t k = (size_t)g * VW;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const T* f = q == 0 ? h : b;
-            const A fn = (A)f[o + k];
+            const Vec fc = *reinterpret_cast<const Vec*>(f + o + k);
+            Vec vl[2], vr[2];
 #pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
-                const A fl = (A)f[oL[dir] + k], fr = (A)f[oR[dir] + k];
-                A s;
-                if (inner[dir]) {
-                    // van Leer limiter (2d/src/solver.f90:146-151)
-                    const A sL = (fn - fl) * inv_den[dir], sR = (fr - fn) * inv_den[dir];
-                    s = (copysign((A)1, sR) + copysign((A)1, sL)) * fabs(sR) * fabs(sL) /
-                        (fabs(sR) + fabs(sL) + smv);
-                } else {
-                    s = (fr - fl) * inv_den[dir];
+                vl[dir] = *reinterpret_cast<const Vec*>(f + oL[dir] + k);
+                vr[dir] = *reinterpret_cast<const Vec*>(f + oR[dir] + k);
+            }
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {
+                Vec so;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    const A fn = (A)fc.v[e], fl = (A)vl[dir].v[e], fr = (A)vr[dir].v[e];
+                    A s;
+                    if (inner[dir]) {
+                        // van Leer limiter (2d/src/solver.f90:146-151)
+                        const A sL = (fn - fl) * inv_den[dir], sR = (fr - fn) * inv_den[dir];
+                        s = (copysign((A)1, sR) + copysign((A)1, sL)) * fabs(sR) * fabs(sL) /
+                            (fabs(sR) + fabs(sL) + smv);
+                    } else {
+                        s = (fr - fl) * inv_den[dir];
+                    }
+                    so.v[e] = (T)s;
                 }
                 T* out = dir == 0 ? (q == 0 ? shx : sbx) : (q == 0 ? shy : sby);
-                out[o + k] = (T)s;
+                *reinterpret_cast<Vec*>(out + o + k) = so;
             }
         }
     }

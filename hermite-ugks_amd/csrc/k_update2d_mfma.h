@@ -14,8 +14,9 @@
 // He_s(eta_jv) for r, s = 0..15 (so Nm <= 14).  One workgroup (four
 // wavefronts) per cell; the tiles of each contraction are shared out over
 // the wavefronts, with a workgroup barrier between contractions.  Operands
-// and results pass through LDS; the node-wise predictor, Maxwellians and
-// implicit update are those of k_update2d.h.
+// and results pass through LDS.  The per-cell predictor and coefficients
+// come from k_cell_pre; the node-wise Maxwellians, Shakhov terms and force
+// weight are products and sums of per-axis records (MR_*).
 #pragma once
 #include "hugks_device.h"
 #include "hugks_kernels.h"
@@ -47,16 +48,32 @@ template <> struct Mfma<double> {
 constexpr int MF_NT = 256;   // threads per workgroup (one cell)
 constexpr int MF_NB = 16;    // Hermite basis rows He_0..He_15
 
+// r! for r = 0..15 (exact in double)
+static __constant__ const double kFact[MF_NB] = {
+    1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0, 362880.0, 3628800.0, 39916800.0,
+    479001600.0, 6227020800.0, 87178291200.0, 1307674368000.0};
+
 __host__ __device__ constexpr int mf_np(int nv) { return (nv + 15) / 16 * 16; }
 
-// LDS elements (of T) of one cell: G/F [2][nn], Hx, Hy [16][NP],
-// Tm [2][16][NP], D [2][16][16], R [2][16][NP]; then, as doubles, the
-// per-axis factors of the two Maxwellians and of the force weight,
-// [6][NP] (counted here in T elements).
-__host__ __device__ inline size_t mf_lds_elems(int nv) {
+// Per-axis record of the node-wise update (doubles): for the u axis (TX,
+// by iu) and the v axis (TY, by jv) every factor that depends on that
+// velocity component alone, so that a node's Maxwellians, Shakhov terms
+// and force weight are a few products and sums of one TX and one TY entry:
+//   MR_E0, MR_E1  exp(-lam c^2) at t^n, t^{n+1}; TX also carries rho lam / pi
+//   MR_X0, MR_X1  2 lam c^2, TX also K - 5: their sum is 2 lam |c|^2 + K - 5
+//   MR_C          peculiar velocity at t^n (heat flux)
+//   MR_OM         exp(-xi^2 / 2), TX also dt / (2 pi sqrt(RT)): the force weight
+//   MR_Q0, MR_Q1  Shakhov projection cs q_x c_u (TX), cs q_y c_v (TY)
+constexpr int MF_REC = 8;
+enum : int { MR_E0 = 0, MR_X0 = 1, MR_E1 = 2, MR_X1 = 3, MR_C = 4, MR_OM = 5, MR_Q0 = 6, MR_Q1 = 7 };
+
+// LDS bytes of one cell: G/F [2][nn], Hx, Hy [16][NP], Tm [2][16][NP],
+// D [2][16][16], R [2][16][NP] (elements of T); 16-byte alignment; the
+// per-axis records TX, TY [NP][MF_REC] (doubles).
+__host__ __device__ inline size_t mf_lds_bytes(int nv, size_t tsize) {
     const size_t np = mf_np(nv), nn = (size_t)nv * nv;
-    return 2 * nn + 2 * MF_NB * np + 2 * MF_NB * np + 2 * MF_NB * MF_NB + 2 * MF_NB * np + 1 +
-           6 * np * 2;   // doubles take at most two T elements; +1 aligns them
+    return tsize * (2 * nn + 2 * MF_NB * np + 2 * MF_NB * np + 2 * MF_NB * MF_NB + 2 * MF_NB * np) +
+           16 + 2 * MF_REC * np * sizeof(double);
 }
 
 // Per-cell scalars of the update, one lane per cell, computed once instead
@@ -143,7 +160,6 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb,
     const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ wnode_d,
-    const double* __restrict__ und, const double* __restrict__ vnd,
     const double* __restrict__ u1, const double* __restrict__ v1) {
     using A = typename Arith<T>::A;
     using M = Mfma<T>;
@@ -176,47 +192,18 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     T* Tm = Hy + MF_NB * np;                      // [2][16][np]
     T* D = Tm + 2 * MF_NB * np;                   // [2][16][16]
     T* R = D + 2 * MF_NB * MF_NB;                 // [2][16][np]
-    // per-axis factors (FP64): the Maxwellians at t^n and t^{n+1} and the
-    // force weight are products of a u factor and a v factor,
-    // exp(-lam (u-U)^2 - lam (v-V)^2) = ex(u) ey(v)
-    double* Ex = reinterpret_cast<double*>(
-        (reinterpret_cast<uintptr_t>(R + 2 * MF_NB * np) + 7) & ~(uintptr_t)7);
-    double* Eo_x = Ex;                            // [np] old Maxwellian, u
-    double* Eo_y = Ex + np;                       //      v
-    double* En_x = Ex + 2 * np;                   // new Maxwellian
-    double* En_y = Ex + 3 * np;
-    double* Om_x = Ex + 4 * np;                   // force weight exp(-xi^2/2)
-    double* Om_y = Ex + 5 * np;
+    // records at the next 16-byte boundary, addressed from the LDS base so
+    // that their reads stay LDS reads
+    const int rec_off = (int)(((size_t)(R + 2 * MF_NB * np - G) * sizeof(T) + 15) / 16 * 2);
+    double* TX = smem_raw + rec_off;
+    double* TY = TX + MF_REC * np;
 
-    // ---- load f^n, weighted copies, heat flux at t^n (solver.f90:337),
-    // FP64 whatever the storage type
+    // ---- per-axis records and the Hermite bases He_0..He_15 at
+    // xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt) (build_probabilists_hermite,
+    // hermite_symmetric2d.f90:220-233); zero basis columns beyond nv
     const double rt = P[CP_RT];
     const double sqrt_rt = sqrt(rt);
-    T rh[EPT], rb[EPT];
-    double q4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-        const int k = tid + e * MF_NT;
-        rh[e] = rb[e] = (T)0;
-        if (k < nn) {
-            const size_t o = (size_t)cell * c.ldv + k;
-            rh[e] = h[o];
-            rb[e] = b[o];
-            const double wt = wnode_d[k];
-            const double cu = und[k] - po[1], cv = vnd[k] - po[2];
-            const double c2 = cu * cu + cv * cv;
-            q4[0] += wt * cu * c2 * (double)rh[e];
-            q4[1] += wt * cu * (double)rb[e];
-            q4[2] += wt * cv * c2 * (double)rh[e];
-            q4[3] += wt * cv * (double)rb[e];
-            const A wa = wnode[k];
-            G[k] = (T)(wa * (A)rh[e]);
-            G[nn + k] = (T)(wa * (A)rb[e]);
-        }
-    }
-    // Hermite bases He_0..He_15 at xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt)
-    // (build_probabilists_hermite, hermite_symmetric2d.f90:220-233); zero
-    // columns beyond nv
+    const double om_scale = dt / (2.0 * c.pi * sqrt_rt);
     for (int t = tid; t < 2 * np; t += MF_NT) {
         const bool isx = t < np;
         const int idx = isx ? t : t - np;
@@ -224,10 +211,15 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
         if (idx < nv) {
             const double xv = isx ? u1[idx] : v1[idx];
             const double co = xv - (isx ? po[1] : po[2]), cn = xv - (isx ? pn[1] : pn[2]);
-            (isx ? Eo_x : Eo_y)[idx] = exp(-po[3] * co * co);
-            (isx ? En_x : En_y)[idx] = exp(-pn[3] * cn * cn);
+            double* rec = (isx ? TX : TY) + MF_REC * idx;
+            const double co2 = co * co, cn2 = cn * cn;
+            rec[MR_E0] = (isx ? P[CP_HCO] : 1.0) * exp(-po[3] * co2);
+            rec[MR_X0] = 2.0 * po[3] * co2 + (isx ? c.K - 5.0 : 0.0);
+            rec[MR_E1] = (isx ? P[CP_HCN] : 1.0) * exp(-pn[3] * cn2);
+            rec[MR_X1] = 2.0 * pn[3] * cn2 + (isx ? c.K - 5.0 : 0.0);
+            rec[MR_C] = co;
             const A x = (A)(co / sqrt_rt);
-            (isx ? Om_x : Om_y)[idx] = (double)fexp((A)-0.5 * x * x);
+            rec[MR_OM] = (isx ? om_scale : 1.0) * (double)fexp((A)-0.5 * x * x);
             A p0 = (A)1, p1 = x;
             Hb[idx] = (T)p0;
             Hb[np + idx] = (T)p1;
@@ -241,8 +233,49 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             for (int n = 0; n < MF_NB; ++n) Hb[n * np + idx] = (T)0;
         }
     }
+    __syncthreads();
+
+    // ---- load f^n, weighted copies, heat flux at t^n (solver.f90:337),
+    // FP64 whatever the storage type.  Node (iu, jv) of k = iu + nv*jv
+    // without an integer division: (k + 1/2)/nv stays at least 1/(2 nv)
+    // away from an integer.
+    const float rnv = 1.0f / (float)nv;
+    T rh[EPT], rb[EPT];
+    double q4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int k = tid + e * MF_NT;
+        rh[e] = rb[e] = (T)0;
+        if (k < nn) {
+            const size_t o = (size_t)cell * c.ldv + k;
+            rh[e] = h[o];
+            rb[e] = b[o];
+            const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
+            const double wt = wnode_d[k];
+            const double cu = TX[MF_REC * iu + MR_C], cv = TY[MF_REC * jv + MR_C];
+            const double c2 = cu * cu + cv * cv;
+            const double wh = wt * (double)rh[e], wb = wt * (double)rb[e];
+            q4[0] += cu * c2 * wh;
+            q4[1] += cu * wb;
+            q4[2] += cv * c2 * wh;
+            q4[3] += cv * wb;
+            const A wa = wnode[k];
+            G[k] = (T)(wa * (A)rh[e]);
+            G[nn + k] = (T)(wa * (A)rb[e]);
+        }
+    }
     block_sum<4, MF_NT>(q4, red);   // also orders the LDS writes above
     const double qf0 = 0.5 * (q4[0] + q4[1]), qf1 = 0.5 * (q4[2] + q4[3]);
+    // Shakhov projections (read after the barriers of the contractions)
+    for (int t = tid; t < 2 * nv; t += MF_NT) {
+        const bool isx = t < nv;
+        const int idx = isx ? t : t - nv;
+        double* rec = (isx ? TX : TY) + MF_REC * idx;
+        const double qa = isx ? qf0 : qf1;
+        const double cn = (isx ? u1[idx] : v1[idx]) - (isx ? pn[1] : pn[2]);
+        rec[MR_Q0] = P[CP_CSO] * qa * rec[MR_C];
+        rec[MR_Q1] = P[CP_CSN] * qa * cn;
+    }
 
     const int nt = np / 16;             // tiles along a velocity index
     const int ksn = (nv + 3) / 4;       // K steps over a velocity index
@@ -278,12 +311,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
         for (int v = 0; v < 4; ++v) {
             const int r = M::row(lane, v), s = lr;
             T val = (T)0;
-            if (r + s <= nm) {
-                double fr = 1.0, fs = 1.0;
-                for (int q = 2; q <= r; ++q) fr *= (double)q;
-                for (int q = 2; q <= s; ++q) fs *= (double)q;
-                val = (T)((A)(jac / (fr * fs)) * (A)acc[v]);
-            }
+            if (r + s <= nm) val = (T)((A)(jac / (kFact[r] * kFact[s])) * (A)acc[v]);
             D[(f * MF_NB + r) * MF_NB + s] = val;
         }
     }
@@ -327,48 +355,47 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     }
     __syncthreads();
 
-    // ---- force and implicit BGK update (solver.f90:337-388)
-    const A inv_sqrt_rt = (A)(1.0 / sqrt_rt);
-    const A two_pi = (A)(2.0 * c.pi);
-    const double Hco = P[CP_HCO], Hcn = P[CP_HCN];
+    // ---- force and implicit BGK update (solver.f90:337-388):
+    //   H' = H (1 + cq (2 lam |c|^2 + K - 5)),  B' = kf (H' + 2 cq H),
+    //   h^{n+1} = (h + div + dt/2 (H'^{n+1}/tau + (H'^n - h)/tau^n) + dt F) / (1 + dt/(2 tau))
     const double kfo = P[CP_KFO], kfn = P[CP_KFN];
-    const double cso = P[CP_CSO], csn = P[CP_CSN];
-    const double half_dt = 0.5 * dt;
-    const double itau = P[CP_ITAU], itau_o = P[CP_ITAUO];
+    const double a_n = 0.5 * dt * P[CP_ITAU], a_o = 0.5 * dt * P[CP_ITAUO];
     const double iden = P[CP_IDEN];
-    const double iarea = 1.0 / area;
-    const float rnv = 1.0f / (float)nv;
+    const A iarea = (A)(1.0 / area);
+    const T* fh_l = vfh + (size_t)vl * c.ldv;
+    const T* fh_r = vfh + (size_t)vr * c.ldv;
+    const T* fh_d = hfh + (size_t)hd * c.ldv;
+    const T* fh_u = hfh + (size_t)hu * c.ldv;
+    const T* fb_l = vfb + (size_t)vl * c.ldv;
+    const T* fb_r = vfb + (size_t)vr * c.ldv;
+    const T* fb_d = hfb + (size_t)hd * c.ldv;
+    const T* fb_u = hfb + (size_t)hu * c.ldv;
+    T* ho_p = h + (size_t)cell * c.ldv;
+    T* bo_p = b + (size_t)cell * c.ldv;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = tid + e * MF_NT;
         if (k < nn) {
-            // node (iu, jv) of k = iu + nv*jv without an integer division:
-            // (k + 1/2)/nv stays at least 1/(2 nv) away from an integer
             const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
-            const double u = und[k], v = vnd[k];
-            const A om = (A)(Om_x[iu] * Om_y[jv]) / two_pi;
-            const double Fh = (double)(-inv_sqrt_rt * om * (A)F[k]);
-            const double Fb = (double)(-inv_sqrt_rt * om * (A)F[nn + k]);
-            const double cuo = u - po[1], cvo = v - po[2], c2o = cuo * cuo + cvo * cvo;
-            const double Ho = Hco * (Eo_x[iu] * Eo_y[jv]), Bo = Ho * kfo;
-            const double cqo = cso * (cuo * qf0 + cvo * qf1);
-            const double Hpo = Ho + cqo * (2.0 * po[3] * c2o + c.K - 5.0) * Ho;
-            const double Bpo = Bo + cqo * (2.0 * po[3] * c2o + c.K - 3.0) * Bo;
-            const double cun = u - pn[1], cvn = v - pn[2], c2n = cun * cun + cvn * cvn;
-            const double Hn = Hcn * (En_x[iu] * En_y[jv]), Bn = Hn * kfn;
-            const double cqn = csn * (cun * qf0 + cvn * qf1);
-            const double Hpn = Hn + cqn * (2.0 * pn[3] * c2n + c.K - 5.0) * Hn;
-            const double Bpn = Bn + cqn * (2.0 * pn[3] * c2n + c.K - 3.0) * Bn;
-            const double dfh = ((double)vfh[(size_t)vl * c.ldv + k] - (double)vfh[(size_t)vr * c.ldv + k] +
-                                (double)hfh[(size_t)hd * c.ldv + k] - (double)hfh[(size_t)hu * c.ldv + k]) * iarea;
-            const double dfb = ((double)vfb[(size_t)vl * c.ldv + k] - (double)vfb[(size_t)vr * c.ldv + k] +
-                                (double)hfb[(size_t)hd * c.ldv + k] - (double)hfb[(size_t)hu * c.ldv + k]) * iarea;
+            const double* rx = TX + MF_REC * iu;
+            const double* ry = TY + MF_REC * jv;
+            const double Ho = rx[MR_E0] * ry[MR_E0];
+            const double cqo = rx[MR_Q0] + ry[MR_Q0];
+            const double Hpo = fma(Ho, cqo * (rx[MR_X0] + ry[MR_X0]), Ho);
+            const double Bpo = kfo * fma(2.0 * cqo, Ho, Hpo);
+            const double Hn = rx[MR_E1] * ry[MR_E1];
+            const double cqn = rx[MR_Q1] + ry[MR_Q1];
+            const double Hpn = fma(Hn, cqn * (rx[MR_X1] + ry[MR_X1]), Hn);
+            const double Bpn = kfn * fma(2.0 * cqn, Hn, Hpn);
+            const A om = (A)(rx[MR_OM] * ry[MR_OM]);
+            const double gh = (double)(om * (A)F[k]), gb = (double)(om * (A)F[nn + k]);
+            const double dfh = (double)(((A)fh_l[k] - (A)fh_r[k] + (A)fh_d[k] - (A)fh_u[k]) * iarea);
+            const double dfb = (double)(((A)fb_l[k] - (A)fb_r[k] + (A)fb_d[k] - (A)fb_u[k]) * iarea);
             const double ho = (double)rh[e], bo = (double)rb[e];
-            const double hn = (ho + dfh + half_dt * (Hpn * itau + (Hpo - ho) * itau_o) - Fh * dt) * iden;
-            const double bn = (bo + dfb + half_dt * (Bpn * itau + (Bpo - bo) * itau_o) - Fb * dt) * iden;
-            const size_t o = (size_t)cell * c.ldv + k;
-            h[o] = (T)hn;
-            b[o] = (T)bn;
+            const double hn = (ho + dfh + a_n * Hpn + a_o * (Hpo - ho) + gh) * iden;
+            const double bn = (bo + dfb + a_n * Bpn + a_o * (Bpo - bo) + gb) * iden;
+            ho_p[k] = (T)hn;
+            bo_p[k] = (T)bn;
         }
     }
 }

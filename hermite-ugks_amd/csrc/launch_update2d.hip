@@ -49,7 +49,7 @@ static hipError_t launch_update_t(const Launch2D& L, hipStream_t s, bool prepare
 bool update2d_mfma_supported(const Consts& c) { return c.nv <= 48 && c.nm <= MF_NB - 2; }
 
 size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64) {
-    return mf_lds_elems(c.nv) * (fp64 ? sizeof(double) : sizeof(float));
+    return mf_lds_bytes(c.nv, fp64 ? sizeof(double) : sizeof(float));
 }
 
 template <typename T>
@@ -67,7 +67,7 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool pr
         HookScope hs(L, "update2d");                                                            \
         hipLaunchKernelGGL((k_update2d_mfma<T, E>), dim3(ncell), dim3(MF_NT), lds, s, c, L.st,  \
             L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,     \
-            (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.unode_d, L.vnode_d, L.u1, L.v1);   \
+            (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.u1, L.v1);   \
         break;                                                                                  \
     }
     if (!prepare) {
