@@ -55,25 +55,53 @@ static __constant__ const double kFact[MF_NB] = {
 
 __host__ __device__ constexpr int mf_np(int nv) { return (nv + 15) / 16 * 16; }
 
-// Per-axis record of the node-wise update (doubles): for the u axis (TX,
-// by iu) and the v axis (TY, by jv) every factor that depends on that
-// velocity component alone, so that a node's Maxwellians, Shakhov terms
-// and force weight are a few products and sums of one TX and one TY entry:
+// Per-axis records of the node-wise update (doubles, one array of NP per
+// slot): for the u axis (TX, by iu) and the v axis (TY, by jv) every factor
+// that depends on that velocity component alone, so that a node's
+// Maxwellians, Shakhov terms and force weight are a few products and sums
+// of one TX and one TY entry:
 //   MR_E0, MR_E1  exp(-lam c^2) at t^n, t^{n+1}; TX also carries rho lam / pi
 //   MR_X0, MR_X1  2 lam c^2, TX also K - 5: their sum is 2 lam |c|^2 + K - 5
-//   MR_C          peculiar velocity at t^n (heat flux)
+//   MR_CQ0        peculiar velocity at t^n (heat flux), then the Shakhov
+//                 projection cs q_x c_u (TX), cs q_y c_v (TY) at t^n
 //   MR_OM         exp(-xi^2 / 2), TX also dt / (2 pi sqrt(RT)): the force weight
-//   MR_Q0, MR_Q1  Shakhov projection cs q_x c_u (TX), cs q_y c_v (TY)
-constexpr int MF_REC = 8;
-enum : int { MR_E0 = 0, MR_X0 = 1, MR_E1 = 2, MR_X1 = 3, MR_C = 4, MR_OM = 5, MR_Q0 = 6, MR_Q1 = 7 };
+//   MR_Q1         Shakhov projection at t^{n+1}
+constexpr int MF_REC = 7;
+enum : int { MR_E0 = 0, MR_X0 = 1, MR_E1 = 2, MR_X1 = 3, MR_CQ0 = 4, MR_OM = 5, MR_Q1 = 6 };
 
-// LDS bytes of one cell: G/F [2][nn], Hx, Hy [16][NP], Tm [2][16][NP],
-// D [2][16][16], R [2][16][NP] (elements of T); 16-byte alignment; the
-// per-axis records TX, TY [NP][MF_REC] (doubles).
+// LDS layout of one cell (offsets in elements of T, records in doubles).
+// Row strides are chosen for conflict-free operand reads: a tile operand
+// read as 16 rows x 4 consecutive columns needs a row stride of 4 x odd
+// (Hx, Hy, Tm, D, G), one read as 4 rows x 16 consecutive columns an odd
+// multiple of 16 (R).
+//   G [2][nv][ldG]  weight * f, later F [2][nn] (unpadded)
+//   Hx, Hy [16][sH]   Hermite bases
+//   Tm [2][16][sH]  first contraction; R [2][16][sR] reuses it
+//   D [2][16][sD]
+//   TX, TY [MF_REC][NP] (doubles, 16-byte aligned)
+struct MfLayout {
+    int np, sH, sR, sD, ldG;
+    int oHx, oHy, oTm, oD, endT;
+};
+__host__ __device__ inline MfLayout mf_layout(int nv, int tsize) {
+    MfLayout L;
+    L.np = mf_np(nv);
+    L.sH = L.np + 4;
+    L.sR = ((L.np / 16) & 1) ? L.np : L.np + 16;
+    L.sD = MF_NB + 4;
+    L.ldG = (nv + 3) / 4 * 4;
+    if (((L.ldG / 4) & 1) == 0) L.ldG += 4;
+    L.oHx = 2 * nv * L.ldG;
+    L.oHy = L.oHx + MF_NB * L.sH;
+    L.oTm = L.oHy + MF_NB * L.sH;
+    L.oD = L.oTm + 2 * MF_NB * (L.sH > L.sR ? L.sH : L.sR);
+    const int end = L.oD + 2 * MF_NB * L.sD;
+    L.endT = (end * tsize + 15) / 16 * 16 / tsize;   // records start 16-byte aligned
+    return L;
+}
 __host__ __device__ inline size_t mf_lds_bytes(int nv, size_t tsize) {
-    const size_t np = mf_np(nv), nn = (size_t)nv * nv;
-    return tsize * (2 * nn + 2 * MF_NB * np + 2 * MF_NB * np + 2 * MF_NB * MF_NB + 2 * MF_NB * np) +
-           16 + 2 * MF_REC * np * sizeof(double);
+    const MfLayout L = mf_layout(nv, (int)tsize);
+    return (size_t)L.endT * tsize + 2 * MF_REC * L.np * sizeof(double);
 }
 
 // Per-cell scalars of the update, one lane per cell, computed once instead
@@ -186,17 +214,17 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     const double ax = P[CP_AX], ay = P[CP_AY];
 
     // ---- LDS carve-up
-    T* G = reinterpret_cast<T*>(smem_raw);        // [2][nn]; later F
-    T* Hx = G + 2 * nn;                           // [16][np]
-    T* Hy = Hx + MF_NB * np;                      // [16][np]
-    T* Tm = Hy + MF_NB * np;                      // [2][16][np]
-    T* D = Tm + 2 * MF_NB * np;                   // [2][16][16]
-    T* R = D + 2 * MF_NB * MF_NB;                 // [2][16][np]
-    // records at the next 16-byte boundary, addressed from the LDS base so
-    // that their reads stay LDS reads
-    const int rec_off = (int)(((size_t)(R + 2 * MF_NB * np - G) * sizeof(T) + 15) / 16 * 2);
-    double* TX = smem_raw + rec_off;
-    double* TY = TX + MF_REC * np;
+    const MfLayout ly = mf_layout(nv, (int)sizeof(T));
+    const int sH = ly.sH, sR = ly.sR, sD = ly.sD, ldG = ly.ldG, GS = nv * ly.ldG;
+    T* const G = reinterpret_cast<T*>(smem_raw);
+    T* const Hx = G + ly.oHx;
+    T* const Hy = G + ly.oHy;
+    T* const Tm = G + ly.oTm;
+    T* const D = G + ly.oD;
+    T* const R = Tm;       // Tm is dead once D is formed
+    T* const F = G;        // G is dead once Tm is formed
+    double* const TX = smem_raw + ly.endT * (int)sizeof(T) / 8;
+    double* const TY = TX + MF_REC * np;
 
     // ---- per-axis records and the Hermite bases He_0..He_15 at
     // xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt) (build_probabilists_hermite,
@@ -211,26 +239,26 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
         if (idx < nv) {
             const double xv = isx ? u1[idx] : v1[idx];
             const double co = xv - (isx ? po[1] : po[2]), cn = xv - (isx ? pn[1] : pn[2]);
-            double* rec = (isx ? TX : TY) + MF_REC * idx;
+            double* rec = (isx ? TX : TY) + idx;
             const double co2 = co * co, cn2 = cn * cn;
-            rec[MR_E0] = (isx ? P[CP_HCO] : 1.0) * exp(-po[3] * co2);
-            rec[MR_X0] = 2.0 * po[3] * co2 + (isx ? c.K - 5.0 : 0.0);
-            rec[MR_E1] = (isx ? P[CP_HCN] : 1.0) * exp(-pn[3] * cn2);
-            rec[MR_X1] = 2.0 * pn[3] * cn2 + (isx ? c.K - 5.0 : 0.0);
-            rec[MR_C] = co;
+            rec[MR_E0 * np] = (isx ? P[CP_HCO] : 1.0) * exp(-po[3] * co2);
+            rec[MR_X0 * np] = 2.0 * po[3] * co2 + (isx ? c.K - 5.0 : 0.0);
+            rec[MR_E1 * np] = (isx ? P[CP_HCN] : 1.0) * exp(-pn[3] * cn2);
+            rec[MR_X1 * np] = 2.0 * pn[3] * cn2 + (isx ? c.K - 5.0 : 0.0);
+            rec[MR_CQ0 * np] = co;
             const A x = (A)(co / sqrt_rt);
-            rec[MR_OM] = (isx ? om_scale : 1.0) * (double)fexp((A)-0.5 * x * x);
+            rec[MR_OM * np] = (isx ? om_scale : 1.0) * (double)fexp((A)-0.5 * x * x);
             A p0 = (A)1, p1 = x;
             Hb[idx] = (T)p0;
-            Hb[np + idx] = (T)p1;
+            Hb[sH + idx] = (T)p1;
             for (int n = 2; n < MF_NB; ++n) {
                 const A p2 = x * p1 - (A)(n - 1) * p0;
-                Hb[n * np + idx] = (T)p2;
+                Hb[n * sH + idx] = (T)p2;
                 p0 = p1;
                 p1 = p2;
             }
         } else {
-            for (int n = 0; n < MF_NB; ++n) Hb[n * np + idx] = (T)0;
+            for (int n = 0; n < MF_NB; ++n) Hb[n * sH + idx] = (T)0;
         }
     }
     __syncthreads();
@@ -252,7 +280,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             rb[e] = b[o];
             const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
             const double wt = wnode_d[k];
-            const double cu = TX[MF_REC * iu + MR_C], cv = TY[MF_REC * jv + MR_C];
+            const double cu = TX[MR_CQ0 * np + iu], cv = TY[MR_CQ0 * np + jv];
             const double c2 = cu * cu + cv * cv;
             const double wh = wt * (double)rh[e], wb = wt * (double)rb[e];
             q4[0] += cu * c2 * wh;
@@ -260,8 +288,8 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             q4[2] += cv * c2 * wh;
             q4[3] += cv * wb;
             const A wa = wnode[k];
-            G[k] = (T)(wa * (A)rh[e]);
-            G[nn + k] = (T)(wa * (A)rb[e]);
+            G[jv * ldG + iu] = (T)(wa * (A)rh[e]);
+            G[GS + jv * ldG + iu] = (T)(wa * (A)rb[e]);
         }
     }
     block_sum<4, MF_NT>(q4, red);   // also orders the LDS writes above
@@ -270,11 +298,11 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     for (int t = tid; t < 2 * nv; t += MF_NT) {
         const bool isx = t < nv;
         const int idx = isx ? t : t - nv;
-        double* rec = (isx ? TX : TY) + MF_REC * idx;
+        double* rec = (isx ? TX : TY) + idx;
         const double qa = isx ? qf0 : qf1;
         const double cn = (isx ? u1[idx] : v1[idx]) - (isx ? pn[1] : pn[2]);
-        rec[MR_Q0] = P[CP_CSO] * qa * rec[MR_C];
-        rec[MR_Q1] = P[CP_CSN] * qa * cn;
+        rec[MR_CQ0 * np] = P[CP_CSO] * qa * rec[MR_CQ0 * np];
+        rec[MR_Q1 * np] = P[CP_CSN] * qa * cn;
     }
 
     const int nt = np / 16;             // tiles along a velocity index
@@ -289,12 +317,12 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
         for (int ks = 0; ks < ksn; ++ks) {
             const int iu = 4 * ks + lk;
             const bool in = iu < nv;
-            const T a = in ? Hx[lr * np + iu] : (T)0;
-            const T bb = (in && jv < nv) ? G[f * nn + jv * nv + iu] : (T)0;
+            const T a = in ? Hx[lr * sH + iu] : (T)0;
+            const T bb = (in && jv < nv) ? G[f * GS + jv * ldG + iu] : (T)0;
             acc = M::mma(a, bb, acc);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Tm[(f * MF_NB + M::row(lane, v)) * np + jv] = acc[v];
+        for (int v = 0; v < 4; ++v) Tm[(f * MF_NB + M::row(lane, v)) * sH + jv] = acc[v];
     }
     __syncthreads();
     // ---- D = Tm Hy^T / (r! s!), r + s <= Nm
@@ -303,8 +331,8 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
         V acc = {0, 0, 0, 0};
         for (int ks = 0; ks < ksn; ++ks) {
             const int jv = 4 * ks + lk;
-            const T a = Tm[(f * MF_NB + lr) * np + jv];     // zero beyond nv
-            const T bb = Hy[lr * np + jv];
+            const T a = Tm[(f * MF_NB + lr) * sH + jv];     // zero beyond nv
+            const T bb = Hy[lr * sH + jv];
             acc = M::mma(a, bb, acc);
         }
 #pragma unroll
@@ -312,7 +340,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             const int r = M::row(lane, v), s = lr;
             T val = (T)0;
             if (r + s <= nm) val = (T)((A)(jac / (kFact[r] * kFact[s])) * (A)acc[v]);
-            D[(f * MF_NB + r) * MF_NB + s] = val;
+            D[(f * MF_NB + r) * sD + s] = val;
         }
     }
     __syncthreads();
@@ -325,18 +353,17 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
 #pragma unroll
         for (int ks = 0; ks < MF_NB / 4; ++ks) {
             const int s = 4 * ks + lk;
-            const T a1 = lr >= 1 ? (T)(axA * (A)D[(f * MF_NB + lr - 1) * MF_NB + s]) : (T)0;
-            acc = M::mma(a1, Hy[s * np + jv], acc);
-            const T a2 = (T)(ayA * (A)D[(f * MF_NB + lr) * MF_NB + s]);
-            const T b2 = s + 1 < MF_NB ? Hy[(s + 1) * np + jv] : (T)0;
+            const T a1 = lr >= 1 ? (T)(axA * (A)D[(f * MF_NB + lr - 1) * sD + s]) : (T)0;
+            acc = M::mma(a1, Hy[s * sH + jv], acc);
+            const T a2 = (T)(ayA * (A)D[(f * MF_NB + lr) * sD + s]);
+            const T b2 = s + 1 < MF_NB ? Hy[(s + 1) * sH + jv] : (T)0;
             acc = M::mma(a2, b2, acc);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) R[(f * MF_NB + M::row(lane, v)) * np + jv] = acc[v];
+        for (int v = 0; v < 4; ++v) R[(f * MF_NB + M::row(lane, v)) * sR + jv] = acc[v];
     }
     __syncthreads();
     // ---- F = Hx^T R, into the G region (G is dead)
-    T* F = G;
     for (int job = wv; job < 2 * nt * nt; job += MF_NT / WAVE) {
         const int f = job / (nt * nt), rem = job % (nt * nt);
         const int it = rem / nt, jt = rem % nt;
@@ -345,7 +372,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
 #pragma unroll
         for (int ks = 0; ks < MF_NB / 4; ++ks) {
             const int r = 4 * ks + lk;
-            acc = M::mma(Hx[r * np + iu_a], R[(f * MF_NB + r) * np + jv], acc);
+            acc = M::mma(Hx[r * sH + iu_a], R[(f * MF_NB + r) * sR + jv], acc);
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -377,17 +404,17 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
         const int k = tid + e * MF_NT;
         if (k < nn) {
             const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
-            const double* rx = TX + MF_REC * iu;
-            const double* ry = TY + MF_REC * jv;
-            const double Ho = rx[MR_E0] * ry[MR_E0];
-            const double cqo = rx[MR_Q0] + ry[MR_Q0];
-            const double Hpo = fma(Ho, cqo * (rx[MR_X0] + ry[MR_X0]), Ho);
+            const double* rx = TX + iu;
+            const double* ry = TY + jv;
+            const double Ho = rx[MR_E0 * np] * ry[MR_E0 * np];
+            const double cqo = rx[MR_CQ0 * np] + ry[MR_CQ0 * np];
+            const double Hpo = fma(Ho, cqo * (rx[MR_X0 * np] + ry[MR_X0 * np]), Ho);
             const double Bpo = kfo * fma(2.0 * cqo, Ho, Hpo);
-            const double Hn = rx[MR_E1] * ry[MR_E1];
-            const double cqn = rx[MR_Q1] + ry[MR_Q1];
-            const double Hpn = fma(Hn, cqn * (rx[MR_X1] + ry[MR_X1]), Hn);
+            const double Hn = rx[MR_E1 * np] * ry[MR_E1 * np];
+            const double cqn = rx[MR_Q1 * np] + ry[MR_Q1 * np];
+            const double Hpn = fma(Hn, cqn * (rx[MR_X1 * np] + ry[MR_X1 * np]), Hn);
             const double Bpn = kfn * fma(2.0 * cqn, Hn, Hpn);
-            const A om = (A)(rx[MR_OM] * ry[MR_OM]);
+            const A om = (A)(rx[MR_OM * np] * ry[MR_OM * np]);
             const double gh = (double)(om * (A)F[k]), gb = (double)(om * (A)F[nn + k]);
             const double dfh = (double)(((A)fh_l[k] - (A)fh_r[k] + (A)fh_d[k] - (A)fh_u[k]) * iarea);
             const double dfb = (double)(((A)fb_l[k] - (A)fb_r[k] + (A)fb_d[k] - (A)fb_u[k]) * iarea);
