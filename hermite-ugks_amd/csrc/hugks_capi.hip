@@ -388,11 +388,16 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         const bool xper = d->bc[0] == HUGKS_BC_PERIODIC, yper = d->bc[2] == HUGKS_BC_PERIODIC;
         const size_t nint = (size_t)(xper ? d->nx + 1 : d->nx - 1) * d->ny +
                             (size_t)d->nx * (yper ? d->ny + 1 : d->ny - 1);
-        L.flux_mode = 1;
+        L.stencil_ok = fsz < ((size_t)1 << 32);
+        L.flux_mode = L.stencil_ok ? HUGKS_FLUX_STENCIL : HUGKS_FLUX_SPLIT;
         bool maxw_nodes = false;
         if (const char* fm = getenv("HUGKS_FLUX_KERNEL")) {
-            if (strcmp(fm, "fused") == 0) L.flux_mode = 0;
-            if (strcmp(fm, "split-nodes") == 0) maxw_nodes = true;
+            if (strcmp(fm, "fused") == 0) L.flux_mode = HUGKS_FLUX_FUSED;
+            if (strcmp(fm, "split") == 0) L.flux_mode = HUGKS_FLUX_SPLIT;
+            if (strcmp(fm, "split-nodes") == 0) {
+                L.flux_mode = HUGKS_FLUX_SPLIT;
+                maxw_nodes = true;
+            }
         }
         // Factors of a tensor-product quadrature, w[i + nv j] = wx[i] wy[j]
         // (both quadratures of velocity.f90:43,128 are such products); the
@@ -688,12 +693,14 @@ int hugks_set_timing(hugks_handle* h, int32_t on) {
 
 int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kernel) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
-    if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_SPLIT ||
+    if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_STENCIL ||
         update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA)
         return fail(h, HUGKS_E_ARG, "unknown kernel choice");
     if (h->dim != 2) return HUGKS_OK;   // 1D has one kernel of each kind
     if (update_kernel == HUGKS_UPDATE_MFMA && !update2d_mfma_supported(h->L.c))
         return fail(h, HUGKS_E_UNSUPPORTED, "MFMA update needs nv <= 48 and Nm <= 14");
+    if (flux_kernel == HUGKS_FLUX_STENCIL && !h->L.stencil_ok)
+        return fail(h, HUGKS_E_UNSUPPORTED, "stencil flux needs distribution arrays under 4 GiB");
     h->L.flux_mode = flux_kernel;
     h->L.update_mode = update_kernel;
     return HUGKS_OK;
@@ -742,7 +749,7 @@ static int tune_case(const hugks_desc* d, int* flux_kernel, int* update_kernel, 
     int bf = h->L.flux_mode, bu = h->L.update_mode;
     double best = 1e300;
     if (d->dim == 2) {
-        for (int fk = HUGKS_FLUX_FUSED; fk <= HUGKS_FLUX_SPLIT; ++fk)
+        for (int fk = HUGKS_FLUX_FUSED; fk <= HUGKS_FLUX_STENCIL; ++fk)
             for (int uk = HUGKS_UPDATE_LDS; uk <= HUGKS_UPDATE_MFMA; ++uk) {
                 if (hugks_set_schedule(h, fk, uk) != HUGKS_OK) continue;
                 hipError_t e = launch_begin(h->L.st, INFINITY, h->stream);

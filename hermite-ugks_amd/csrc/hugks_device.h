@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "hugks.h"
 
 namespace hugks {
 
@@ -232,6 +233,54 @@ __device__ __forceinline__ int tiled_cell(int L, int nx, int ny) {
     const int ws = nx - s * TILE_W < TILE_W ? nx - s * TILE_W : TILE_W;
     const int jj = rem / ws, ii = rem - jj * ws;
     return s * TILE_W + ii + nx * jj;
+}
+
+// ----------------------------------------------------------- slope stencil
+// Cells of the slope of `cell` along dir (0: x, 1: y), as interp_inner /
+// interp_boundary choose them (2d/src/solver.f90:64-159): periodic
+// directions wrap (the ghost copy of 2d/src/boundary.f90:16-53); along a
+// wall the two boundary cells take one-sided differences (inner = false).
+struct SlopeNb {
+    int cl, cr;
+    bool inner;
+};
+__host__ __device__ __forceinline__ SlopeNb slope_nb(const Consts& c, int cell, int dir) {
+    const int i = cell % c.nx, j = cell / c.nx;
+    const bool per = c.bc[2 * dir] == HUGKS_BC_PERIODIC;
+    const int n = dir == 0 ? c.nx : c.ny;
+    const int p = dir == 0 ? i : j;
+    const int step = dir == 0 ? 1 : c.nx;
+    SlopeNb r;
+    r.inner = true;
+    if (per) {
+        const int pl = (p - 1 + n) % n, pr = (p + 1) % n;
+        r.cl = dir == 0 ? pl + c.nx * j : i + c.nx * pl;
+        r.cr = dir == 0 ? pr + c.nx * j : i + c.nx * pr;
+    } else if (p == 0) {
+        r.cl = cell;
+        r.cr = cell + step;
+        r.inner = false;
+    } else if (p == n - 1) {
+        r.cl = cell - step;
+        r.cr = cell;
+        r.inner = false;
+    } else {
+        r.cl = cell - step;
+        r.cr = cell + step;
+    }
+    return r;
+}
+
+// Slope at a node from its stencil values: van Leer (solver.f90:146-151)
+// for an inner cell, the one-sided difference at a wall.
+template <typename A>
+__device__ __forceinline__ A vl_slope(A fl, A fn, A fr, bool inner, A inv_den, A smv) {
+    if (inner) {
+        const A sL = (fn - fl) * inv_den, sR = (fr - fn) * inv_den;
+        return (copysign((A)1, sR) + copysign((A)1, sL)) * fabs(sR) * fabs(sL) /
+               (fabs(sR) + fabs(sL) + smv);
+    }
+    return (fr - fl) * inv_den;
 }
 
 template <typename T> struct Arith;           // per-node arithmetic type

@@ -28,7 +28,9 @@ struct Launch2D {
                              // weights do not factor
     const double *acc_cell, *acc_v, *acc_h;
     unsigned nwall;      // number of wall faces (non-periodic sides)
-    int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline
+    int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline,
+                         // 2 the pipeline with slopes from the cell stencils
+    int stencil_ok;      // distribution arrays under 4 GiB (32-bit byte offsets)
     int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles
     double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
     double* cellpre;     // per-cell scalars of the MFMA update [ncell][24]

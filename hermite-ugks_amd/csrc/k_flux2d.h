@@ -6,6 +6,29 @@
 
 namespace hugks {
 
+// Cells next to a wall, each once: the x-wall columns (i = 0, nx-1) when x
+// is not periodic, then the y-wall rows (j = 0, ny-1) without the cells
+// already counted.
+__host__ __device__ inline int wall_cell_count(const Consts& c) {
+    const bool xw = c.bc[0] != HUGKS_BC_PERIODIC, yw = c.bc[2] != HUGKS_BC_PERIODIC;
+    const int nxc = xw ? (c.nx > 1 ? 2 : 1) : 0;
+    const int nyc = yw ? (c.ny > 1 ? 2 : 1) : 0;
+    return nxc * c.ny + nyc * (c.nx - nxc);
+}
+__device__ __forceinline__ int wall_cell(const Consts& c, int b) {
+    const bool xw = c.bc[0] != HUGKS_BC_PERIODIC;
+    const int nxc = xw ? (c.nx > 1 ? 2 : 1) : 0;
+    if (b < nxc * c.ny) {
+        const int j = b / nxc;
+        return (b % nxc ? c.nx - 1 : 0) + c.nx * j;
+    }
+    b -= nxc * c.ny;
+    const int w = c.nx - nxc;            // cells per wall row
+    const int row = b / w;
+    const int i = (nxc ? 1 : 0) + b % w;
+    return i + c.nx * (row ? c.ny - 1 : 0);
+}
+
 // ------------------------------------------------------------------ slopes
 // interp_inner (van Leer) / interp_boundary (one-sided), 2d/src/solver.f90:
 // 64-159.  Periodic directions use wrapped neighbours everywhere (the ghost
@@ -15,40 +38,25 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __restrict__ st,
                                                  const T* __restrict__ h, const T* __restrict__ b,
                                                  T* __restrict__ shx, T* __restrict__ shy,
-                                                 T* __restrict__ sbx, T* __restrict__ sby) {
+                                                 T* __restrict__ sbx, T* __restrict__ sby,
+                                                 int wall_only) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
-    // one workgroup per cell, cells in XCD-aware strip order; the neighbour
-    // cells of both directions are fixed per workgroup
-    const int cell = tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
-    const int i = cell % c.nx, j = cell / c.nx;
+    // one workgroup per cell: every cell in XCD-aware strip order, or only
+    // the cells next to a wall (their normal slopes feed the wall fluxes)
+    const int cell = wall_only ? wall_cell(c, (int)blockIdx.x)
+                               : tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
     const A smv = (A)c.smv;
     int cl[2], cr[2];
     bool inner[2];
     A inv_den[2];
 #pragma unroll
     for (int dir = 0; dir < 2; ++dir) {
-        const bool per = c.bc[2 * dir] == HUGKS_BC_PERIODIC;
-        const int n = dir == 0 ? c.nx : c.ny;
-        const int p = dir == 0 ? i : j;
+        const SlopeNb nb = slope_nb(c, cell, dir);
         const A len = (A)(dir == 0 ? c.dx : c.dy);
-        inner[dir] = true;
-        if (per) {
-            const int pl = (p - 1 + n) % n, pr = (p + 1) % n;
-            cl[dir] = dir == 0 ? pl + c.nx * j : i + c.nx * pl;
-            cr[dir] = dir == 0 ? pr + c.nx * j : i + c.nx * pr;
-        } else if (p == 0) {
-            cl[dir] = cell;
-            cr[dir] = dir == 0 ? cell + 1 : cell + c.nx;
-            inner[dir] = false;
-        } else if (p == n - 1) {
-            cl[dir] = dir == 0 ? cell - 1 : cell - c.nx;
-            cr[dir] = cell;
-            inner[dir] = false;
-        } else {
-            cl[dir] = dir == 0 ? cell - 1 : cell - c.nx;
-            cr[dir] = dir == 0 ? cell + 1 : cell + c.nx;
-        }
+        cl[dir] = nb.cl;
+        cr[dir] = nb.cr;
+        inner[dir] = nb.inner;
         inv_den[dir] = (A)1 / ((A)0.5 * len + (A)0.5 * len);
     }
     // 16-byte vectors of VW nodes: the padded row length (a multiple of 64)
@@ -77,17 +85,8 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
                 Vec so;
 #pragma unroll
                 for (int e = 0; e < VW; ++e) {
-                    const A fn = (A)fc.v[e], fl = (A)vl[dir].v[e], fr = (A)vr[dir].v[e];
-                    A s;
-                    if (inner[dir]) {
-                        // van Leer limiter (2d/src/solver.f90:146-151)
-                        const A sL = (fn - fl) * inv_den[dir], sR = (fr - fn) * inv_den[dir];
-                        s = (copysign((A)1, sR) + copysign((A)1, sL)) * fabs(sR) * fabs(sL) /
-                            (fabs(sR) + fabs(sL) + smv);
-                    } else {
-                        s = (fr - fl) * inv_den[dir];
-                    }
-                    so.v[e] = (T)s;
+                    so.v[e] = (T)vl_slope((A)vl[dir].v[e], (A)fc.v[e], (A)vr[dir].v[e], inner[dir],
+                                          inv_den[dir], smv);
                 }
                 T* out = dir == 0 ? (q == 0 ? shx : sbx) : (q == 0 ? shy : sby);
                 *reinterpret_cast<Vec*>(out + o + k) = so;

@@ -46,14 +46,18 @@ struct FaceGeo {
     int dir, face, cL, cR;
 };
 
-// Interior face g -> orientation, face slot and its two cells (the face
-// numbering of the vertical and horizontal face arrays).
+// Interior face g -> orientation, face slot and its two cells.  Each
+// orientation's faces are walked in strips of TILE_W face columns, row by
+// row inside a strip (tiled_cell on the face grid), so that with the
+// XCD-contiguous workgroup order the cells around consecutive faces stay
+// in one L2.
 __device__ __forceinline__ FaceGeo face_geo(const Consts& c, int g, int nvi) {
     FaceGeo r;
     if (g < nvi) {
         const bool per = c.bc[0] == HUGKS_BC_PERIODIC;
         const int nf = per ? c.nx + 1 : c.nx - 1;
-        const int i = (per ? 0 : 1) + g % nf, j = g / nf;
+        const int t = tiled_cell(g, nf, c.ny);
+        const int i = (per ? 0 : 1) + t % nf, j = t / nf;
         r.dir = 0;
         r.face = i + (c.nx + 1) * j;
         r.cL = (i - 1 + c.nx) % c.nx + c.nx * j;
@@ -61,7 +65,9 @@ __device__ __forceinline__ FaceGeo face_geo(const Consts& c, int g, int nvi) {
     } else {
         g -= nvi;
         const bool per = c.bc[2] == HUGKS_BC_PERIODIC;
-        const int i = g % c.nx, j = (per ? 0 : 1) + g / c.nx;
+        const int nr = per ? c.ny + 1 : c.ny - 1;
+        const int t = tiled_cell(g, c.nx, nr);
+        const int i = t % c.nx, j = (per ? 0 : 1) + t / c.nx;
         r.dir = 1;
         r.face = i + c.nx * j;
         r.cL = i + c.nx * ((j - 1 + c.ny) % c.ny);
@@ -70,12 +76,100 @@ __device__ __forceinline__ FaceGeo face_geo(const Consts& c, int g, int nvi) {
     return r;
 }
 
+// Slope stencils of the two cells of a face, as byte offsets of cell rows
+// (side 1: the left/lower cell, upwind for d = 1; side 0: the right/upper
+// cell): the cell itself, its two neighbours along the face normal and
+// the two along the face, with the inner/one-sided choice of slope_nb.
+struct Stencil {
+    uint32_t u[2], nl[2], nr[2], tl[2], tr[2];
+    bool inN[2], inT[2];
+};
+template <typename T>
+__device__ __forceinline__ Stencil face_stencil(const Consts& c, const FaceGeo& fg) {
+    Stencil S;
+    const uint32_t rowB = (uint32_t)c.ldv * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int cell = side ? fg.cL : fg.cR;
+        const SlopeNb n = slope_nb(c, cell, fg.dir), t = slope_nb(c, cell, 1 - fg.dir);
+        S.u[side] = (uint32_t)cell * rowB;
+        S.nl[side] = (uint32_t)n.cl * rowB;
+        S.nr[side] = (uint32_t)n.cr * rowB;
+        S.tl[side] = (uint32_t)t.cl * rowB;
+        S.tr[side] = (uint32_t)t.cr * rowB;
+        S.inN[side] = n.inner;
+        S.inT[side] = t.inner;
+    }
+    return S;
+}
+template <typename T>
+__device__ __forceinline__ T ld_b(const T* __restrict__ base, uint32_t boff) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+
+// Upwind values of one node: f and its normal and tangential slopes for h
+// and b, either read from the slope arrays or, with the stencil, formed
+// from the five stencil cells exactly as k_slope2d forms them.
+template <typename T, bool ST>
+struct Upwind {
+    using A = typename Arith<T>::A;
+    const T *h, *b, *sN, *sT, *bN, *bT;
+    size_t baseL, baseR;
+    Stencil S;
+    A idN, idT, smv;
+    __device__ __forceinline__ void get(int d, int k, A& hu, A& bu, A& sn, A& st, A& bn,
+                                        A& bt) const {
+        if constexpr (!ST) {
+            const size_t o = (d ? baseL : baseR) + k;
+            hu = (A)h[o];
+            bu = (A)b[o];
+            sn = (A)sN[o];
+            bn = (A)bN[o];
+            st = (A)sT[o];
+            bt = (A)bT[o];
+        } else {
+            const uint32_t kb = (uint32_t)k * (uint32_t)sizeof(T);
+            const uint32_t ou = (d ? S.u[1] : S.u[0]) + kb;
+            const uint32_t onl = (d ? S.nl[1] : S.nl[0]) + kb;
+            const uint32_t onr = (d ? S.nr[1] : S.nr[0]) + kb;
+            const uint32_t otl = (d ? S.tl[1] : S.tl[0]) + kb;
+            const uint32_t otr = (d ? S.tr[1] : S.tr[0]) + kb;
+            const bool inN = d ? S.inN[1] : S.inN[0];
+            const bool inT = d ? S.inT[1] : S.inT[0];
+            hu = (A)ld_b(h, ou);
+            bu = (A)ld_b(b, ou);
+            sn = (A)(T)vl_slope((A)ld_b(h, onl), hu, (A)ld_b(h, onr), inN, idN, smv);
+            bn = (A)(T)vl_slope((A)ld_b(b, onl), bu, (A)ld_b(b, onr), inN, idN, smv);
+            st = (A)(T)vl_slope((A)ld_b(h, otl), hu, (A)ld_b(h, otr), inT, idT, smv);
+            bt = (A)(T)vl_slope((A)ld_b(b, otl), bu, (A)ld_b(b, otr), inT, idT, smv);
+        }
+    }
+};
+template <typename T, bool ST>
+__device__ __forceinline__ Upwind<T, ST> make_upwind(const Consts& c, const FaceGeo& fg,
+                                                     const T* h, const T* b, const T* shx,
+                                                     const T* shy, const T* sbx, const T* sby) {
+    using A = typename Arith<T>::A;
+    Upwind<T, ST> U;
+    U.h = h;
+    U.b = b;
+    U.sN = fg.dir == 0 ? shx : shy;
+    U.sT = fg.dir == 0 ? shy : shx;
+    U.bN = fg.dir == 0 ? sbx : sby;
+    U.bT = fg.dir == 0 ? sby : sbx;
+    U.baseL = (size_t)fg.cL * c.ldv;
+    U.baseR = (size_t)fg.cR * c.ldv;
+    if constexpr (ST) U.S = face_stencil<T>(c, fg);
+    const A lenN = (A)(fg.dir == 0 ? c.dx : c.dy), lenT = (A)(fg.dir == 0 ? c.dy : c.dx);
+    U.idN = (A)1 / ((A)0.5 * lenN + (A)0.5 * lenN);
+    U.idT = (A)1 / ((A)0.5 * lenT + (A)0.5 * lenT);
+    U.smv = (A)c.smv;
+    return U;
+}
+
 // ------------------------------------------------------------------ K1
-template <int DIR, typename T>
-__device__ __forceinline__ void fsum_face(const Consts& c, const FaceGeo& fg, double vshift,
-                                          const T* __restrict__ h, const T* __restrict__ b,
-                                          const T* __restrict__ shx, const T* __restrict__ shy,
-                                          const T* __restrict__ sbx, const T* __restrict__ sby,
+template <int DIR, typename T, bool ST>
+__device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const Upwind<T, ST>& up,
                                           const typename Arith<T>::A* __restrict__ unode,
                                           const typename Arith<T>::A* __restrict__ vnode,
                                           const double* __restrict__ und,
@@ -83,15 +177,10 @@ __device__ __forceinline__ void fsum_face(const Consts& c, const FaceGeo& fg, do
                                           const double* __restrict__ wnd, double* __restrict__ out) {
     using A = typename Arith<T>::A;
     const int lane = threadIdx.x & (WAVE - 1);
-    const T* sN = DIR == 0 ? shx : shy;
-    const T* sT = DIR == 0 ? shy : shx;
-    const T* bN = DIR == 0 ? sbx : sby;
-    const T* bT = DIR == 0 ? sby : sbx;
     const double len = DIR == 0 ? c.dx : c.dy;
     const double hl = 0.5 * len;
     const double tsign = DIR == 0 ? 1.0 : -1.0;
     const A vs = (A)vshift;
-    const size_t baseL = (size_t)fg.cL * c.ldv, baseR = (size_t)fg.cR * c.ldv;
     double s[FSUM_N];
 #pragma unroll
     for (int q = 0; q < FSUM_N; ++q) s[q] = 0.0;
@@ -99,12 +188,13 @@ __device__ __forceinline__ void fsum_face(const Consts& c, const FaceGeo& fg, do
     for (int k = lane; k < c.nn; k += WAVE) {
         // upwind choice in the node arithmetic, as the fused kernel makes it
         const int d = upwind_delta((DIR == 0 ? unode[k] : vnode[k]) - vs);
-        const size_t o = (d ? baseL : baseR) + k;
+        A hu, bu, snA, stA, bnA, btA;
+        up.get(d, k, hu, bu, snA, stA, bnA, btA);
         const double sgn = d ? hl : -hl;
-        const double sn = (double)sN[o], bn = (double)bN[o];
-        const double hf = (double)h[o] + sgn * sn;
-        const double bf = (double)b[o] + sgn * bn;
-        const double st = tsign * (double)sT[o], bt = tsign * (double)bT[o];
+        const double sn = (double)snA, bn = (double)bnA;
+        const double hf = (double)hu + sgn * sn;
+        const double bf = (double)bu + sgn * bn;
+        const double st = tsign * (double)stA, bt = tsign * (double)btA;
         const double u = und[k], v = vnd[k], w = wnd[k];
         const double vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
         const double a = w * hf, e = w * sn, f = w * st, g = w * bf, p = w * bn, r = w * bt;
@@ -154,7 +244,7 @@ __device__ __forceinline__ void fsum_face(const Consts& c, const FaceGeo& fg, do
     if (q >= 0) out[q] = x;
 }
 
-template <typename T>
+template <typename T, bool ST>
 __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
@@ -164,18 +254,20 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
     const double* __restrict__ acc_v, const double* __restrict__ acc_h,
     double* __restrict__ fsum, int nvi, int nint) {
     if (st->stopped) return;
-    const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+    const int g = __builtin_amdgcn_readfirstlane(
+        (int)xcd_block(blockIdx.x, gridDim.x) * SPLIT_FPW + (int)(threadIdx.x / WAVE));
     if (g >= nint) return;
     const FaceGeo fg = face_geo(c, g, nvi);
     const double dt = st->dt;
+    const Upwind<T, ST> up = make_upwind<T, ST>(c, fg, h, b, shx, shy, sbx, sby);
     if (fg.dir == 0) {
         const double vshift = 0.5 * acc_v[2 * fg.face] * dt;
-        fsum_face<0, T>(c, fg, vshift, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd,
-                        fsum + (size_t)g * FSUM_LD);
+        fsum_face<0, T, ST>(c, vshift, up, unode, vnode, und, vnd, wnd,
+                            fsum + (size_t)g * FSUM_LD);
     } else {
         const double vshift = 0.5 * acc_h[2 * fg.face] * dt;
-        fsum_face<1, T>(c, fg, vshift, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd,
-                        fsum + (size_t)g * FSUM_LD);
+        fsum_face<1, T, ST>(c, vshift, up, unode, vnode, und, vnd, wnd,
+                            fsum + (size_t)g * FSUM_LD);
     }
 }
 
@@ -531,27 +623,19 @@ __global__ __launch_bounds__(256) void k_fslope_t(Consts c, const DevState* __re
 }
 
 // ------------------------------------------------------------------ K5
-template <int DIR, typename T>
-__device__ __forceinline__ void fnode_face(const Consts& c, const FaceGeo& fg,
-                                           const double* __restrict__ fc, double fn, double ft,
-                                           const T* __restrict__ h, const T* __restrict__ b,
-                                           const T* __restrict__ shx, const T* __restrict__ shy,
-                                           const T* __restrict__ sbx, const T* __restrict__ sby,
+template <int DIR, typename T, bool ST>
+__device__ __forceinline__ void fnode_face(const Consts& c, const double* __restrict__ fc,
+                                           double fn, double ft, const Upwind<T, ST>& up,
                                            const typename Arith<T>::A* __restrict__ unode,
                                            const typename Arith<T>::A* __restrict__ vnode,
                                            double dt, T* __restrict__ ofh, T* __restrict__ ofb) {
     using A = typename Arith<T>::A;
     const int lane = threadIdx.x & (WAVE - 1);
-    const T* sN = DIR == 0 ? shx : shy;
-    const T* sT = DIR == 0 ? shy : shx;
-    const T* bN = DIR == 0 ? sbx : sby;
-    const T* bT = DIR == 0 ? sby : sbx;
     const double len = DIR == 0 ? c.dx : c.dy;
     const double flen = DIR == 0 ? c.dy : c.dx;
     const A tsign = DIR == 0 ? (A)1 : (A)-1;
     const A hlen = (A)(0.5 * len);
     const A vshift = (A)(0.5 * fn * dt);
-    const size_t baseL = (size_t)fg.cL * c.ldv, baseR = (size_t)fg.cR * c.ldv;
     const double p0 = fc[FC_P], p1 = fc[FC_P + 1], p2 = fc[FC_P + 2], p3 = fc[FC_P + 3];
     const double tau = fc[FC_TAU];
     const A U = (A)p1, V = (A)p2, lam = (A)p3;
@@ -579,12 +663,12 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const FaceGeo& fg,
         const A u = unode[k], v = vnode[k];
         const A vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
         const int d = upwind_delta(vn - vshift);
-        const size_t o = (d ? baseL : baseR) + k;
+        A hu, bu, sn, stR, bn, btR;
+        up.get(d, k, hu, bu, sn, stR, bn, btR);
         const A sgn = d ? hlen : -hlen;
-        const A sn = (A)sN[o], bn = (A)bN[o];
-        const A hr = (A)h[o] + sgn * sn;
-        const A br = (A)b[o] + sgn * bn;
-        const A st = tsign * (A)sT[o], bt = tsign * (A)bT[o];
+        const A hr = hu + sgn * sn;
+        const A br = bu + sgn * bn;
+        const A st = tsign * stR, bt = tsign * btR;
         const A cu = vn - U, cv = vt - V;
         const A cp2 = cu * cu + cv * cv;
         const A c2 = vn * vn + vt * vt;
@@ -616,7 +700,7 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const FaceGeo& fg,
     }
 }
 
-template <typename T>
+template <typename T, bool ST>
 __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
@@ -625,19 +709,21 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
     const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
     T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint) {
     if (st->stopped) return;
-    const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+    const int g = __builtin_amdgcn_readfirstlane(
+        (int)xcd_block(blockIdx.x, gridDim.x) * SPLIT_FPW + (int)(threadIdx.x / WAVE));
     if (g >= nint) return;
     const FaceGeo fg = face_geo(c, g, nvi);
     const double dt = st->dt;
     const double* fc = fcoef + (size_t)g * FCOEF_LD;
+    const Upwind<T, ST> up = make_upwind<T, ST>(c, fg, h, b, shx, shy, sbx, sby);
     if (fg.dir == 0) {
         const double fn = acc_v[2 * fg.face], ft = acc_v[2 * fg.face + 1];
-        fnode_face<0, T>(c, fg, fc, fn, ft, h, b, shx, shy, sbx, sby, unode, vnode, dt,
-                         vfh + (size_t)fg.face * c.ldv, vfb + (size_t)fg.face * c.ldv);
+        fnode_face<0, T, ST>(c, fc, fn, ft, up, unode, vnode, dt,
+                             vfh + (size_t)fg.face * c.ldv, vfb + (size_t)fg.face * c.ldv);
     } else {
         const double fn = acc_h[2 * fg.face], ft = acc_h[2 * fg.face + 1];
-        fnode_face<1, T>(c, fg, fc, fn, ft, h, b, shx, shy, sbx, sby, unode, vnode, dt,
-                         hfh + (size_t)fg.face * c.ldv, hfb + (size_t)fg.face * c.ldv);
+        fnode_face<1, T, ST>(c, fc, fn, ft, up, unode, vnode, dt,
+                             hfh + (size_t)fg.face * c.ldv, hfb + (size_t)fg.face * c.ldv);
     }
 }
 

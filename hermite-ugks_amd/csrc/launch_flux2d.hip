@@ -9,11 +9,17 @@ template <typename T>
 static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     const Consts& c = L.c;
     const size_t ncell = (size_t)c.nx * c.ny;
-    {
+    if (L.flux_mode != HUGKS_FLUX_STENCIL) {
         HookScope hs(L, "slope2d");
         hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)ncell), dim3(256), 0, s, c,
                            L.st, (const T*)L.h, (const T*)L.b, (T*)L.shx, (T*)L.shy, (T*)L.sbx,
-                           (T*)L.sby);
+                           (T*)L.sby, 0);
+    } else if (L.nwall) {
+        // the wall fluxes read the slope arrays of the wall cells
+        HookScope hs(L, "slope2d");
+        hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)wall_cell_count(c)), dim3(256), 0, s,
+                           c, L.st, (const T*)L.h, (const T*)L.b, (T*)L.shx, (T*)L.shy,
+                           (T*)L.sbx, (T*)L.sby, 1);
     }
     const bool xper = c.bc[0] == HUGKS_BC_PERIODIC, yper = c.bc[2] == HUGKS_BC_PERIODIC;
     const int nvf = (xper ? c.nx + 1 : c.nx - 1) * c.ny;
@@ -24,12 +30,14 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     const A* wn = (const A*)L.wnode;
     const int fpw = FACES_PER_WG;
     const int nint = nvf + nhf;
-    if (L.flux_mode == 1 && nint > 0) {
+    if (L.flux_mode != HUGKS_FLUX_FUSED && nint > 0) {
+        const bool stn = L.flux_mode == HUGKS_FLUX_STENCIL;
         const unsigned gw = (unsigned)((nint + SPLIT_FPW - 1) / SPLIT_FPW);
         const unsigned gl = (unsigned)((nint + 255) / 256);
         {
             HookScope hs(L, "flux_sum");
-            hipLaunchKernelGGL((k_fsum<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+            auto kf = stn ? k_fsum<T, true> : k_fsum<T, false>;
+            hipLaunchKernelGGL(kf, dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
                                (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
                                (const T*)L.sbx, (const T*)L.sby, un, vn, L.unode_d, L.vnode_d,
                                L.wnode_d, L.acc_v, L.acc_h, L.fsum, nvf, nint);
@@ -56,7 +64,8 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
         }
         {
             HookScope hs(L, "flux_node");
-            hipLaunchKernelGGL((k_fnode<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+            auto kn = stn ? k_fnode<T, true> : k_fnode<T, false>;
+            hipLaunchKernelGGL(kn, dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
                                (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
                                (const T*)L.sbx, (const T*)L.sby, un, vn, L.acc_v, L.acc_h,
                                L.fcoef, (T*)L.vfh, (T*)L.vfb, (T*)L.hfh, (T*)L.hfb, nvf, nint);
@@ -68,7 +77,7 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
                            (const T*)L.shx, (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby, un,
                            vn, wn, L.acc_v, (T*)L.vfh, (T*)L.vfb, L.vflux, nvf);
     }
-    if (L.flux_mode != 1 && nhf > 0) {
+    if (L.flux_mode == HUGKS_FLUX_FUSED && nhf > 0) {
         HookScope hs(L, "flux_hface");
         hipLaunchKernelGGL((k_flux_interior<1, T>), dim3((unsigned)((nhf + fpw - 1) / fpw)),
                            dim3(WAVE * fpw), 0, s, c, L.st, L.W, (const T*)L.h, (const T*)L.b,

@@ -208,13 +208,18 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
 
 /* Kernel schedule of the 2D step (ignored in 1D).  Every schedule computes
  * the same step; they differ in speed by case shape.
- *   flux:   HUGKS_FLUX_FUSED  one kernel per face orientation
- *           HUGKS_FLUX_SPLIT  five-kernel pipeline (default)
+ *   flux:   HUGKS_FLUX_FUSED    one kernel per face orientation
+ *           HUGKS_FLUX_SPLIT    five-kernel pipeline reading the slope arrays
+ *           HUGKS_FLUX_STENCIL  five-kernel pipeline forming the van Leer
+ *                               slopes from each upwind cell's stencil
+ *                               (default; needs each distribution array
+ *                               under 4 GiB, else SPLIT is the default)
  *   update: HUGKS_UPDATE_LDS  Hermite contractions as LDS dot products
  *           HUGKS_UPDATE_MFMA Hermite contractions as matrix tiles (default
  *                             when nv <= 48 and Nm <= 14) */
 #define HUGKS_FLUX_FUSED   0
 #define HUGKS_FLUX_SPLIT   1
+#define HUGKS_FLUX_STENCIL 2
 #define HUGKS_UPDATE_LDS   0
 #define HUGKS_UPDATE_MFMA  1
 int hugks_set_schedule(hugks_handle *h, int32_t flux_kernel, int32_t update_kernel);

@@ -101,12 +101,13 @@ def test_advance_until_output_time(have_gpu):
     assert np.all(field_err(w, o2.get()[0]) <= TOL_FP64)
 
 
-@pytest.mark.parametrize("mode", ["fused", "split", "split-nodes"])
+@pytest.mark.parametrize("mode", ["fused", "split", "split-nodes", "stencil"])
 def test_flux_kernels_agree(have_gpu, monkeypatch, mode):
     """The interior-face flux implementations (one fused kernel; the
-    five-kernel pipeline with the Maxwellian sums from 1D moments of the
-    tensor-product quadrature, or node by node) against the oracle, FP64
-    and mixed."""
+    five-kernel pipeline reading the slope arrays, with the Maxwellian sums
+    from 1D moments of the tensor-product quadrature or node by node; the
+    pipeline forming the slopes from the cell stencils) against the oracle,
+    FP64 and mixed."""
     monkeypatch.setenv("HUGKS_FLUX_KERNEL", mode)
     for prec, tol in ((capi.FP64, TOL_FP64), (capi.MIXED_FP32, TOL_MIXED)):
         case = cases.build_2d(rt(nx=10, ny=12, nv=16, nm=8), prec, rt_variant="cartesian", seed=5)
