@@ -389,11 +389,11 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         const size_t nint = (size_t)(xper ? d->nx + 1 : d->nx - 1) * d->ny +
                             (size_t)d->nx * (yper ? d->ny + 1 : d->ny - 1);
         L.stencil_ok = fsz < ((size_t)1 << 32);
-        L.flux_mode = L.stencil_ok ? HUGKS_FLUX_STENCIL : HUGKS_FLUX_SPLIT;
+        L.flux_mode = HUGKS_FLUX_SPLIT;
         bool maxw_nodes = false;
         if (const char* fm = getenv("HUGKS_FLUX_KERNEL")) {
             if (strcmp(fm, "fused") == 0) L.flux_mode = HUGKS_FLUX_FUSED;
-            if (strcmp(fm, "split") == 0) L.flux_mode = HUGKS_FLUX_SPLIT;
+            if (strcmp(fm, "stencil") == 0 && L.stencil_ok) L.flux_mode = HUGKS_FLUX_STENCIL;
             if (strcmp(fm, "split-nodes") == 0) {
                 L.flux_mode = HUGKS_FLUX_SPLIT;
                 maxw_nodes = true;

@@ -138,10 +138,10 @@ struct Upwind {
             const bool inT = d ? S.inT[1] : S.inT[0];
             hu = (A)ld_b(h, ou);
             bu = (A)ld_b(b, ou);
-            sn = (A)(T)vl_slope((A)ld_b(h, onl), hu, (A)ld_b(h, onr), inN, idN, smv);
-            bn = (A)(T)vl_slope((A)ld_b(b, onl), bu, (A)ld_b(b, onr), inN, idN, smv);
-            st = (A)(T)vl_slope((A)ld_b(h, otl), hu, (A)ld_b(h, otr), inT, idT, smv);
-            bt = (A)(T)vl_slope((A)ld_b(b, otl), bu, (A)ld_b(b, otr), inT, idT, smv);
+            sn = (A)(T)vl_slope<A, true>((A)ld_b(h, onl), hu, (A)ld_b(h, onr), inN, idN, smv);
+            bn = (A)(T)vl_slope<A, true>((A)ld_b(b, onl), bu, (A)ld_b(b, onr), inN, idN, smv);
+            st = (A)(T)vl_slope<A, true>((A)ld_b(h, otl), hu, (A)ld_b(h, otr), inT, idT, smv);
+            bt = (A)(T)vl_slope<A, true>((A)ld_b(b, otl), bu, (A)ld_b(b, otr), inT, idT, smv);
         }
     }
 };
@@ -184,7 +184,8 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
     double s[FSUM_N];
 #pragma unroll
     for (int q = 0; q < FSUM_N; ++q) s[q] = 0.0;
-#pragma unroll 2
+    constexpr int UNR = ST ? 1 : 2;   // the stencil holds ten loads per node
+#pragma unroll UNR
     for (int k = lane; k < c.nn; k += WAVE) {
         // upwind choice in the node arithmetic, as the fused kernel makes it
         const int d = upwind_delta((DIR == 0 ? unode[k] : vnode[k]) - vs);

@@ -210,10 +210,10 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
  * the same step; they differ in speed by case shape.
  *   flux:   HUGKS_FLUX_FUSED    one kernel per face orientation
  *           HUGKS_FLUX_SPLIT    five-kernel pipeline reading the slope arrays
+ *                               (default)
  *           HUGKS_FLUX_STENCIL  five-kernel pipeline forming the van Leer
  *                               slopes from each upwind cell's stencil
- *                               (default; needs each distribution array
- *                               under 4 GiB, else SPLIT is the default)
+ *                               (needs each distribution array under 4 GiB)
  *   update: HUGKS_UPDATE_LDS  Hermite contractions as LDS dot products
  *           HUGKS_UPDATE_MFMA Hermite contractions as matrix tiles (default
  *                             when nv <= 48 and Nm <= 14) */

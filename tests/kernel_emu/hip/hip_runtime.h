@@ -32,6 +32,7 @@
 #define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(::emu::dyn_smem);
 // wave-uniform values only: the emulation has no scalar register file
 #define __builtin_amdgcn_readfirstlane(x) (x)
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 
 struct dim3 {
     unsigned x, y, z;
