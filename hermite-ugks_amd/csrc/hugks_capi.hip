@@ -316,6 +316,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     ALLOC(L.st, sizeof(DevState));
     ALLOC(L.W, ncell * (dim + 2) * sizeof(double));
     ALLOC(L.part, ncell * np * sizeof(double));
+    ALLOC(L.red, RED_SCRATCH * sizeof(double));
     ALLOC(L.h, fsz);
     ALLOC(L.b, fsz);
     ALLOC(L.shx, fsz);

@@ -34,6 +34,7 @@ struct Launch2D {
     int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles
     double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
     double* cellpre;     // per-cell scalars of the MFMA update [ncell][24]
+    double* red;         // partials of the cell reductions [RED_SCRATCH]
     int update_nt;       // threads per update workgroup
     size_t update_lds;   // dynamic LDS bytes of the update kernel
     LaunchHook hook;     // may be null
@@ -45,6 +46,8 @@ struct Launch2D {
 constexpr int FACE_SUM_LD = 40;
 constexpr int FACE_COEF_LD = 40;
 constexpr int FACE_MAXW_LD = 16;
+// Doubles of the cell-reduction partials (k_misc.h: PRED_NB x (1 + PRED_LD)).
+constexpr int RED_SCRATCH = 1024 * 10;
 
 // Brackets one launch with the hook.
 struct HookScope {

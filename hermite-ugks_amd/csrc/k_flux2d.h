@@ -30,6 +30,7 @@ __device__ __forceinline__ int wall_cell(const Consts& c, int b) {
 }
 
 // ------------------------------------------------------------------ slopes
+constexpr int SLOPE_CPW = 4;   // cells (wavefronts) per workgroup of k_slope2d
 // interp_inner (van Leer) / interp_boundary (one-sided), 2d/src/solver.f90:
 // 64-159.  Periodic directions use wrapped neighbours everywhere (the ghost
 // copy of 2d/src/boundary.f90:16-53); wall directions use one-sided slopes
@@ -42,10 +43,15 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
                                                  int wall_only) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
-    // one workgroup per cell: every cell in XCD-aware strip order, or only
-    // the cells next to a wall (their normal slopes feed the wall fluxes)
-    const int cell = wall_only ? wall_cell(c, (int)blockIdx.x)
-                               : tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
+    // one wavefront per cell, four cells per workgroup: every cell in
+    // XCD-aware strip order, or only the cells next to a wall (their normal
+    // slopes feed the wall fluxes)
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int L = __builtin_amdgcn_readfirstlane(
+        (int)(wall_only ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * SLOPE_CPW +
+        (int)(threadIdx.x / WAVE));
+    if (L >= (wall_only ? wall_cell_count(c) : c.nx * c.ny)) return;
+    const int cell = wall_only ? wall_cell(c, L) : tiled_cell(L, c.nx, c.ny);
     const A smv = (A)c.smv;
     int cl[2], cr[2];
     bool inner[2];
@@ -68,7 +74,7 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
     const size_t o = (size_t)cell * c.ldv;
     const size_t oL[2] = {(size_t)cl[0] * c.ldv, (size_t)cl[1] * c.ldv};
     const size_t oR[2] = {(size_t)cr[0] * c.ldv, (size_t)cr[1] * c.ldv};
-    for (int g = threadIdx.x; g < ng; g += 256) {
+    for (int g = lane; g < ng; g += WAVE) {
         const size_t k = (size_t)g * VW;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {

@@ -8,7 +8,7 @@ namespace hugks {
 
 constexpr int RED_NT = 1024;
 
-template <int NV>
+template <int NV, int NT = RED_NT>
 __device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
 #pragma unroll
@@ -25,7 +25,7 @@ __device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         double s = lds[i];
-        for (int w = 1; w < RED_NT / WAVE; ++w) s = fmax(s, lds[w * NV + i]);
+        for (int w = 1; w < NT / WAVE; ++w) s = fmax(s, lds[w * NV + i]);
         v[i] = s;
     }
     __syncthreads();
@@ -42,11 +42,53 @@ __global__ void k_begin(DevState* st, double t_stop) {
     }
 }
 
-// timestep, 2d/src/solver.f90:13-61 and 1d/src/solver.f90:16-47; one
-// workgroup.  The 1D loop tests simtime >= maxtime before the step
-// (1d/src/main.f90:84), so that test is made here.
+// Signal rate of one cell for the time step (timestep, 2d/src/solver.f90:
+// 13-61 and 1d/src/solver.f90:16-47).
+__device__ __forceinline__ double cell_rate(const Consts& c, int dim, const double* __restrict__ W,
+                                            int cell) {
+    if (dim == 2) {
+        double p[4];
+        prim_from_w(c, W + 4 * (size_t)cell, p);
+        const double sos = sqrt(0.5 * c.gamma / p[3]);
+        const double pu = fmax(c.umax_dt, fabs(p[1])) + sos;
+        const double pv = fmax(c.vmax_dt, fabs(p[2])) + sos;
+        return c.dt_rule == HUGKS_DT_2D_WALL_NORMAL ? pv / c.dy
+                                                    : (c.dy * pu + c.dx * pv) / (c.dx * c.dy);
+    }
+    const double* w = W + 3 * (size_t)cell;
+    const double U = w[1] / w[0];
+    const double lam = 0.5 * w[0] / (c.gamma - 1.0) / (w[2] - 0.5 * w[1] * w[1] / w[0]);
+    const double sos = sqrt(0.5 * c.gamma / lam);
+    return (fmax(c.umax_dt, fabs(U)) + sos) / c.dx;
+}
+
+// Reductions over the cells run in two passes: PRED_NT-thread workgroups
+// reduce strided cell ranges to one partial each (at most PRED_NB), then
+// one workgroup reduces the partials in index order.
+constexpr int PRED_NT = 256;
+constexpr int PRED_NB = 1024;
+constexpr int PRED_LD = 9;     // per-partial slots of the finalize pass
+static_assert(PRED_NB * (1 + PRED_LD) <= RED_SCRATCH, "reduction scratch");
+
+// First pass of the time step: the largest signal rate per workgroup.
+__global__ __launch_bounds__(PRED_NT) void k_rate_part(Consts c, int dim,
+                                                       const DevState* __restrict__ st,
+                                                       const double* __restrict__ W,
+                                                       double* __restrict__ red) {
+    __shared__ double lds[PRED_NT / WAVE];
+    if (st->stopped) return;
+    const int ncell = c.nx * c.ny;
+    double t[1] = {0.0};
+    for (int cell = blockIdx.x * PRED_NT + threadIdx.x; cell < ncell; cell += gridDim.x * PRED_NT)
+        t[0] = fmax(t[0], cell_rate(c, dim, W, cell));
+    block_max<1, PRED_NT>(t, lds);
+    if (threadIdx.x == 0) red[blockIdx.x] = t[0];
+}
+
+// timestep: dt from the largest rate.  The 1D loop tests simtime >=
+// maxtime before the step (1d/src/main.f90:84), so that test is made here.
 __global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState* st,
-                                                     const double* __restrict__ W) {
+                                                     const double* __restrict__ red, int nred) {
     __shared__ double lds[RED_NT / WAVE];
     if (st->stopped) return;
     if (dim == 1 && st->simtime >= c.maxtime) {
@@ -56,27 +98,8 @@ __global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState
         }
         return;
     }
-    const int ncell = c.nx * c.ny;
-    const double area = c.dx * c.dy;
     double t[1] = {0.0};
-    for (int cell = threadIdx.x; cell < ncell; cell += RED_NT) {
-        if (dim == 2) {
-            double p[4];
-            prim_from_w(c, W + 4 * (size_t)cell, p);
-            const double sos = sqrt(0.5 * c.gamma / p[3]);
-            const double pu = fmax(c.umax_dt, fabs(p[1])) + sos;
-            const double pv = fmax(c.vmax_dt, fabs(p[2])) + sos;
-            const double rate = c.dt_rule == HUGKS_DT_2D_WALL_NORMAL ? pv / c.dy
-                                                                     : (c.dy * pu + c.dx * pv) / area;
-            t[0] = fmax(t[0], rate);
-        } else {
-            const double* w = W + 3 * (size_t)cell;
-            const double U = w[1] / w[0];
-            const double lam = 0.5 * w[0] / (c.gamma - 1.0) / (w[2] - 0.5 * w[1] * w[1] / w[0]);
-            const double sos = sqrt(0.5 * c.gamma / lam);
-            t[0] = fmax(t[0], (fmax(c.umax_dt, fabs(U)) + sos) / c.dx);
-        }
-    }
+    for (int q = threadIdx.x; q < nred; q += RED_NT) t[0] = fmax(t[0], red[q]);
     block_max<1>(t, lds);
     if (threadIdx.x == 0) {
         const double temp = fmax(t[0], c.force_rate);
@@ -87,19 +110,20 @@ __global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState
     }
 }
 
-// Residual and loop state after update(): 2d/src/solver.f90:393 with
-// 2d/src/main.f90:82,102-104; 1d/src/solver.f90:262 with 1d/src/main.f90:
-// 106-107.  Partials per cell: 2D (dw^2 x4, |rho|); 1D (dw^2 x3, |w| x3,
-// steady residual x3).  One workgroup, fixed summation order.
-__global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState* st,
-                                                     const double* __restrict__ part) {
-    __shared__ double lds[9 * (RED_NT / WAVE)];
+// First pass of the finalisation: per workgroup, the sums of the per-cell
+// residual partials (fixed order) and, in 1D, the maxima of the steady
+// residuals.
+__global__ __launch_bounds__(PRED_NT) void k_part_sum(Consts c, int dim,
+                                                      const DevState* __restrict__ st,
+                                                      const double* __restrict__ part,
+                                                      double* __restrict__ red) {
+    __shared__ double lds[6 * (PRED_NT / WAVE)];
     if (st->stopped) return;
     const int ncell = c.nx * c.ny;
     const int np = dim == 2 ? 5 : 9;
     double s[6] = {0, 0, 0, 0, 0, 0};
     double mx[3] = {0, 0, 0};
-    for (int cell = threadIdx.x; cell < ncell; cell += RED_NT) {
+    for (int cell = blockIdx.x * PRED_NT + threadIdx.x; cell < ncell; cell += gridDim.x * PRED_NT) {
         const double* p = part + (size_t)np * cell;
         if (dim == 2) {
             for (int q = 0; q < 5; ++q) s[q] += p[q];
@@ -107,6 +131,32 @@ __global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState
             for (int q = 0; q < 6; ++q) s[q] += p[q];
             for (int q = 0; q < 3; ++q) mx[q] = fmax(mx[q], p[6 + q]);
         }
+    }
+    block_sum<6, PRED_NT>(s, lds);
+    if (dim == 1) block_max<3, PRED_NT>(mx, lds);
+    if (threadIdx.x == 0) {
+        double* r = red + (size_t)PRED_LD * blockIdx.x;
+        for (int q = 0; q < 6; ++q) r[q] = s[q];
+        for (int q = 0; q < 3; ++q) r[6 + q] = mx[q];
+    }
+}
+
+// Residual and loop state after update(): 2d/src/solver.f90:393 with
+// 2d/src/main.f90:82,102-104; 1d/src/solver.f90:262 with 1d/src/main.f90:
+// 106-107.  Partials per cell: 2D (dw^2 x4, |rho|); 1D (dw^2 x3, |w| x3,
+// steady residual x3), reduced by k_part_sum; one workgroup sums its
+// partials in index order.
+__global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState* st,
+                                                     const double* __restrict__ red, int nred) {
+    __shared__ double lds[9 * (RED_NT / WAVE)];
+    if (st->stopped) return;
+    const int ncell = c.nx * c.ny;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    double mx[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nred; b += RED_NT) {
+        const double* r = red + (size_t)PRED_LD * b;
+        for (int q = 0; q < 6; ++q) s[q] += r[q];
+        for (int q = 0; q < 3; ++q) mx[q] = fmax(mx[q], r[6 + q]);
     }
     block_sum<6, RED_NT>(s, lds);
     if (dim == 1) block_max<3>(mx, lds);

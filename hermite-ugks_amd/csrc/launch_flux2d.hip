@@ -11,13 +11,16 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     const size_t ncell = (size_t)c.nx * c.ny;
     if (L.flux_mode != HUGKS_FLUX_STENCIL) {
         HookScope hs(L, "slope2d");
-        hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)ncell), dim3(256), 0, s, c,
+        hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)((ncell + SLOPE_CPW - 1) / SLOPE_CPW)),
+                           dim3(WAVE * SLOPE_CPW), 0, s, c,
                            L.st, (const T*)L.h, (const T*)L.b, (T*)L.shx, (T*)L.shy, (T*)L.sbx,
                            (T*)L.sby, 0);
     } else if (L.nwall) {
         // the wall fluxes read the slope arrays of the wall cells
         HookScope hs(L, "slope2d");
-        hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)wall_cell_count(c)), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_slope2d<T>),
+                           dim3((unsigned)((wall_cell_count(c) + SLOPE_CPW - 1) / SLOPE_CPW)),
+                           dim3(WAVE * SLOPE_CPW), 0, s,
                            c, L.st, (const T*)L.h, (const T*)L.b, (T*)L.shx, (T*)L.shy,
                            (T*)L.sbx, (T*)L.sby, 1);
     }

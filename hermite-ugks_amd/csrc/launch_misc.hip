@@ -3,9 +3,17 @@
 
 namespace hugks {
 
+static int pred_blocks(const Consts& c) {
+    const int n = (c.nx * c.ny + PRED_NT - 1) / PRED_NT;
+    return n < PRED_NB ? n : PRED_NB;
+}
+
 hipError_t launch_finalize(const Launch2D& L, int dim, hipStream_t s) {
     HookScope hs(L, "finalize");
-    hipLaunchKernelGGL((k_finalize), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, L.part);
+    const int nb = pred_blocks(L.c);
+    double* red = L.red + PRED_NB;
+    hipLaunchKernelGGL((k_part_sum), dim3(nb), dim3(PRED_NT), 0, s, L.c, dim, L.st, L.part, red);
+    hipLaunchKernelGGL((k_finalize), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, red, nb);
     return hipGetLastError();
 }
 
@@ -16,7 +24,9 @@ hipError_t launch_begin(DevState* st, double t_stop, hipStream_t s) {
 
 hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s) {
     HookScope hs(L, "timestep");
-    hipLaunchKernelGGL((k_timestep), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, L.W);
+    const int nb = pred_blocks(L.c);
+    hipLaunchKernelGGL((k_rate_part), dim3(nb), dim3(PRED_NT), 0, s, L.c, dim, L.st, L.W, L.red);
+    hipLaunchKernelGGL((k_timestep), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, L.red, nb);
     return hipGetLastError();
 }
 
