@@ -17,12 +17,15 @@
 //   K3 k_fmaxw     one wavefront per face, no distribution reads: the
 //                  Maxwellian sums of the force compatibility
 //                  (flux.f90:691-705) and of the Shakhov terms;
-//                  k_fmaxw_sep computes the same sums from 1D moments
+//                  fmaxw_sep_lane computes the same sums from 1D moments
 //                  when the quadrature is a tensor product
 //   K4 k_fslope_t  one lane per face: time slope aT (flux.f90:126-135) and
 //                  the conserved flux (flux.f90:152-246)
 //   K5 k_fnode     one wavefront per face, second read of the inputs: the
 //                  distribution fluxes per node (flux.f90:207-236)
+//
+// For a tensor-product quadrature K2, K3 and K4 run as one lane-per-face
+// kernel, k_fface.
 //
 // The scalar moment algebra runs once per face on one lane instead of on
 // every lane of the face's wavefront, and the distribution functions are
@@ -286,19 +289,14 @@ enum : int {
     FC_AT = 29,    // 4 (K4)
 };
 
-__global__ __launch_bounds__(256) void k_fstate(Consts c, const DevState* __restrict__ st,
-                                                const double* __restrict__ W,
-                                                const double* __restrict__ acc_v,
-                                                const double* __restrict__ acc_h,
-                                                const double* __restrict__ fsum,
-                                                double* __restrict__ fcoef, int nvi, int nint) {
-    if (st->stopped) return;
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= nint) return;
-    const FaceGeo fg = face_geo(c, g, nvi);
+__device__ __forceinline__ void fstate_lane(const Consts& c, double dt,
+                                            const double* __restrict__ W,
+                                            const double* __restrict__ acc_v,
+                                            const double* __restrict__ acc_h,
+                                            const double* __restrict__ fsum, double* fcoef,
+                                            const FaceGeo& fg, int g) {
     const double* s1 = fsum + (size_t)g * FSUM_LD;
     double* fc = fcoef + (size_t)g * FCOEF_LD;
-    const double dt = st->dt;
     const double* acc = fg.dir == 0 ? acc_v : acc_h;
     const double fn = acc[2 * fg.face], ft = acc[2 * fg.face + 1];
     const double len = fg.dir == 0 ? c.dx : c.dy;
@@ -372,6 +370,18 @@ __global__ __launch_bounds__(256) void k_fstate(Consts c, const DevState* __rest
     for (int q = 1; q <= 6; ++q) fc[FC_MT + q - 1] = Mt[q];
 }
 
+__global__ __launch_bounds__(256) void k_fstate(Consts c, const DevState* __restrict__ st,
+                                                const double* __restrict__ W,
+                                                const double* __restrict__ acc_v,
+                                                const double* __restrict__ acc_h,
+                                                const double* __restrict__ fsum,
+                                                double* __restrict__ fcoef, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nint) return;
+    fstate_lane(c, st->dt, W, acc_v, acc_h, fsum, fcoef, face_geo(c, g, nvi), g);
+}
+
 // ------------------------------------------------------------------ K3
 template <int DIR, typename T>
 __device__ __forceinline__ void fmaxw_face(const Consts& c, const double* __restrict__ fc,
@@ -443,7 +453,7 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fmaxw(
         fmaxw_face<1, T>(c, fc, fn, ft, und, vnd, wnd, out);
 }
 
-// K3 for a tensor-product quadrature, one lane per face.  With nodes
+// K3 for a tensor-product quadrature, on the face's lane.  With nodes
 // (u_i, v_j), weights wx_i wy_j and the Maxwellian
 // exp(-lam (cu^2 + cv^2)) = exp(-lam cu^2) exp(-lam cv^2), every sum of
 // fmaxw_face is a short sum of products <f>_n <g>_t of 1D moments along
@@ -455,20 +465,14 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fmaxw(
 //        <vt^2 r> <cv vt^2 r>,  r = 2 lam cv^2
 // (2 lam cp^2 + K - 5 = q5 + r, c^2 = vn^2 + vt^2).  2 nv exponentials and
 // about 30 nv multiply-adds per face instead of nv^2 of each.
-__global__ __launch_bounds__(256) void k_fmaxw_sep(Consts c, const DevState* __restrict__ st,
-                                                   const double* __restrict__ u1,
-                                                   const double* __restrict__ v1,
-                                                   const double* __restrict__ wx,
-                                                   const double* __restrict__ wy,
-                                                   const double* __restrict__ acc_v,
-                                                   const double* __restrict__ acc_h,
-                                                   const double* __restrict__ fcoef,
-                                                   double* __restrict__ fmaxw, int nvi,
-                                                   int nint) {
-    if (st->stopped) return;
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= nint) return;
-    const FaceGeo fg = face_geo(c, g, nvi);
+__device__ __forceinline__ void fmaxw_sep_lane(const Consts& c, const double* __restrict__ u1,
+                                               const double* __restrict__ v1,
+                                               const double* __restrict__ wx,
+                                               const double* __restrict__ wy,
+                                               const double* __restrict__ acc_v,
+                                               const double* __restrict__ acc_h,
+                                               const double* fcoef, const FaceGeo& fg, int g,
+                                               double* out) {
     const double* acc = fg.dir == 0 ? acc_v : acc_h;
     const double fn = acc[2 * fg.face], ft = acc[2 * fg.face + 1];
     const double* fc = fcoef + (size_t)g * FCOEF_LD;
@@ -526,7 +530,6 @@ __global__ __launch_bounds__(256) void k_fmaxw_sep(Consts c, const DevState* __r
         t[11] += ecv * vt2 * r;
     }
     const double R = rho * (lam / c.pi), lR = lam * R, kfac = c.K / (2.0 * lam);
-    double* out = fmaxw + (size_t)g * FMAXW_LD;
     out[0] = lR * (fn * n[3] * t[0] + ft * n[2] * t[1]);
     out[1] = lR * (fn * n[1] * t[2] + ft * n[0] * t[3]);
     out[2] = lR * (fn * (n[4] * t[0] + n[1] * t[4] + kfac * n[1] * t[0]) +
@@ -544,19 +547,13 @@ __global__ __launch_bounds__(256) void k_fmaxw_sep(Consts c, const DevState* __r
 }
 
 // ------------------------------------------------------------------ K4
-__global__ __launch_bounds__(256) void k_fslope_t(Consts c, const DevState* __restrict__ st,
-                                                  const double* __restrict__ acc_v,
-                                                  const double* __restrict__ acc_h,
-                                                  const double* __restrict__ fmaxw,
-                                                  double* __restrict__ fcoef,
-                                                  double* __restrict__ vflux,
-                                                  double* __restrict__ hflux, int nvi, int nint) {
-    if (st->stopped) return;
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= nint) return;
-    const FaceGeo fg = face_geo(c, g, nvi);
-    double* fc = fcoef + (size_t)g * FCOEF_LD;
-    const double* s2 = fmaxw + (size_t)g * FMAXW_LD;   // s2[q] = K3 sum q
+__device__ __forceinline__ void fslope_t_lane(const Consts& c, const double* __restrict__ acc_v,
+                                              const double* __restrict__ acc_h,
+                                              const double* s2, double* fcoef,
+                                              double* __restrict__ vflux,
+                                              double* __restrict__ hflux, const FaceGeo& fg,
+                                              int g) {
+    double* fc = fcoef + (size_t)g * FCOEF_LD;          // s2[q] = K3 sum q
     const double* acc = fg.dir == 0 ? acc_v : acc_h;
     const double fn = acc[2 * fg.face], ft = acc[2 * fg.face + 1];
     double p[4], aL[4], aR[4], ab[4], fl[4], Mt[7];
@@ -621,6 +618,43 @@ __global__ __launch_bounds__(256) void k_fslope_t(Consts c, const DevState* __re
     out[1] = flen * g1;
     out[2] = flen * g2;
     out[3] = flen * out4[3];
+}
+
+__global__ __launch_bounds__(256) void k_fslope_t(Consts c, const DevState* __restrict__ st,
+                                                  const double* __restrict__ acc_v,
+                                                  const double* __restrict__ acc_h,
+                                                  const double* __restrict__ fmaxw,
+                                                  double* __restrict__ fcoef,
+                                                  double* __restrict__ vflux,
+                                                  double* __restrict__ hflux, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nint) return;
+    fslope_t_lane(c, acc_v, acc_h, fmaxw + (size_t)g * FMAXW_LD, fcoef, vflux, hflux,
+                  face_geo(c, g, nvi), g);
+}
+
+// K2, K3 (tensor-product quadrature) and K4 of one face on one lane, the
+// Maxwellian sums in registers: one launch instead of three.
+__global__ __launch_bounds__(256) void k_fface(Consts c, const DevState* __restrict__ st,
+                                               const double* __restrict__ W,
+                                               const double* __restrict__ u1,
+                                               const double* __restrict__ v1,
+                                               const double* __restrict__ wx,
+                                               const double* __restrict__ wy,
+                                               const double* __restrict__ acc_v,
+                                               const double* __restrict__ acc_h,
+                                               const double* __restrict__ fsum, double* fcoef,
+                                               double* __restrict__ vflux,
+                                               double* __restrict__ hflux, int nvi, int nint) {
+    if (st->stopped) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    fstate_lane(c, st->dt, W, acc_v, acc_h, fsum, fcoef, fg, g);
+    double s2[FMAXW_N];
+    fmaxw_sep_lane(c, u1, v1, wx, wy, acc_v, acc_h, fcoef, fg, g, s2);
+    fslope_t_lane(c, acc_v, acc_h, s2, fcoef, vflux, hflux, fg, g);
 }
 
 // ------------------------------------------------------------------ K5

@@ -85,9 +85,12 @@ __global__ __launch_bounds__(PRED_NT) void k_rate_part(Consts c, int dim,
     if (threadIdx.x == 0) red[blockIdx.x] = t[0];
 }
 
-// timestep: dt from the largest rate.  The 1D loop tests simtime >=
-// maxtime before the step (1d/src/main.f90:84), so that test is made here.
+// timestep: dt from the largest rate, over the nred partials of
+// k_rate_part or, with nred = 0, straight over the cells (small grids: one
+// launch instead of two).  The 1D loop tests simtime >= maxtime before the
+// step (1d/src/main.f90:84), so that test is made here.
 __global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState* st,
+                                                     const double* __restrict__ W,
                                                      const double* __restrict__ red, int nred) {
     __shared__ double lds[RED_NT / WAVE];
     if (st->stopped) return;
@@ -99,7 +102,12 @@ __global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState
         return;
     }
     double t[1] = {0.0};
-    for (int q = threadIdx.x; q < nred; q += RED_NT) t[0] = fmax(t[0], red[q]);
+    if (nred > 0) {
+        for (int q = threadIdx.x; q < nred; q += RED_NT) t[0] = fmax(t[0], red[q]);
+    } else {
+        for (int cell = threadIdx.x; cell < c.nx * c.ny; cell += RED_NT)
+            t[0] = fmax(t[0], cell_rate(c, dim, W, cell));
+    }
     block_max<1>(t, lds);
     if (threadIdx.x == 0) {
         const double temp = fmax(t[0], c.force_rate);
@@ -147,16 +155,31 @@ __global__ __launch_bounds__(PRED_NT) void k_part_sum(Consts c, int dim,
 // steady residual x3), reduced by k_part_sum; one workgroup sums its
 // partials in index order.
 __global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState* st,
+                                                     const double* __restrict__ part,
                                                      const double* __restrict__ red, int nred) {
     __shared__ double lds[9 * (RED_NT / WAVE)];
     if (st->stopped) return;
     const int ncell = c.nx * c.ny;
     double s[6] = {0, 0, 0, 0, 0, 0};
     double mx[3] = {0, 0, 0};
-    for (int b = threadIdx.x; b < nred; b += RED_NT) {
-        const double* r = red + (size_t)PRED_LD * b;
-        for (int q = 0; q < 6; ++q) s[q] += r[q];
-        for (int q = 0; q < 3; ++q) mx[q] = fmax(mx[q], r[6 + q]);
+    if (nred > 0) {
+        for (int b = threadIdx.x; b < nred; b += RED_NT) {
+            const double* r = red + (size_t)PRED_LD * b;
+            for (int q = 0; q < 6; ++q) s[q] += r[q];
+            for (int q = 0; q < 3; ++q) mx[q] = fmax(mx[q], r[6 + q]);
+        }
+    } else {
+        // small grids: straight over the per-cell partials
+        const int np = dim == 2 ? 5 : 9;
+        for (int cell = threadIdx.x; cell < ncell; cell += RED_NT) {
+            const double* p = part + (size_t)np * cell;
+            if (dim == 2) {
+                for (int q = 0; q < 5; ++q) s[q] += p[q];
+            } else {
+                for (int q = 0; q < 6; ++q) s[q] += p[q];
+                for (int q = 0; q < 3; ++q) mx[q] = fmax(mx[q], p[6 + q]);
+            }
+        }
     }
     block_sum<6, RED_NT>(s, lds);
     if (dim == 1) block_max<3>(mx, lds);

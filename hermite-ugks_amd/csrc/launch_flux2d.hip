@@ -45,25 +45,30 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
                                (const T*)L.sbx, (const T*)L.sby, un, vn, L.unode_d, L.vnode_d,
                                L.wnode_d, L.acc_v, L.acc_h, L.fsum, nvf, nint);
         }
-        {
-            HookScope hs(L, "flux_state");
-            hipLaunchKernelGGL((k_fstate), dim3(gl), dim3(256), 0, s, c, L.st, L.W, L.acc_v,
-                               L.acc_h, L.fsum, L.fcoef, nvf, nint);
-        }
         if (L.wx) {
-            HookScope hs(L, "flux_maxw");
-            hipLaunchKernelGGL((k_fmaxw_sep), dim3(gl), dim3(256), 0, s, c, L.st, L.u1, L.v1,
-                               L.wx, L.wy, L.acc_v, L.acc_h, L.fcoef, L.fmaxw, nvf, nint);
+            // tensor-product quadrature: face state, Maxwellian sums and
+            // conserved flux in one lane-per-face kernel
+            HookScope hs(L, "flux_face");
+            hipLaunchKernelGGL((k_fface), dim3(gl), dim3(256), 0, s, c, L.st, L.W, L.u1, L.v1,
+                               L.wx, L.wy, L.acc_v, L.acc_h, L.fsum, L.fcoef, L.vflux, L.hflux,
+                               nvf, nint);
         } else {
-            HookScope hs(L, "flux_maxw");
-            hipLaunchKernelGGL((k_fmaxw<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
-                               L.unode_d, L.vnode_d, L.wnode_d, L.acc_v, L.acc_h, L.fcoef,
-                               L.fmaxw, nvf, nint);
-        }
-        {
-            HookScope hs(L, "flux_slope_t");
-            hipLaunchKernelGGL((k_fslope_t), dim3(gl), dim3(256), 0, s, c, L.st, L.acc_v,
-                               L.acc_h, L.fmaxw, L.fcoef, L.vflux, L.hflux, nvf, nint);
+            {
+                HookScope hs(L, "flux_state");
+                hipLaunchKernelGGL((k_fstate), dim3(gl), dim3(256), 0, s, c, L.st, L.W, L.acc_v,
+                                   L.acc_h, L.fsum, L.fcoef, nvf, nint);
+            }
+            {
+                HookScope hs(L, "flux_maxw");
+                hipLaunchKernelGGL((k_fmaxw<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+                                   L.unode_d, L.vnode_d, L.wnode_d, L.acc_v, L.acc_h, L.fcoef,
+                                   L.fmaxw, nvf, nint);
+            }
+            {
+                HookScope hs(L, "flux_slope_t");
+                hipLaunchKernelGGL((k_fslope_t), dim3(gl), dim3(256), 0, s, c, L.st, L.acc_v,
+                                   L.acc_h, L.fmaxw, L.fcoef, L.vflux, L.hflux, nvf, nint);
+            }
         }
         {
             HookScope hs(L, "flux_node");

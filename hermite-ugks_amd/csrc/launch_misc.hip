@@ -3,7 +3,12 @@
 
 namespace hugks {
 
+// Workgroups of the first reduction pass; 0 (one single-workgroup pass)
+// for grids a single workgroup reduces quickly, where a second launch
+// would cost more than it saves.
+constexpr int PRED_MIN_CELLS = 16384;
 static int pred_blocks(const Consts& c) {
+    if (c.nx * c.ny < PRED_MIN_CELLS) return 0;
     const int n = (c.nx * c.ny + PRED_NT - 1) / PRED_NT;
     return n < PRED_NB ? n : PRED_NB;
 }
@@ -12,8 +17,9 @@ hipError_t launch_finalize(const Launch2D& L, int dim, hipStream_t s) {
     HookScope hs(L, "finalize");
     const int nb = pred_blocks(L.c);
     double* red = L.red + PRED_NB;
-    hipLaunchKernelGGL((k_part_sum), dim3(nb), dim3(PRED_NT), 0, s, L.c, dim, L.st, L.part, red);
-    hipLaunchKernelGGL((k_finalize), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, red, nb);
+    if (nb > 0)
+        hipLaunchKernelGGL((k_part_sum), dim3(nb), dim3(PRED_NT), 0, s, L.c, dim, L.st, L.part, red);
+    hipLaunchKernelGGL((k_finalize), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, L.part, red, nb);
     return hipGetLastError();
 }
 
@@ -25,8 +31,9 @@ hipError_t launch_begin(DevState* st, double t_stop, hipStream_t s) {
 hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s) {
     HookScope hs(L, "timestep");
     const int nb = pred_blocks(L.c);
-    hipLaunchKernelGGL((k_rate_part), dim3(nb), dim3(PRED_NT), 0, s, L.c, dim, L.st, L.W, L.red);
-    hipLaunchKernelGGL((k_timestep), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, L.red, nb);
+    if (nb > 0)
+        hipLaunchKernelGGL((k_rate_part), dim3(nb), dim3(PRED_NT), 0, s, L.c, dim, L.st, L.W, L.red);
+    hipLaunchKernelGGL((k_timestep), dim3(1), dim3(RED_NT), 0, s, L.c, dim, L.st, L.W, L.red, nb);
     return hipGetLastError();
 }
 
