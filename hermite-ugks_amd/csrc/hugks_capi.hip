@@ -237,9 +237,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
                                                          !(d->mass_tol >= 0))))
         return fail(nullptr, HUGKS_E_ARG, "steady test needs dim 1 and tolerances >= 0");
     if (!(d->cfl > 0) || !(d->prandtl > 0) || !(d->muref > 0) || !(d->gamma > 1) ||
-        !(d->innerk >= 0) || !(d->pi > 0))
+        !(d->innerk >= 0) || !(d->pi > 0) || !(d->smv > 0))
         return fail(nullptr, HUGKS_E_ARG,
-                    "cfl, prandtl, muref, pi must be positive, gamma > 1, innerk >= 0");
+                    "cfl, prandtl, muref, pi, smv must be positive, gamma > 1, innerk >= 0");
     for (int s = 0; s < (dim == 2 ? 4 : 2); ++s) {
         if (d->bc[s] < HUGKS_BC_PERIODIC || d->bc[s] > HUGKS_BC_DIFFUSE)
             return fail(nullptr, HUGKS_E_ARG, "unknown boundary kind");

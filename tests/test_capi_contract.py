@@ -108,7 +108,7 @@ def create(desc):
 BAD = [
     ("abi_version", 99), ("dim", 3), ("nx", 0), ("ny", -1), ("nv", 1), ("nm", -1),
     ("precision", 7), ("tau_model", 5), ("dt_rule", 9), ("force_variant", 4), ("dx", 0.0),
-    ("dy", -1.0), ("cfl", 0.0), ("prandtl", 0.0),
+    ("dy", -1.0), ("cfl", 0.0), ("prandtl", 0.0), ("smv", 0.0),
 ]
 
 

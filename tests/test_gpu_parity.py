@@ -142,3 +142,18 @@ def test_flux_weights_not_a_tensor_product(have_gpu):
     err = field_err(r["w"], r["wr"])
     print("perturbed weights W err", err, flush=True)
     assert np.all(err <= TOL_FP64), err
+
+
+@pytest.mark.parametrize("mode", ["split", "stencil", "fused"])
+def test_odd_velocity_grids(have_gpu, monkeypatch, mode):
+    """nv^2 odd (the face kernels read node pairs, the last pair ending on
+    the first pad node) and nv = 14 (a pair of nodes on both sides of the
+    upwind switch), FP64 and mixed, against the oracle."""
+    monkeypatch.setenv("HUGKS_FLUX_KERNEL", mode)
+    for nv in (13, 14, 15):
+        for prec, tol in ((capi.FP64, TOL_FP64), (capi.MIXED_FP32, TOL_MIXED)):
+            case = cases.build_2d(rt(nx=6, ny=8, nv=nv, nm=8), prec, rt_variant="cartesian", seed=nv)
+            r = run_pair(case, 4)
+            err = field_err(r["w"], r["wr"])
+            print(mode, "nv", nv, "prec", prec, "W err", err, flush=True)
+            assert np.all(err <= tol), err
