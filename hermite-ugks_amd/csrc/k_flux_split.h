@@ -120,32 +120,6 @@ struct Upwind {
     size_t baseL, baseR;
     Stencil S;
     A idN, idT, smv;
-    // Nodes k and k+1 (k even) from the slope arrays with one two-element
-    // load per array; node k+1 is re-read alone in the rare pair whose two
-    // nodes take different upwind cells.
-    __device__ __forceinline__ void get2(int d0, int d1, int k, A (&hu)[2], A (&bu)[2],
-                                         A (&sn)[2], A (&st)[2], A (&bn)[2], A (&bt)[2]) const {
-        struct alignas(2 * sizeof(T)) P2 { T x, y; };
-        const size_t o = (d0 ? baseL : baseR) + k;
-        const P2 ph = *reinterpret_cast<const P2*>(h + o), pb = *reinterpret_cast<const P2*>(b + o);
-        const P2 pn = *reinterpret_cast<const P2*>(sN + o), pq = *reinterpret_cast<const P2*>(bN + o);
-        const P2 pt = *reinterpret_cast<const P2*>(sT + o), pr = *reinterpret_cast<const P2*>(bT + o);
-        hu[0] = (A)ph.x; hu[1] = (A)ph.y;
-        bu[0] = (A)pb.x; bu[1] = (A)pb.y;
-        sn[0] = (A)pn.x; sn[1] = (A)pn.y;
-        bn[0] = (A)pq.x; bn[1] = (A)pq.y;
-        st[0] = (A)pt.x; st[1] = (A)pt.y;
-        bt[0] = (A)pr.x; bt[1] = (A)pr.y;
-        if (d1 != d0) {
-            const size_t o1 = (d1 ? baseL : baseR) + k + 1;
-            hu[1] = (A)h[o1];
-            bu[1] = (A)b[o1];
-            sn[1] = (A)sN[o1];
-            bn[1] = (A)bN[o1];
-            st[1] = (A)sT[o1];
-            bt[1] = (A)bT[o1];
-        }
-    }
     __device__ __forceinline__ void get(int d, int k, A& hu, A& bu, A& sn, A& st, A& bn,
                                         A& bt) const {
         if constexpr (!ST) {
@@ -213,8 +187,13 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
     double s[FSUM_N];
 #pragma unroll
     for (int q = 0; q < FSUM_N; ++q) s[q] = 0.0;
-    // one node's contributions to the 37 sums
-    auto node = [&](int k, int d, A hu, A bu, A snA, A stA, A bnA, A btA) {
+    constexpr int UNR = ST ? 1 : 2;   // the stencil holds ten loads per node
+#pragma unroll UNR
+    for (int k = lane; k < c.nn; k += WAVE) {
+        // upwind choice in the node arithmetic, as the fused kernel makes it
+        const int d = upwind_delta((DIR == 0 ? unode[k] : vnode[k]) - vs);
+        A hu, bu, snA, stA, bnA, btA;
+        up.get(d, k, hu, bu, snA, stA, bnA, btA);
         const double sgn = d ? hl : -hl;
         const double sn = (double)snA, bn = (double)bnA;
         const double hf = (double)hu + sgn * sn;
@@ -263,28 +242,6 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
         s[34] += vt2vn * a;
         s[35] += (vt2 * vt) * a;
         s[36] += vt * g;
-    };
-    if constexpr (ST) {
-        for (int k = lane; k < c.nn; k += WAVE) {
-            // upwind choice in the node arithmetic, as the fused kernel makes it
-            const int d = upwind_delta((DIR == 0 ? unode[k] : vnode[k]) - vs);
-            A hu, bu, snA, stA, bnA, btA;
-            up.get(d, k, hu, bu, snA, stA, bnA, btA);
-            node(k, d, hu, bu, snA, stA, bnA, btA);
-        }
-    } else {
-        // node pairs (k, k+1); with nv^2 odd the last pair's second node is
-        // the first pad node (weight zero, finite zero data)
-        const int npair = (c.nn + 1) / 2;
-        for (int kp = lane; kp < npair; kp += WAVE) {
-            const int k = 2 * kp;
-            const int d0 = upwind_delta((DIR == 0 ? unode[k] : vnode[k]) - vs);
-            const int d1 = upwind_delta((DIR == 0 ? unode[k + 1] : vnode[k + 1]) - vs);
-            A hu[2], bu[2], snA[2], stA[2], bnA[2], btA[2];
-            up.get2(d0, d1, k, hu, bu, snA, stA, bnA, btA);
-            node(k, d0, hu[0], bu[0], snA[0], stA[0], bnA[0], btA[0]);
-            node(k + 1, d1, hu[1], bu[1], snA[1], stA[1], bnA[1], btA[1]);
-        }
     }
     int q;
     const double x = wave_reduce_scatter<FSUM_N>(s, q);
@@ -736,8 +693,13 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
         a_b[q] = (A)fc[FC_AB + q];
         a_T[q] = (A)fc[FC_AT + q];
     }
-    // one node's flux of h and b
-    auto node = [&](A vn, A vt, int d, A hu, A bu, A sn, A stR, A bn, A btR, A& oh, A& ob) {
+#pragma unroll 2
+    for (int k = lane; k < c.nn; k += WAVE) {
+        const A u = unode[k], v = vnode[k];
+        const A vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
+        const int d = upwind_delta(vn - vshift);
+        A hu, bu, sn, stR, bn, btR;
+        up.get(d, k, hu, bu, sn, stR, bn, btR);
         const A sgn = d ? hlen : -hlen;
         const A hr = hu + sgn * sn;
         const A br = bu + sgn * bn;
@@ -768,43 +730,8 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
         xb = xb - tm2fn * vn * gb + mt6 * fnA * vn * bn;
         xh = xh - tm2ft * vn * gbh + mt6 * ftA * vn * st;
         xb = xb - tm2ft * vn * gbb + mt6 * ftA * vn * bt;
-        oh = flenA * xh;
-        ob = flenA * xb;
-    };
-    if constexpr (ST) {
-        for (int k = lane; k < c.nn; k += WAVE) {
-            const A u = unode[k], v = vnode[k];
-            const A vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
-            const int d = upwind_delta(vn - vshift);
-            A hu, bu, sn, stR, bn, btR, oh, ob;
-            up.get(d, k, hu, bu, sn, stR, bn, btR);
-            node(vn, vt, d, hu, bu, sn, stR, bn, btR, oh, ob);
-            ofh[k] = (T)oh;
-            ofb[k] = (T)ob;
-        }
-    } else {
-        // node pairs (k, k+1), one two-element store per array; the pad
-        // node after an odd nv^2 is not written
-        struct alignas(2 * sizeof(T)) P2 { T x, y; };
-        const int npair = (c.nn + 1) / 2;
-        for (int kp = lane; kp < npair; kp += WAVE) {
-            const int k = 2 * kp;
-            const A u0 = unode[k], v0 = vnode[k], u1 = unode[k + 1], v1 = vnode[k + 1];
-            const A vn0 = DIR == 0 ? u0 : v0, vt0 = DIR == 0 ? v0 : -u0;
-            const A vn1 = DIR == 0 ? u1 : v1, vt1 = DIR == 0 ? v1 : -u1;
-            const int d0 = upwind_delta(vn0 - vshift), d1 = upwind_delta(vn1 - vshift);
-            A hu[2], bu[2], sn[2], stR[2], bn[2], btR[2], oh[2], ob[2];
-            up.get2(d0, d1, k, hu, bu, sn, stR, bn, btR);
-            node(vn0, vt0, d0, hu[0], bu[0], sn[0], stR[0], bn[0], btR[0], oh[0], ob[0]);
-            node(vn1, vt1, d1, hu[1], bu[1], sn[1], stR[1], bn[1], btR[1], oh[1], ob[1]);
-            if (k + 1 < c.nn) {
-                *reinterpret_cast<P2*>(ofh + k) = P2{(T)oh[0], (T)oh[1]};
-                *reinterpret_cast<P2*>(ofb + k) = P2{(T)ob[0], (T)ob[1]};
-            } else {
-                ofh[k] = (T)oh[0];
-                ofb[k] = (T)ob[0];
-            }
-        }
+        ofh[k] = (T)(flenA * xh);
+        ofb[k] = (T)(flenA * xb);
     }
 }
 
