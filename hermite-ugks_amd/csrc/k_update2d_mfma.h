@@ -45,7 +45,12 @@ template <> struct Mfma<double> {
     }
 };
 
-constexpr int MF_NT = 256;   // threads per workgroup (one cell)
+// Threads per workgroup (one cell): 256 or 320, whichever leaves fewer of
+// the EPT x threads node slots idle (mf_threads).
+__host__ __device__ inline int mf_threads(int nn) {
+    const int w256 = (nn + 255) / 256 * 256 - nn, w320 = (nn + 319) / 320 * 320 - nn;
+    return w320 < w256 ? 320 : 256;
+}
 constexpr int MF_NB = 16;    // Hermite basis rows He_0..He_15
 
 // r! for r = 0..15 (exact in double)
@@ -182,8 +187,8 @@ __global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict
 
 // Shakhov-BGK update of one cell per workgroup with the Hermite force on
 // MFMA; the cell's scalars come from k_cell_pre.
-template <typename T, int EPT>
-__global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
+template <typename T, int EPT, int NT>
+__global__ __launch_bounds__(NT) void k_update2d_mfma(
     Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb,
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     using M = Mfma<T>;
     using V = typename M::V;
     HIP_DYNAMIC_SHARED(double, smem_raw)
-    __shared__ double red[4 * (MF_NT / WAVE)];
+    __shared__ double red[4 * (NT / WAVE)];
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const int cell = tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     const double rt = P[CP_RT];
     const double sqrt_rt = sqrt(rt);
     const double om_scale = dt / (2.0 * c.pi * sqrt_rt);
-    for (int t = tid; t < 2 * np; t += MF_NT) {
+    for (int t = tid; t < 2 * np; t += NT) {
         const bool isx = t < np;
         const int idx = isx ? t : t - np;
         T* Hb = isx ? Hx : Hy;
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     double q4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-        const int k = tid + e * MF_NT;
+        const int k = tid + e * NT;
         rh[e] = rb[e] = (T)0;
         if (k < nn) {
             const size_t o = (size_t)cell * c.ldv + k;
@@ -292,10 +297,10 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
             G[GS + jv * ldG + iu] = (T)(wa * (A)rb[e]);
         }
     }
-    block_sum<4, MF_NT>(q4, red);   // also orders the LDS writes above
+    block_sum<4, NT>(q4, red);   // also orders the LDS writes above
     const double qf0 = 0.5 * (q4[0] + q4[1]), qf1 = 0.5 * (q4[2] + q4[3]);
     // Shakhov projections (read after the barriers of the contractions)
-    for (int t = tid; t < 2 * nv; t += MF_NT) {
+    for (int t = tid; t < 2 * nv; t += NT) {
         const bool isx = t < nv;
         const int idx = isx ? t : t - nv;
         double* rec = (isx ? TX : TY) + idx;
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     const int lr = lane & 15, lk = lane >> 4;
 
     // ---- Tm = Hx G: tile (f, jt), rows r, columns jv
-    for (int job = wv; job < 2 * nt; job += MF_NT / WAVE) {
+    for (int job = wv; job < 2 * nt; job += NT / WAVE) {
         const int f = job / nt, jt = job % nt;
         const int jv = 16 * jt + lr;
         V acc = {0, 0, 0, 0};
@@ -327,7 +332,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     __syncthreads();
     // ---- D = Tm Hy^T / (r! s!), r + s <= Nm
     const double jac = 1.0 / rt;
-    for (int f = wv; f < 2; f += MF_NT / WAVE) {
+    for (int f = wv; f < 2; f += NT / WAVE) {
         V acc = {0, 0, 0, 0};
         for (int ks = 0; ks < ksn; ++ks) {
             const int jv = 4 * ks + lk;
@@ -346,7 +351,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     __syncthreads();
     // ---- R(r, jv) = ax sum_s D(r-1, s) Hy(s, jv) + ay sum_s D(r, s) Hy(s+1, jv)
     const A axA = (A)ax, ayA = (A)ay;
-    for (int job = wv; job < 2 * nt; job += MF_NT / WAVE) {
+    for (int job = wv; job < 2 * nt; job += NT / WAVE) {
         const int f = job / nt, jt = job % nt;
         const int jv = 16 * jt + lr;
         V acc = {0, 0, 0, 0};
@@ -364,7 +369,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     }
     __syncthreads();
     // ---- F = Hx^T R, into the G region (G is dead)
-    for (int job = wv; job < 2 * nt * nt; job += MF_NT / WAVE) {
+    for (int job = wv; job < 2 * nt * nt; job += NT / WAVE) {
         const int f = job / (nt * nt), rem = job % (nt * nt);
         const int it = rem / nt, jt = rem % nt;
         const int iu_a = 16 * it + lr, jv = 16 * jt + lr;
@@ -401,7 +406,7 @@ __global__ __launch_bounds__(MF_NT) void k_update2d_mfma(
     T* bo_p = b + (size_t)cell * c.ldv;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-        const int k = tid + e * MF_NT;
+        const int k = tid + e * NT;
         if (k < nn) {
             const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
             const double* rx = TX + iu;

@@ -55,39 +55,44 @@ size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64) {
 template <typename T>
 static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool prepare) {
     const Consts& c = L.c;
-    const int ept = (c.nn + MF_NT - 1) / MF_NT;
+    const int nt = mf_threads(c.nn);
+    const int ept = (c.nn + nt - 1) / nt;
     const unsigned ncell = (unsigned)(c.nx * c.ny);
     const size_t lds = update2d_mfma_lds_bytes(c, sizeof(T) == 8);
     using A = typename Arith<T>::A;
-#define HUGKS_UPDM_CASE(E)                                                                      \
-    case E: {                                                                                   \
+#define HUGKS_UPDM_CASE(E, N)                                                                   \
+    if (ept == E && nt == N) {                                                                  \
         if (prepare)                                                                            \
-            return hipFuncSetAttribute((const void*)k_update2d_mfma<T, E>,                      \
+            return hipFuncSetAttribute((const void*)k_update2d_mfma<T, E, N>,                   \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
         HookScope hs(L, "update2d");                                                            \
-        hipLaunchKernelGGL((k_update2d_mfma<T, E>), dim3(ncell), dim3(MF_NT), lds, s, c, L.st,  \
+        hipLaunchKernelGGL((k_update2d_mfma<T, E, N>), dim3(ncell), dim3(N), lds, s, c, L.st,   \
             L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,     \
-            (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.u1, L.v1);   \
-        break;                                                                                  \
+            (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.u1, L.v1);                         \
+        return hipGetLastError();                                                               \
     }
     if (!prepare) {
         HookScope hs(L, "cell_pre");
         hipLaunchKernelGGL((k_cell_pre), dim3((ncell + 255) / 256), dim3(256), 0, s, c, L.st, L.W,
                            L.part, L.vflux, L.hflux, L.acc_cell, L.cellpre);
     }
-    switch (ept) {
-        HUGKS_UPDM_CASE(1)
-        HUGKS_UPDM_CASE(2)
-        HUGKS_UPDM_CASE(3)
-        HUGKS_UPDM_CASE(4)
-        HUGKS_UPDM_CASE(5)
-        HUGKS_UPDM_CASE(6)
-        HUGKS_UPDM_CASE(7)
-        HUGKS_UPDM_CASE(8)
-        default: return hipErrorInvalidValue;
-    }
+    HUGKS_UPDM_CASE(1, 256)
+    HUGKS_UPDM_CASE(2, 256)
+    HUGKS_UPDM_CASE(3, 256)
+    HUGKS_UPDM_CASE(4, 256)
+    HUGKS_UPDM_CASE(5, 256)
+    HUGKS_UPDM_CASE(6, 256)
+    HUGKS_UPDM_CASE(7, 256)
+    HUGKS_UPDM_CASE(8, 256)
+    HUGKS_UPDM_CASE(1, 320)
+    HUGKS_UPDM_CASE(2, 320)
+    HUGKS_UPDM_CASE(3, 320)
+    HUGKS_UPDM_CASE(4, 320)
+    HUGKS_UPDM_CASE(5, 320)
+    HUGKS_UPDM_CASE(6, 320)
+    HUGKS_UPDM_CASE(7, 320)
 #undef HUGKS_UPDM_CASE
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
