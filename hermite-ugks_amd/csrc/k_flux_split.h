@@ -187,7 +187,7 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
     double s[FSUM_N];
 #pragma unroll
     for (int q = 0; q < FSUM_N; ++q) s[q] = 0.0;
-    constexpr int UNR = ST ? 1 : 4;   // loads in flight; the stencil holds ten per node
+    constexpr int UNR = ST ? 1 : 2;   // the stencil holds ten loads per node
 #pragma unroll UNR
     for (int k = lane; k < c.nn; k += WAVE) {
         // upwind choice in the node arithmetic, as the fused kernel makes it

@@ -45,11 +45,13 @@ template <> struct Mfma<double> {
     }
 };
 
-// Threads per workgroup (one cell): 256 or 320, whichever leaves fewer of
-// the EPT x threads node slots idle (mf_threads).
-__host__ __device__ inline int mf_threads(int nn) {
+// Threads per workgroup (one cell).  FP64: 256 or 320, whichever leaves
+// fewer of the EPT x threads node slots idle (at nv = 40, 320 x 5 nodes:
+// 14.0 -> 12.7 ms at 512^2 cells).  FP32 storage: 256 (320 measured slower,
+// 5.9 -> 6.8 ms).
+__host__ __device__ inline int mf_threads(int nn, bool fp64) {
     const int w256 = (nn + 255) / 256 * 256 - nn, w320 = (nn + 319) / 320 * 320 - nn;
-    return w320 < w256 ? 320 : 256;
+    return fp64 && w320 < w256 ? 320 : 256;
 }
 constexpr int MF_NB = 16;    // Hermite basis rows He_0..He_15
 

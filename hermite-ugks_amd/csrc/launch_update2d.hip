@@ -55,7 +55,7 @@ size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64) {
 template <typename T>
 static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool prepare) {
     const Consts& c = L.c;
-    const int nt = mf_threads(c.nn);
+    const int nt = mf_threads(c.nn, sizeof(T) == 8);
     const int ept = (c.nn + nt - 1) / nt;
     const unsigned ncell = (unsigned)(c.nx * c.ny);
     const size_t lds = update2d_mfma_lds_bytes(c, sizeof(T) == 8);
