@@ -659,6 +659,94 @@ int hugks_get_time(hugks_handle* h, double* simtime, int32_t* iter) {
     return HUGKS_OK;
 }
 
+// ----------------------------------------------------------- checkpoint
+namespace {
+constexpr char kCkMagic[8] = {'H', 'U', 'G', 'K', 'S', 'C', 'K', '1'};
+struct CkHeader {
+    char magic[8];
+    int32_t abi, dim, nx, ny, nv, nm, precision, ldv;
+    int32_t iter, steady_count, flags, pad;
+    double dt, simtime, mass0, mass_rel, res[4], steady[3];
+};
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+        if (f) fclose(f);
+    }
+};
+}  // namespace
+
+int hugks_checkpoint_write(hugks_handle* h, const char* path) {
+    if (!h || !path) return fail(h, HUGKS_E_ARG, "null handle or path");
+    const Consts& c = h->L.c;
+    const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
+    const size_t fbytes = h->ncell * c.ldv * es, wbytes = h->ncell * (h->dim + 2) * sizeof(double);
+    DevState st;
+    hipError_t e = hipMemcpyAsync(&st, h->L.st, sizeof(st), hipMemcpyDeviceToHost, h->stream);
+    std::vector<char> buf(wbytes > fbytes ? wbytes : fbytes);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "checkpoint_write");
+    CkHeader hd;
+    memset(&hd, 0, sizeof(hd));
+    memcpy(hd.magic, kCkMagic, 8);
+    hd.abi = HUGKS_ABI_VERSION;
+    hd.dim = h->dim; hd.nx = c.nx; hd.ny = c.ny; hd.nv = c.nv; hd.nm = c.nm;
+    hd.precision = h->d.precision; hd.ldv = c.ldv;
+    hd.iter = st.iter; hd.steady_count = st.steady_count; hd.flags = st.flags;
+    hd.dt = st.dt; hd.simtime = st.simtime; hd.mass0 = st.mass0; hd.mass_rel = st.mass_rel;
+    for (int q = 0; q < 4; ++q) hd.res[q] = st.res[q];
+    for (int q = 0; q < 3; ++q) hd.steady[q] = st.steady[q];
+    FileCloser fc{fopen(path, "wb")};
+    if (!fc.f) return fail(h, HUGKS_E_ARG, std::string("cannot open for writing: ") + path);
+    bool ok = fwrite(&hd, sizeof(hd), 1, fc.f) == 1;
+    const void* srcs[3] = {h->L.W, h->L.h, h->L.b};
+    const size_t sizes[3] = {wbytes, fbytes, fbytes};
+    for (int a = 0; a < 3 && ok; ++a) {
+        e = hipMemcpy(buf.data(), srcs[a], sizes[a], hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(h, e, "checkpoint_write");
+        ok = fwrite(buf.data(), 1, sizes[a], fc.f) == sizes[a];
+    }
+    if (!ok || fflush(fc.f) != 0) return fail(h, HUGKS_E_ARG, std::string("write failed: ") + path);
+    return HUGKS_OK;
+}
+
+int hugks_checkpoint_read(hugks_handle* h, const char* path) {
+    if (!h || !path) return fail(h, HUGKS_E_ARG, "null handle or path");
+    const Consts& c = h->L.c;
+    const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
+    const size_t fbytes = h->ncell * c.ldv * es, wbytes = h->ncell * (h->dim + 2) * sizeof(double);
+    FileCloser fc{fopen(path, "rb")};
+    if (!fc.f) return fail(h, HUGKS_E_ARG, std::string("cannot open: ") + path);
+    CkHeader hd;
+    if (fread(&hd, sizeof(hd), 1, fc.f) != 1 || memcmp(hd.magic, kCkMagic, 8) != 0)
+        return fail(h, HUGKS_E_ARG, std::string("not a checkpoint file: ") + path);
+    if (hd.abi != HUGKS_ABI_VERSION || hd.dim != h->dim || hd.nx != c.nx || hd.ny != c.ny ||
+        hd.nv != c.nv || hd.precision != h->d.precision || hd.ldv != c.ldv)
+        return fail(h, HUGKS_E_ARG, "checkpoint shape or precision differs from the handle");
+    std::vector<char> buf(wbytes > fbytes ? wbytes : fbytes);
+    void* dsts[3] = {h->L.W, h->L.h, h->L.b};
+    const size_t sizes[3] = {wbytes, fbytes, fbytes};
+    for (int a = 0; a < 3; ++a) {
+        if (fread(buf.data(), 1, sizes[a], fc.f) != sizes[a])
+            return fail(h, HUGKS_E_ARG, std::string("checkpoint file is truncated: ") + path);
+        hipError_t e = hipMemcpy(dsts[a], buf.data(), sizes[a], hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(h, e, "checkpoint_read");
+    }
+    DevState st;
+    hipError_t e = hipMemcpy(&st, h->L.st, sizeof(st), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "checkpoint_read");
+    st.iter = hd.iter; st.steady_count = hd.steady_count; st.flags = hd.flags;
+    st.dt = hd.dt; st.simtime = hd.simtime; st.mass0 = hd.mass0; st.mass_rel = hd.mass_rel;
+    for (int q = 0; q < 4; ++q) st.res[q] = hd.res[q];
+    for (int q = 0; q < 3; ++q) st.steady[q] = hd.steady[q];
+    st.stopped = 0;
+    st.stop_reason = HUGKS_STOP_NONE;
+    st.steps = 0;
+    e = hipMemcpy(h->L.st, &st, sizeof(st), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(h, e, "checkpoint_read");
+    return HUGKS_OK;
+}
+
 int hugks_kernel_times(hugks_handle* h, int32_t cap, const char** names, double* ms,
                        int32_t* calls, int32_t* count) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");

@@ -130,6 +130,20 @@ module hugks_capi
       integer(c_int32_t), intent(out) :: iter
     end function hugks_get_time
 
+    integer(c_int) function hugks_checkpoint_write_c(handle, path) &
+         bind(c, name='hugks_checkpoint_write')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: handle
+      character(kind=c_char), intent(in) :: path(*)
+    end function hugks_checkpoint_write_c
+
+    integer(c_int) function hugks_checkpoint_read_c(handle, path) &
+         bind(c, name='hugks_checkpoint_read')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: handle
+      character(kind=c_char), intent(in) :: path(*)
+    end function hugks_checkpoint_read_c
+
     type(c_ptr) function hugks_device_info_c() bind(c, name='hugks_device_info')
       import :: c_ptr
     end function hugks_device_info_c
@@ -187,6 +201,19 @@ contains
     character(len=:), allocatable :: msg
     msg = c_string(hugks_last_error_c(handle))
   end function hugks_last_error
+
+  ! Whole solver state to / from a file (C strings built here).
+  integer(c_int) function hugks_checkpoint_write(handle, path)
+    type(c_ptr), intent(in) :: handle
+    character(len=*), intent(in) :: path
+    hugks_checkpoint_write = hugks_checkpoint_write_c(handle, trim(path) // c_null_char)
+  end function hugks_checkpoint_write
+
+  integer(c_int) function hugks_checkpoint_read(handle, path)
+    type(c_ptr), intent(in) :: handle
+    character(len=*), intent(in) :: path
+    hugks_checkpoint_read = hugks_checkpoint_read_c(handle, trim(path) // c_null_char)
+  end function hugks_checkpoint_read
 
   function hugks_device_info() result(msg)
     character(len=:), allocatable :: msg

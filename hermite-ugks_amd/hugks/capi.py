@@ -63,7 +63,8 @@ class StepInfo(C.Structure):
 EXPORTS = [
     "hugks_create", "hugks_destroy", "hugks_last_error", "hugks_step", "hugks_advance",
     "hugks_advance_until",
-    "hugks_get_macro", "hugks_get_f", "hugks_set_f", "hugks_get_time", "hugks_kernel_times",
+    "hugks_get_macro", "hugks_get_f", "hugks_set_f", "hugks_get_time",
+    "hugks_checkpoint_write", "hugks_checkpoint_read", "hugks_kernel_times",
     "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
     "hugks_set_schedule", "hugks_get_schedule", "hugks_tune", "hugks_batch_create",
     "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_handle",
@@ -97,6 +98,8 @@ def load(path=LIBPATH):
     lib.hugks_get_f.argtypes = [H, _dp, _dp]
     lib.hugks_set_f.argtypes = [H, _dp, _dp, _dp]
     lib.hugks_get_time.argtypes = [H, _dp, C.POINTER(C.c_int32)]
+    lib.hugks_checkpoint_write.argtypes = [H, C.c_char_p]
+    lib.hugks_checkpoint_read.argtypes = [H, C.c_char_p]
     lib.hugks_kernel_times.argtypes = [H, C.c_int32, C.POINTER(C.c_char_p), _dp,
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.hugks_reset_kernel_times.argtypes = [H]
@@ -252,6 +255,14 @@ class Solver:
         out = np.empty(rows)
         self._check(self.lib.hugks_debug_fetch(self.h, int(which), _ptr(out)))
         return out
+
+    def checkpoint(self, path):
+        """Writes the whole solver state to path (hugks_checkpoint_write)."""
+        self._check(self.lib.hugks_checkpoint_write(self.h, str(path).encode()))
+
+    def restore(self, path):
+        """Loads a state written by checkpoint() into this handle."""
+        self._check(self.lib.hugks_checkpoint_read(self.h, str(path).encode()))
 
     def time(self):
         t = C.c_double()

@@ -254,6 +254,17 @@ int hugks_batch_advance(hugks_batch *b, int32_t max_steps, hugks_step_info *info
 /* Loop state (simtime, iter) of the handle. */
 int hugks_get_time(hugks_handle *h, double *simtime, int32_t *iter);
 
+/* Checkpoint / restore of the whole solver state (the reference has no
+ * restart, 2d/src/initialization.f90:17-18).  The file holds the shape and
+ * precision, the loop state (simtime, iter, dt, residuals, mass reference,
+ * steady counter) and W, h, b exactly as stored on the device (FP32 f on
+ * the mixed paths), so a handle restored from it continues bitwise as the
+ * one that wrote it.  hugks_checkpoint_read needs a handle created with
+ * the same dim, nx, ny, nv and precision; HUGKS_E_ARG names any mismatch or
+ * an unreadable file. */
+int hugks_checkpoint_write(hugks_handle *h, const char *path);
+int hugks_checkpoint_read(hugks_handle *h, const char *path);
+
 /* Per-kernel device time accumulated since the last reset, from HIP
  * events around each launch.  names/ms/calls have room for cap entries;
  * *count receives the number of kernels.  names point into the library. */
