@@ -189,7 +189,10 @@ __global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict
 
 // Shakhov-BGK update of one cell per workgroup with the Hermite force on
 // MFMA; the cell's scalars come from k_cell_pre.
-template <typename T, int EPT, int NT>
+// NVC > 0: the velocity grid size as a compile-time constant (common
+// Gauss-Hermite sizes), so that the LDS layout and the tile loops fold;
+// NVC = 0: any nv, read at run time.
+template <typename T, int EPT, int NT, int NVC>
 __global__ __launch_bounds__(NT) void k_update2d_mfma(
     Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
@@ -204,7 +207,8 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const int cell = tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
-    const int nx = c.nx, nv = c.nv, nm = c.nm, nn = c.nn;
+    const int nx = c.nx, nm = c.nm;
+    const int nv = NVC ? NVC : c.nv, nn = NVC ? NVC * NVC : c.nn;
     const int np = mf_np(nv);
     const int i = cell % nx, j = cell / nx;
     const double dt = st->dt;
@@ -321,6 +325,7 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
         const int f = job / nt, jt = job % nt;
         const int jv = 16 * jt + lr;
         V acc = {0, 0, 0, 0};
+#pragma unroll
         for (int ks = 0; ks < ksn; ++ks) {
             const int iu = 4 * ks + lk;
             const bool in = iu < nv;
@@ -336,6 +341,7 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
     const double jac = 1.0 / rt;
     for (int f = wv; f < 2; f += NT / WAVE) {
         V acc = {0, 0, 0, 0};
+#pragma unroll
         for (int ks = 0; ks < ksn; ++ks) {
             const int jv = 4 * ks + lk;
             const T a = Tm[(f * MF_NB + lr) * sH + jv];     // zero beyond nv

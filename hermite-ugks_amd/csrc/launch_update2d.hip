@@ -60,15 +60,15 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool pr
     const unsigned ncell = (unsigned)(c.nx * c.ny);
     const size_t lds = update2d_mfma_lds_bytes(c, sizeof(T) == 8);
     using A = typename Arith<T>::A;
-#define HUGKS_UPDM_CASE(E, N)                                                                   \
-    if (ept == E && nt == N) {                                                                  \
+#define HUGKS_UPDM_CASE(E, N, V)                                                                \
+    if (ept == E && nt == N && nvc == V) {                                                      \
         if (prepare)                                                                            \
-            return hipFuncSetAttribute((const void*)k_update2d_mfma<T, E, N>,                   \
+            return hipFuncSetAttribute((const void*)k_update2d_mfma<T, E, N, V>,                \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
         HookScope hs(L, "update2d");                                                            \
-        hipLaunchKernelGGL((k_update2d_mfma<T, E, N>), dim3(ncell), dim3(N), lds, s, c, L.st,   \
-            L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,     \
-            (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.u1, L.v1);                         \
+        hipLaunchKernelGGL((k_update2d_mfma<T, E, N, V>), dim3(ncell), dim3(N), lds, s, c,      \
+            L.st, L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,                \
+            (const T*)L.hfh, (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.u1, L.v1);        \
         return hipGetLastError();                                                               \
     }
     if (!prepare) {
@@ -76,21 +76,40 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool pr
         hipLaunchKernelGGL((k_cell_pre), dim3((ncell + 255) / 256), dim3(256), 0, s, c, L.st, L.W,
                            L.part, L.vflux, L.hflux, L.acc_cell, L.cellpre);
     }
-    HUGKS_UPDM_CASE(1, 256)
-    HUGKS_UPDM_CASE(2, 256)
-    HUGKS_UPDM_CASE(3, 256)
-    HUGKS_UPDM_CASE(4, 256)
-    HUGKS_UPDM_CASE(5, 256)
-    HUGKS_UPDM_CASE(6, 256)
-    HUGKS_UPDM_CASE(7, 256)
-    HUGKS_UPDM_CASE(8, 256)
-    HUGKS_UPDM_CASE(1, 320)
-    HUGKS_UPDM_CASE(2, 320)
-    HUGKS_UPDM_CASE(3, 320)
-    HUGKS_UPDM_CASE(4, 320)
-    HUGKS_UPDM_CASE(5, 320)
-    HUGKS_UPDM_CASE(6, 320)
-    HUGKS_UPDM_CASE(7, 320)
+    // compiled-in grid sizes (EPT and threads as mf_threads chooses them)
+    int nvc = (c.nv == 16 || c.nv == 20 || c.nv == 24 || c.nv == 28 || c.nv == 32 ||
+                     c.nv == 40) ? c.nv : 0;
+    if (sizeof(T) == 4) {
+        HUGKS_UPDM_CASE(1, 256, 16)
+        HUGKS_UPDM_CASE(2, 256, 20)
+        HUGKS_UPDM_CASE(3, 256, 24)
+        HUGKS_UPDM_CASE(4, 256, 28)
+        HUGKS_UPDM_CASE(4, 256, 32)
+        HUGKS_UPDM_CASE(7, 256, 40)
+    } else {
+        HUGKS_UPDM_CASE(1, 256, 16)
+        HUGKS_UPDM_CASE(2, 256, 20)
+        HUGKS_UPDM_CASE(2, 320, 24)
+        HUGKS_UPDM_CASE(3, 320, 28)
+        HUGKS_UPDM_CASE(4, 256, 32)
+        HUGKS_UPDM_CASE(5, 320, 40)
+    }
+    nvc = 0;   // any other size (or shape) on the run-time kernel
+    HUGKS_UPDM_CASE(1, 256, 0)
+    HUGKS_UPDM_CASE(2, 256, 0)
+    HUGKS_UPDM_CASE(3, 256, 0)
+    HUGKS_UPDM_CASE(4, 256, 0)
+    HUGKS_UPDM_CASE(5, 256, 0)
+    HUGKS_UPDM_CASE(6, 256, 0)
+    HUGKS_UPDM_CASE(7, 256, 0)
+    HUGKS_UPDM_CASE(8, 256, 0)
+    HUGKS_UPDM_CASE(1, 320, 0)
+    HUGKS_UPDM_CASE(2, 320, 0)
+    HUGKS_UPDM_CASE(3, 320, 0)
+    HUGKS_UPDM_CASE(4, 320, 0)
+    HUGKS_UPDM_CASE(5, 320, 0)
+    HUGKS_UPDM_CASE(6, 320, 0)
+    HUGKS_UPDM_CASE(7, 320, 0)
 #undef HUGKS_UPDM_CASE
     return hipErrorInvalidValue;
 }
