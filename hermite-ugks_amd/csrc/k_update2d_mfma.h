@@ -62,7 +62,7 @@ static __constant__ const double kFact[MF_NB] = {
 
 __host__ __device__ constexpr int mf_np(int nv) { return (nv + 15) / 16 * 16; }
 
-// Per-axis records of the node-wise update (doubles, one array of NP per
+// Per-axis records of the node-wise update (doubles, one array of nv per
 // slot): for the u axis (TX, by iu) and the v axis (TY, by jv) every factor
 // that depends on that velocity component alone, so that a node's
 // Maxwellians, Shakhov terms and force weight are a few products and sums
@@ -84,8 +84,8 @@ enum : int { MR_E0 = 0, MR_X0 = 1, MR_E1 = 2, MR_X1 = 3, MR_CQ0 = 4, MR_OM = 5, 
 //   G [2][nv][ldG]  weight * f, later F [2][nn] (unpadded)
 //   Hx, Hy [16][sH]   Hermite bases
 //   Tm [2][16][sH]  first contraction; R [2][16][sR] reuses it
-//   D [2][16][sD]
-//   TX, TY [MF_REC][NP] (doubles, 16-byte aligned)
+//   D [2][16][sD]    in the G region when it fits there
+//   TX, TY [MF_REC][nv] (doubles, 16-byte aligned)
 struct MfLayout {
     int np, sH, sR, sD, ldG;
     int oHx, oHy, oTm, oD, endT;
@@ -101,14 +101,18 @@ __host__ __device__ inline MfLayout mf_layout(int nv, int tsize) {
     L.oHx = 2 * nv * L.ldG;
     L.oHy = L.oHx + MF_NB * L.sH;
     L.oTm = L.oHy + MF_NB * L.sH;
-    L.oD = L.oTm + 2 * MF_NB * (L.sH > L.sR ? L.sH : L.sR);
-    const int end = L.oD + 2 * MF_NB * L.sD;
+    const int endTm = L.oTm + 2 * MF_NB * (L.sH > L.sR ? L.sH : L.sR);
+    // D lives in the G region when it fits: G is dead once Tm is formed, F
+    // is written there only after the last read of D
+    const bool d_in_g = 2 * nv * L.ldG >= 2 * MF_NB * L.sD;
+    L.oD = d_in_g ? 0 : endTm;
+    const int end = d_in_g ? endTm : endTm + 2 * MF_NB * L.sD;
     L.endT = (end * tsize + 15) / 16 * 16 / tsize;   // records start 16-byte aligned
     return L;
 }
 __host__ __device__ inline size_t mf_lds_bytes(int nv, size_t tsize) {
     const MfLayout L = mf_layout(nv, (int)tsize);
-    return (size_t)L.endT * tsize + 2 * MF_REC * L.np * sizeof(double);
+    return (size_t)L.endT * tsize + 2 * MF_REC * (size_t)nv * sizeof(double);
 }
 
 // Per-cell scalars of the update, one lane per cell, computed once instead
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
     T* const R = Tm;       // Tm is dead once D is formed
     T* const F = G;        // G is dead once Tm is formed
     double* const TX = smem_raw + ly.endT * (int)sizeof(T) / 8;
-    double* const TY = TX + MF_REC * np;
+    double* const TY = TX + MF_REC * nv;
 
     // ---- per-axis records and the Hermite bases He_0..He_15 at
     // xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt) (build_probabilists_hermite,
@@ -252,13 +256,13 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
             const double co = xv - (isx ? po[1] : po[2]), cn = xv - (isx ? pn[1] : pn[2]);
             double* rec = (isx ? TX : TY) + idx;
             const double co2 = co * co, cn2 = cn * cn;
-            rec[MR_E0 * np] = (isx ? P[CP_HCO] : 1.0) * exp(-po[3] * co2);
-            rec[MR_X0 * np] = 2.0 * po[3] * co2 + (isx ? c.K - 5.0 : 0.0);
-            rec[MR_E1 * np] = (isx ? P[CP_HCN] : 1.0) * exp(-pn[3] * cn2);
-            rec[MR_X1 * np] = 2.0 * pn[3] * cn2 + (isx ? c.K - 5.0 : 0.0);
-            rec[MR_CQ0 * np] = co;
+            rec[MR_E0 * nv] = (isx ? P[CP_HCO] : 1.0) * exp(-po[3] * co2);
+            rec[MR_X0 * nv] = 2.0 * po[3] * co2 + (isx ? c.K - 5.0 : 0.0);
+            rec[MR_E1 * nv] = (isx ? P[CP_HCN] : 1.0) * exp(-pn[3] * cn2);
+            rec[MR_X1 * nv] = 2.0 * pn[3] * cn2 + (isx ? c.K - 5.0 : 0.0);
+            rec[MR_CQ0 * nv] = co;
             const A x = (A)(co / sqrt_rt);
-            rec[MR_OM * np] = (isx ? om_scale : 1.0) * (double)fexp((A)-0.5 * x * x);
+            rec[MR_OM * nv] = (isx ? om_scale : 1.0) * (double)fexp((A)-0.5 * x * x);
             A p0 = (A)1, p1 = x;
             Hb[idx] = (T)p0;
             Hb[sH + idx] = (T)p1;
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
             rb[e] = b[o];
             const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
             const double wt = wnode_d[k];
-            const double cu = TX[MR_CQ0 * np + iu], cv = TY[MR_CQ0 * np + jv];
+            const double cu = TX[MR_CQ0 * nv + iu], cv = TY[MR_CQ0 * nv + jv];
             const double c2 = cu * cu + cv * cv;
             const double wh = wt * (double)rh[e], wb = wt * (double)rb[e];
             q4[0] += cu * c2 * wh;
@@ -312,8 +316,8 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
         double* rec = (isx ? TX : TY) + idx;
         const double qa = isx ? qf0 : qf1;
         const double cn = (isx ? u1[idx] : v1[idx]) - (isx ? pn[1] : pn[2]);
-        rec[MR_CQ0 * np] = P[CP_CSO] * qa * rec[MR_CQ0 * np];
-        rec[MR_Q1 * np] = P[CP_CSN] * qa * cn;
+        rec[MR_CQ0 * nv] = P[CP_CSO] * qa * rec[MR_CQ0 * nv];
+        rec[MR_Q1 * nv] = P[CP_CSN] * qa * cn;
     }
 
     const int nt = np / 16;             // tiles along a velocity index
@@ -419,15 +423,15 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
             const int jv = (int)(((float)k + 0.5f) * rnv), iu = k - jv * nv;
             const double* rx = TX + iu;
             const double* ry = TY + jv;
-            const double Ho = rx[MR_E0 * np] * ry[MR_E0 * np];
-            const double cqo = rx[MR_CQ0 * np] + ry[MR_CQ0 * np];
-            const double Hpo = fma(Ho, cqo * (rx[MR_X0 * np] + ry[MR_X0 * np]), Ho);
+            const double Ho = rx[MR_E0 * nv] * ry[MR_E0 * nv];
+            const double cqo = rx[MR_CQ0 * nv] + ry[MR_CQ0 * nv];
+            const double Hpo = fma(Ho, cqo * (rx[MR_X0 * nv] + ry[MR_X0 * nv]), Ho);
             const double Bpo = kfo * fma(2.0 * cqo, Ho, Hpo);
-            const double Hn = rx[MR_E1 * np] * ry[MR_E1 * np];
-            const double cqn = rx[MR_Q1 * np] + ry[MR_Q1 * np];
-            const double Hpn = fma(Hn, cqn * (rx[MR_X1 * np] + ry[MR_X1 * np]), Hn);
+            const double Hn = rx[MR_E1 * nv] * ry[MR_E1 * nv];
+            const double cqn = rx[MR_Q1 * nv] + ry[MR_Q1 * nv];
+            const double Hpn = fma(Hn, cqn * (rx[MR_X1 * nv] + ry[MR_X1 * nv]), Hn);
             const double Bpn = kfn * fma(2.0 * cqn, Hn, Hpn);
-            const A om = (A)(rx[MR_OM * np] * ry[MR_OM * np]);
+            const A om = (A)(rx[MR_OM * nv] * ry[MR_OM * nv]);
             const double gh = (double)(om * (A)F[k]), gb = (double)(om * (A)F[nn + k]);
             const double dfh = (double)(((A)fh_l[k] - (A)fh_r[k] + (A)fh_d[k] - (A)fh_u[k]) * iarea);
             const double dfb = (double)(((A)fb_l[k] - (A)fb_r[k] + (A)fb_d[k] - (A)fb_u[k]) * iarea);
