@@ -287,6 +287,24 @@ __device__ __forceinline__ A vl_slope(A fl, A fn, A fr, bool inner, A inv_den, A
     return (fr - fl) * inv_den;
 }
 
+// ------------------------------------------------------------ buffer loads
+// Raw buffer loads: a scalar resource (base, byte size) and a 32-bit byte
+// offset per lane, so a load needs no per-lane 64-bit address arithmetic;
+// the hardware returns zero for offsets past the size.  Word 3 of the
+// resource is the gfx9-family raw-buffer format (32-bit data).
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             BUF_RSRC_WORD3);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
 template <typename T> struct Arith;           // per-node arithmetic type
 template <> struct Arith<double> { using A = double; };
 template <> struct Arith<float> { using A = float; };

@@ -118,18 +118,33 @@ struct Upwind {
     using A = typename Arith<T>::A;
     const T *h, *b, *sN, *sT, *bN, *bT;
     size_t baseL, baseR;
+    uint32_t rowL, rowR;   // byte offsets of the two cells' rows (arrays under 4 GiB)
+    bool small;            // every distribution array is under 4 GiB
+    __amdgpu_buffer_rsrc_t rh, rb, rsN, rsT, rbN, rbT;   // the six arrays, when small
     Stencil S;
     A idN, idT, smv;
     __device__ __forceinline__ void get(int d, int k, A& hu, A& bu, A& sn, A& st, A& bn,
                                         A& bt) const {
         if constexpr (!ST) {
-            const size_t o = (d ? baseL : baseR) + k;
-            hu = (A)h[o];
-            bu = (A)b[o];
-            sn = (A)sN[o];
-            bn = (A)bN[o];
-            st = (A)sT[o];
-            bt = (A)bT[o];
+            if (small) {
+                // one 32-bit byte offset for the six arrays: loads from a
+                // uniform base with no per-lane 64-bit address arithmetic
+                const uint32_t ob = (d ? rowL : rowR) + (uint32_t)k * (uint32_t)sizeof(T);
+                hu = (A)buf_ld<T>(rh, ob);
+                bu = (A)buf_ld<T>(rb, ob);
+                sn = (A)buf_ld<T>(rsN, ob);
+                bn = (A)buf_ld<T>(rbN, ob);
+                st = (A)buf_ld<T>(rsT, ob);
+                bt = (A)buf_ld<T>(rbT, ob);
+            } else {
+                const size_t o = (d ? baseL : baseR) + k;
+                hu = (A)h[o];
+                bu = (A)b[o];
+                sn = (A)sN[o];
+                bn = (A)bN[o];
+                st = (A)sT[o];
+                bt = (A)bT[o];
+            }
         } else {
             const uint32_t kb = (uint32_t)k * (uint32_t)sizeof(T);
             const uint32_t ou = (d ? S.u[1] : S.u[0]) + kb;
@@ -162,6 +177,20 @@ __device__ __forceinline__ Upwind<T, ST> make_upwind(const Consts& c, const Face
     U.bT = fg.dir == 0 ? sby : sbx;
     U.baseL = (size_t)fg.cL * c.ldv;
     U.baseR = (size_t)fg.cR * c.ldv;
+    U.small = (size_t)c.nx * c.ny * c.ldv * sizeof(T) < ((size_t)1 << 32);
+    U.rowL = (uint32_t)(U.baseL * sizeof(T));
+    U.rowR = (uint32_t)(U.baseR * sizeof(T));
+    if constexpr (!ST) {
+        if (U.small) {
+            const uint32_t bytes = (uint32_t)((size_t)c.nx * c.ny * c.ldv * sizeof(T));
+            U.rh = buf_rsrc(U.h, bytes);
+            U.rb = buf_rsrc(U.b, bytes);
+            U.rsN = buf_rsrc(U.sN, bytes);
+            U.rsT = buf_rsrc(U.sT, bytes);
+            U.rbN = buf_rsrc(U.bN, bytes);
+            U.rbT = buf_rsrc(U.bT, bytes);
+        }
+    }
     if constexpr (ST) U.S = face_stencil<T>(c, fg);
     const A lenN = (A)(fg.dir == 0 ? c.dx : c.dy), lenT = (A)(fg.dir == 0 ? c.dy : c.dx);
     U.idN = (A)1 / ((A)0.5 * lenN + (A)0.5 * lenN);

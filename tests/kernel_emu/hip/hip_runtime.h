@@ -33,6 +33,24 @@
 // wave-uniform values only: the emulation has no scalar register file
 #define __builtin_amdgcn_readfirstlane(x) (x)
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+// raw buffer loads: base and byte size; offsets past the size read zero
+struct __amdgpu_buffer_rsrc_t {
+    const char* base;
+    uint32_t bytes;
+};
+inline __amdgpu_buffer_rsrc_t __builtin_amdgcn_make_buffer_rsrc(void* p, short, int bytes, int) {
+    return {static_cast<const char*>(p), (uint32_t)bytes};
+}
+inline uint32_t __builtin_amdgcn_raw_buffer_load_b32(__amdgpu_buffer_rsrc_t r, int off, int, int) {
+    uint32_t v = 0;
+    if ((uint64_t)(uint32_t)off + 4 <= r.bytes) memcpy(&v, r.base + (uint32_t)off, 4);
+    return v;
+}
+inline uint64_t __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t r, int off, int, int) {
+    uint64_t v = 0;
+    if ((uint64_t)(uint32_t)off + 8 <= r.bytes) memcpy(&v, r.base + (uint32_t)off, 8);
+    return v;
+}
 
 struct dim3 {
     unsigned x, y, z;
