@@ -95,7 +95,9 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
                                           inv_den[dir], smv);
                 }
                 T* out = dir == 0 ? (q == 0 ? shx : sbx) : (q == 0 ? shy : sby);
-                *reinterpret_cast<Vec*>(out + o + k) = so;
+                // streamed out: read back only by the face kernels
+#pragma unroll
+                for (int e = 0; e < VW; ++e) __builtin_nontemporal_store(so.v[e], out + o + k + e);
             }
         }
     }

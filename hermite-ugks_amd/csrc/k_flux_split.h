@@ -759,8 +759,9 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
         xb = xb - tm2fn * vn * gb + mt6 * fnA * vn * bn;
         xh = xh - tm2ft * vn * gbh + mt6 * ftA * vn * st;
         xb = xb - tm2ft * vn * gbb + mt6 * ftA * vn * bt;
-        ofh[k] = (T)(flenA * xh);
-        ofb[k] = (T)(flenA * xb);
+        // streamed out: read back only by the cell update
+        __builtin_nontemporal_store((T)(flenA * xh), ofh + k);
+        __builtin_nontemporal_store((T)(flenA * xb), ofb + k);
     }
 }
 

@@ -438,8 +438,9 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
             const double ho = (double)rh[e], bo = (double)rb[e];
             const double hn = (ho + dfh + a_n * Hpn + a_o * (Hpo - ho) + gh) * iden;
             const double bn = (bo + dfb + a_n * Bpn + a_o * (Bpo - bo) + gb) * iden;
-            ho_p[k] = (T)hn;
-            bo_p[k] = (T)bn;
+            // streamed out: the next step reads them long after
+            __builtin_nontemporal_store((T)hn, ho_p + k);
+            __builtin_nontemporal_store((T)bn, bo_p + k);
         }
     }
 }

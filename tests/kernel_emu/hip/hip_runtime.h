@@ -33,6 +33,8 @@
 // wave-uniform values only: the emulation has no scalar register file
 #define __builtin_amdgcn_readfirstlane(x) (x)
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+// cache hints have no meaning on the CPU
+#define __builtin_nontemporal_store(v, p) (*(p) = (v))
 // raw buffer loads: base and byte size; offsets past the size read zero
 struct __amdgpu_buffer_rsrc_t {
     const char* base;
