@@ -6,7 +6,8 @@
 ! The input is the reference's &params namelist (2d/src/config.f90:47-48,
 ! same names and defaults) optionally followed by a &gpu group
 ! (precision = 'fp64' | 'mixed'; max_steps, 0 = no cap; rt_variant =
-! 'reference' | 'cartesian'; seed).  The driver keeps what the reference
+! 'reference' | 'cartesian'; seed; restart_file, checkpoint_file: continue
+! from / save the whole solver state, hugks_checkpoint_read/_write).  The driver keeps what the reference
 ! main program does around the step - case selection, velocity
 ! quadrature, initial condition, body force, the loop exits and output
 ! times, macro<iter>.dat files and the final mass report
@@ -37,7 +38,8 @@ module ugks2d_setup
   integer :: max_steps = 0
   character(len=16) :: rt_variant = 'reference'
   integer :: seed = 0
-  namelist /gpu/ precision, max_steps, rt_variant, seed
+  character(len=256) :: restart_file = '', checkpoint_file = ''
+  namelist /gpu/ precision, max_steps, rt_variant, seed, restart_file, checkpoint_file
 
   ! constants of the reference (2d/src/data2d.f90:9-11,54)
   real(8), parameter :: PI = 3.1415926535d0
@@ -387,6 +389,8 @@ program ugks2d_gpu
   real(8), allocatable, target :: w(:, :, :)
   real(8) :: mass_initial, mass_final, max_u_axis, max_v_axis, prim(4), t_next
   integer :: iter, i, j, k, taken, cap, chunk, stop_reason
+  real(c_double) :: t_restart = 0.d0
+  integer(c_int32_t) :: it_restart
   logical :: wrote(3)
   integer(8) :: c0, c1, rate
 
@@ -400,7 +404,16 @@ program ugks2d_gpu
   call build_case(desc)
   call hugks_check(hugks_create(desc, h), c_null_ptr, 'create')
   iter = 1
-  call write_macro(h, iter)
+  if (len_trim(restart_file) > 0) then
+    ! continue a saved run: its iteration index and time; the initial mass
+    ! below is then the mass at the restart point
+    call hugks_check(hugks_checkpoint_read(h, restart_file), h, 'restart')
+    call hugks_check(hugks_get_time(h, t_restart, it_restart), h, 'get_time')
+    iter = it_restart
+    write(*, *) "restarted from ", trim(restart_file), " at iteration ", iter, " time ", t_restart
+  else
+    call write_macro(h, iter)
+  end if
   allocate(w(4, nx, ny))
   call hugks_check(hugks_get_macro(h, c_loc(w), c_null_ptr), h, 'get_macro')
   mass_initial = 0.d0
@@ -414,6 +427,12 @@ program ugks2d_gpu
   ! next output point, the step cap or a loop exit
   write(*, *) "begin main loop"
   wrote = .false.
+  if (len_trim(restart_file) > 0 .and. case_id == 6) then
+    ! output times already passed in the saved run
+    wrote(1) = t_restart >= time2
+    wrote(2) = t_restart >= time3
+    wrote(3) = t_restart >= time4
+  end if
   taken = 0
   stop_reason = HUGKS_STOP_NONE
   do
@@ -490,5 +509,9 @@ program ugks2d_gpu
   write(*, '(A,ES24.16)') 'max_abs_v_bottom_axis= ', max_v_axis
   call system_clock(c1)
   write(*, *) "Elapsed time: ", real(c1 - c0) / real(rate), " seconds"
+  if (len_trim(checkpoint_file) > 0) then
+    call hugks_check(hugks_checkpoint_write(h, checkpoint_file), h, 'checkpoint')
+    write(*, *) "state saved to ", trim(checkpoint_file)
+  end if
   call hugks_check(hugks_destroy(h), h, 'destroy')
 end program ugks2d_gpu

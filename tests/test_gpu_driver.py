@@ -68,3 +68,33 @@ def test_driver_matches_reference_program(have_gpu, tmp_path, name, dim):
             scale = np.max(np.abs(b[k]))
             err = np.max(np.abs(a[k] - b[k]))
             assert err <= 1e-10 * scale + floor, (fname, k, err, scale)
+
+
+def test_driver_restart_continues_the_run(have_gpu, tmp_path):
+    """&gpu checkpoint_file / restart_file: 6 steps, save, restart for 4
+    more, writes the same final macro file as 10 steps in one run."""
+    exe = DRIVER[2]
+    base = open(os.path.join(GOLDEN, "cases", "ref2d_poiseuille_smoke.namelist")).read()
+    base = re.sub(r"&gpu.*?/", "", base, flags=re.S).replace("maxtime=0.001", "maxtime=1.0")
+
+    def run(sub, gpu):
+        d = tmp_path / sub
+        d.mkdir()
+        (d / "in.namelist").write_text(base + "\n&gpu\n" + gpu + "\n/\n")
+        r = subprocess.run([exe, "in.namelist"], cwd=d, capture_output=True, text=True, timeout=90)
+        assert r.returncode == 0, f"rc={r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
+        return d
+
+    print("driver restart", flush=True)
+    one = run("one", "max_steps = 10")
+    first = run("first", "max_steps = 6, checkpoint_file = 'state.ckpt'")
+    second = tmp_path / "second"
+    second.mkdir()
+    (second / "state.ckpt").write_bytes((first / "state.ckpt").read_bytes())
+    (second / "in.namelist").write_text(
+        base + "\n&gpu\nmax_steps = 4, restart_file = 'state.ckpt'\n/\n")
+    r = subprocess.run([exe, "in.namelist"], cwd=second, capture_output=True, text=True, timeout=90)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "restarted from" in r.stdout
+    assert (first / "macro7.dat").exists() and (one / "macro11.dat").exists()
+    assert (second / "macro11.dat").read_text() == (one / "macro11.dat").read_text()
