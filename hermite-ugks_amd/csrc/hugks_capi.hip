@@ -892,6 +892,9 @@ int hugks_tune(const hugks_desc* d, int32_t* flux_kernel, int32_t* update_kernel
 
 struct hugks_batch {
     std::vector<hugks_handle*> cases;
+    bool grouped = false;       // one grouped launch per kernel type (k_group.h)
+    GroupPlan plan{};
+    void* gmem = nullptr;       // tables and per-case arguments of the grouped step
     hipStream_t main = nullptr;
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> joins;
@@ -910,6 +913,7 @@ static void batch_release(hugks_batch* b) {
     if (b->fork) hipEventDestroy(b->fork);
     for (hipStream_t s : b->lanes) hipStreamDestroy(s);
     if (b->main) hipStreamDestroy(b->main);
+    if (b->gmem) hipFree(b->gmem);
     for (hugks_handle* h : b->cases) hugks_destroy(h);
     delete b;
 }
@@ -921,6 +925,116 @@ static int batch_fail(hugks_batch* b, int code, const std::string& msg) {
 }
 
 extern "C" {
+
+// A batch runs grouped when every case is a small 2D case of one storage
+// type whose step is the split flux with factoring weights and the MFMA
+// update: then one launch per kernel type covers all cases.
+static bool batch_groupable(const hugks_batch* b) {
+    if (const char* g = getenv("HUGKS_BATCH_GROUPED"))
+        if (strcmp(g, "0") == 0) return false;
+    for (const hugks_handle* h : b->cases) {
+        const Launch2D& L = h->L;
+        if (h->dim != 2 || L.fp64 != b->cases[0]->L.fp64 || !L.wx ||
+            !update2d_mfma_supported(L.c) || L.c.nx * L.c.ny >= PRED_MIN_CELLS)
+            return false;
+    }
+    return true;
+}
+
+// Tables of the grouped launches ([n, start[0..n], case[0..n-1]] each) and
+// the per-case arguments, in one device allocation.
+static hipError_t batch_group_plan(hugks_batch* b) {
+    const int n = (int)b->cases.size();
+    std::vector<std::vector<int>> tabs;
+    std::vector<int> blocks;
+    auto table = [&](const std::vector<std::pair<int, int>>& cs) {   // (case, workgroups)
+        std::vector<int> t(1 + cs.size() + 1 + cs.size());
+        t[0] = (int)cs.size();
+        int acc = 0;
+        for (size_t e = 0; e < cs.size(); ++e) {
+            t[1 + e] = acc;
+            acc += cs[e].second;
+            t[2 + cs.size() + e] = cs[e].first;
+        }
+        t[1 + cs.size()] = acc;
+        tabs.push_back(t);
+        blocks.push_back(acc);
+        return (int)tabs.size() - 1;
+    };
+    std::vector<std::pair<int, int>> one, sl, fs, ff, fn, wl, cp;
+    std::map<std::tuple<int, int, int>, std::vector<std::pair<int, int>>> upd;
+    std::map<std::tuple<int, int, int>, size_t> upd_lds;
+    for (int i = 0; i < n; ++i) {
+        hugks_handle* h = b->cases[i];
+        hugks_set_schedule(h, HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA);
+        int s1, s2, s3, s4, s5, s6, e, t, v;
+        group_blocks(h->L, &s1, &s2, &s3, &s4, &s5, &s6);
+        one.push_back({i, 1});
+        sl.push_back({i, s1});
+        fs.push_back({i, s2});
+        ff.push_back({i, s3});
+        fn.push_back({i, s4});
+        if (s5 > 0) wl.push_back({i, s5});
+        cp.push_back({i, s6});
+        group_update_key(h->L, &e, &t, &v);
+        const auto key = std::make_tuple(e, t, v);
+        upd[key].push_back({i, h->L.c.nx * h->L.c.ny});
+        const size_t lds = update2d_mfma_lds_bytes(h->L.c, h->fp64);
+        upd_lds[key] = std::max(upd_lds[key], lds);
+    }
+    if (upd.size() > 16) return hipErrorInvalidValue;
+    GroupPlan& P = b->plan;
+    P.fp64 = b->cases[0]->fp64;
+    const int k_ts = table(one), k_sl = table(sl), k_fs = table(fs), k_ff = table(ff),
+              k_fn = table(fn), k_wl = wl.empty() ? -1 : table(wl), k_cp = table(cp);
+    std::vector<int> k_up;
+    for (auto& kv : upd) k_up.push_back(table(kv.second));
+    // one allocation: per-case Launch2D, then the tables
+    size_t ints = 0;
+    std::vector<size_t> off;
+    for (auto& t : tabs) {
+        off.push_back(ints);
+        ints += t.size();
+    }
+    const size_t lbytes = (n * sizeof(Launch2D) + 255) / 256 * 256;
+    hipError_t e = hipMalloc(&b->gmem, lbytes + ints * sizeof(int));
+    if (e != hipSuccess) return e;
+    std::vector<Launch2D> Ls(n);
+    for (int i = 0; i < n; ++i) Ls[i] = b->cases[i]->L;
+    std::vector<int> flat;
+    for (auto& t : tabs) flat.insert(flat.end(), t.begin(), t.end());
+    e = hipMemcpy(b->gmem, Ls.data(), n * sizeof(Launch2D), hipMemcpyHostToDevice);
+    char* tb = static_cast<char*>(b->gmem) + lbytes;
+    if (e == hipSuccess)
+        e = hipMemcpy(tb, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    auto tp = [&](int k) { return reinterpret_cast<const int*>(tb) + off[k]; };
+    P.Ls = static_cast<const Launch2D*>(b->gmem);
+    P.t_ts = P.t_fin = tp(k_ts);
+    P.g_ts = P.g_fin = blocks[k_ts];
+    P.t_slope = tp(k_sl); P.g_slope = blocks[k_sl];
+    P.t_fsum = tp(k_fs); P.g_fsum = blocks[k_fs];
+    P.t_fface = tp(k_ff); P.g_fface = blocks[k_ff];
+    P.t_fnode = tp(k_fn); P.g_fnode = blocks[k_fn];
+    P.t_wall = k_wl >= 0 ? tp(k_wl) : nullptr;
+    P.g_wall = k_wl >= 0 ? blocks[k_wl] : 0;
+    P.t_cpre = tp(k_cp); P.g_cpre = blocks[k_cp];
+    P.nupd = 0;
+    int q = 0;
+    for (auto& kv : upd) {
+        GroupUpd& u = P.upd[P.nupd++];
+        u.ept = std::get<0>(kv.first);
+        u.nt = std::get<1>(kv.first);
+        u.nvc = std::get<2>(kv.first);
+        u.blocks = blocks[k_up[q]];
+        u.tab = tp(k_up[q]);
+        u.lds = upd_lds[kv.first];
+        ++q;
+        e = prepare_group_update(P.fp64, u);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_batch** out) {
     if (n <= 0 || !descs || !out) return batch_fail(nullptr, HUGKS_E_ARG, "empty batch or null pointer");
@@ -964,14 +1078,21 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     // cases finish their own set-up on their streams first
     for (hugks_handle* h : b->cases)
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    b->grouped = e == hipSuccess && batch_groupable(b);
+    if (b->grouped) e = batch_group_plan(b);
     if (e == hipSuccess) e = hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) e = hipEventRecord(b->fork, b->main);
-    for (int l = 0; l < nl && e == hipSuccess; ++l) e = hipStreamWaitEvent(b->lanes[l], b->fork, 0);
-    for (int32_t i = 0; i < n && e == hipSuccess; ++i)
-        e = enqueue_step_on(b->cases[i], b->lanes[i % nl], false);
-    for (int l = 0; l < nl && e == hipSuccess; ++l) {
-        e = hipEventRecord(b->joins[l], b->lanes[l]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->joins[l], 0);
+    if (b->grouped) {
+        if (e == hipSuccess) e = launch_group_step(b->plan, b->main);
+    } else {
+        if (e == hipSuccess) e = hipEventRecord(b->fork, b->main);
+        for (int l = 0; l < nl && e == hipSuccess; ++l)
+            e = hipStreamWaitEvent(b->lanes[l], b->fork, 0);
+        for (int32_t i = 0; i < n && e == hipSuccess; ++i)
+            e = enqueue_step_on(b->cases[i], b->lanes[i % nl], false);
+        for (int l = 0; l < nl && e == hipSuccess; ++l) {
+            e = hipEventRecord(b->joins[l], b->lanes[l]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->joins[l], 0);
+        }
     }
     if (e == hipSuccess) {
         e = hipStreamEndCapture(b->main, &b->graph);
@@ -1000,6 +1121,8 @@ const char* hugks_batch_last_error(const hugks_batch* b) {
 }
 
 int32_t hugks_batch_size(const hugks_batch* b) { return b ? (int32_t)b->cases.size() : 0; }
+
+int32_t hugks_batch_grouped(const hugks_batch* b) { return b && b->grouped ? 1 : 0; }
 
 int hugks_batch_handle(hugks_batch* b, int32_t i, hugks_handle** h) {
     if (!b || !h || i < 0 || i >= (int32_t)b->cases.size())

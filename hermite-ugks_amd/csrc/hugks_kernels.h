@@ -46,6 +46,9 @@ struct Launch2D {
 constexpr int FACE_SUM_LD = 40;
 constexpr int FACE_COEF_LD = 40;
 constexpr int FACE_MAXW_LD = 16;
+// Grids below this many cells reduce over their cells in one workgroup (one
+// launch); larger ones in two passes.
+constexpr int PRED_MIN_CELLS = 16384;
 // Doubles of the cell-reduction partials (k_misc.h: PRED_NB x (1 + PRED_LD)).
 constexpr int RED_SCRATCH = 1024 * 10;
 
@@ -73,6 +76,31 @@ hipError_t launch_convert(const void* src, bool src_fp64, double* dst, size_t nc
                           int ldv, bool to_dense, hipStream_t s);
 hipError_t launch_convert_in(const double* src, void* dst, bool dst_fp64, size_t ncell, int nn,
                              int ldv, hipStream_t s);
+
+// Grouped launches of a batch (k_group.h): device tables and grid sizes of
+// one batch step; updates are grouped by kernel instance.
+struct GroupUpd {
+    int ept, nt, nvc;    // the update instance (UpdKey)
+    int blocks;
+    size_t lds;          // largest LDS of the instance's cases
+    const int* tab;
+};
+struct GroupPlan {
+    bool fp64;
+    const Launch2D* Ls;  // per-case arguments, device memory
+    const int *t_ts, *t_slope, *t_fsum, *t_fface, *t_fnode, *t_wall, *t_cpre, *t_fin;
+    int g_ts, g_slope, g_fsum, g_fface, g_fnode, g_wall, g_cpre, g_fin;
+    GroupUpd upd[16];
+    int nupd;
+};
+hipError_t launch_group_step(const GroupPlan& P, hipStream_t s);
+hipError_t prepare_group_update(bool fp64, const GroupUpd& u);
+// per-case workgroup counts of the grouped kernels (slope, fsum, fface,
+// fnode, wall, cell_pre; time step and finalisation take one each)
+void group_blocks(const Launch2D& L, int* slope, int* fsum, int* fface, int* fnode, int* wall,
+                  int* cpre);
+// update instance of a case as (ept, nt, nvc)
+void group_update_key(const Launch2D& L, int* ept, int* nt, int* nvc);
 
 // 1D path
 hipError_t launch_step1d(const Launch2D& L, hipStream_t s);

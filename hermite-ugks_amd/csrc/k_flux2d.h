@@ -36,11 +36,11 @@ constexpr int SLOPE_CPW = 4;   // cells (wavefronts) per workgroup of k_slope2d
 // copy of 2d/src/boundary.f90:16-53); wall directions use one-sided slopes
 // in the two boundary cells.
 template <typename T>
-__global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __restrict__ st,
+__device__ __forceinline__ void k_slope2d_body(Consts c, const DevState* __restrict__ st,
                                                  const T* __restrict__ h, const T* __restrict__ b,
                                                  T* __restrict__ shx, T* __restrict__ shy,
                                                  T* __restrict__ sbx, T* __restrict__ sby,
-                                                 int wall_only) {
+                                                 int wall_only, int bid, int nblk) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
     // one wavefront per cell, four cells per workgroup: every cell in
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
     // slopes feed the wall fluxes)
     const int lane = threadIdx.x & (WAVE - 1);
     const int L = __builtin_amdgcn_readfirstlane(
-        (int)(wall_only ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * SLOPE_CPW +
+        (int)(wall_only ? bid : xcd_block(bid, nblk)) * SLOPE_CPW +
         (int)(threadIdx.x / WAVE));
     if (L >= (wall_only ? wall_cell_count(c) : c.nx * c.ny)) return;
     const int cell = wall_only ? wall_cell(c, L) : tiled_cell(L, c.nx, c.ny);
@@ -102,6 +102,14 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
         }
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __restrict__ st,
+                                                 const T* __restrict__ h, const T* __restrict__ b,
+                                                 T* __restrict__ shx, T* __restrict__ shy,
+                                                 T* __restrict__ sbx, T* __restrict__ sby,
+                                                 int wall_only) {
+    k_slope2d_body<T>(c, st, h, b, shx, shy, sbx, sby, wall_only, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // -------------------------------------------------------------- wall faces
 // One wavefront per wall face (four per workgroup).  Faces are numbered over
@@ -111,17 +119,17 @@ __global__ __launch_bounds__(256) void k_slope2d(Consts c, const DevState* __res
 // _horizon, 2d/src/flux.f90:551-689.  The incoming half-range values are
 // re-read in each pass rather than held across the reductions.
 template <typename T>
-__global__ __launch_bounds__(256) void k_flux_wall(
+__device__ __forceinline__ void k_flux_wall_body(
     Consts c, DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
     const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
     const typename Arith<T>::A* __restrict__ vnode, const typename Arith<T>::A* __restrict__ wnode,
     T* __restrict__ vfh, T* __restrict__ vfb, double* __restrict__ vflux, T* __restrict__ hfh,
-    T* __restrict__ hfb, double* __restrict__ hflux, int nwall) {
+    T* __restrict__ hfb, double* __restrict__ hflux, int nwall, int bid, int nblk) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
     const int lane = threadIdx.x & (WAVE - 1);
-    int f = blockIdx.x * 4 + (int)(threadIdx.x / WAVE);
+    int f = bid * 4 + (int)(threadIdx.x / WAVE);
     if (f >= nwall) return;
     int side = -1, idx = 0;
     for (int s = 0; s < 4; ++s) {
@@ -229,6 +237,16 @@ __global__ __launch_bounds__(256) void k_flux_wall(
         out[2] = sc * g2;
         out[3] = sc * (0.5 * s[3]);
     }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_flux_wall(
+    Consts c, DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
+    const typename Arith<T>::A* __restrict__ vnode, const typename Arith<T>::A* __restrict__ wnode,
+    T* __restrict__ vfh, T* __restrict__ vfb, double* __restrict__ vflux, T* __restrict__ hfh,
+    T* __restrict__ hfb, double* __restrict__ hflux, int nwall) {
+    k_flux_wall_body<T>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, wnode, vfh, vfb, vflux, hfh, hfb, hflux, nwall, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace hugks

@@ -278,17 +278,17 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
 }
 
 template <typename T, bool ST>
-__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
+__device__ __forceinline__ void k_fsum_body(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
     const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
     const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ und,
     const double* __restrict__ vnd, const double* __restrict__ wnd,
     const double* __restrict__ acc_v, const double* __restrict__ acc_h,
-    double* __restrict__ fsum, int nvi, int nint) {
+    double* __restrict__ fsum, int nvi, int nint, int bid, int nblk) {
     if (st->stopped) return;
     const int g = __builtin_amdgcn_readfirstlane(
-        (int)xcd_block(blockIdx.x, gridDim.x) * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+        (int)xcd_block(bid, nblk) * SPLIT_FPW + (int)(threadIdx.x / WAVE));
     if (g >= nint) return;
     const FaceGeo fg = face_geo(c, g, nvi);
     const double dt = st->dt;
@@ -302,6 +302,17 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
         fsum_face<1, T, ST>(c, vshift, up, unode, vnode, und, vnd, wnd,
                             fsum + (size_t)g * FSUM_LD);
     }
+}
+template <typename T, bool ST>
+__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
+    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ und,
+    const double* __restrict__ vnd, const double* __restrict__ wnd,
+    const double* __restrict__ acc_v, const double* __restrict__ acc_h,
+    double* __restrict__ fsum, int nvi, int nint) {
+    k_fsum_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd, acc_v, acc_h, fsum, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ------------------------------------------------------------------ K2
@@ -399,6 +410,7 @@ __device__ __forceinline__ void fstate_lane(const Consts& c, double dt,
     for (int q = 1; q <= 6; ++q) fc[FC_MT + q - 1] = Mt[q];
 }
 
+#ifndef HUGKS_BODIES_ONLY
 __global__ __launch_bounds__(256) void k_fstate(Consts c, const DevState* __restrict__ st,
                                                 const double* __restrict__ W,
                                                 const double* __restrict__ acc_v,
@@ -410,6 +422,7 @@ __global__ __launch_bounds__(256) void k_fstate(Consts c, const DevState* __rest
     if (g >= nint) return;
     fstate_lane(c, st->dt, W, acc_v, acc_h, fsum, fcoef, face_geo(c, g, nvi), g);
 }
+#endif
 
 // ------------------------------------------------------------------ K3
 template <int DIR, typename T>
@@ -649,6 +662,7 @@ __device__ __forceinline__ void fslope_t_lane(const Consts& c, const double* __r
     out[3] = flen * out4[3];
 }
 
+#ifndef HUGKS_BODIES_ONLY
 __global__ __launch_bounds__(256) void k_fslope_t(Consts c, const DevState* __restrict__ st,
                                                   const double* __restrict__ acc_v,
                                                   const double* __restrict__ acc_h,
@@ -662,9 +676,31 @@ __global__ __launch_bounds__(256) void k_fslope_t(Consts c, const DevState* __re
     fslope_t_lane(c, acc_v, acc_h, fmaxw + (size_t)g * FMAXW_LD, fcoef, vflux, hflux,
                   face_geo(c, g, nvi), g);
 }
+#endif
 
 // K2, K3 (tensor-product quadrature) and K4 of one face on one lane, the
 // Maxwellian sums in registers: one launch instead of three.
+__device__ __forceinline__ void k_fface_body(Consts c, const DevState* __restrict__ st,
+                                               const double* __restrict__ W,
+                                               const double* __restrict__ u1,
+                                               const double* __restrict__ v1,
+                                               const double* __restrict__ wx,
+                                               const double* __restrict__ wy,
+                                               const double* __restrict__ acc_v,
+                                               const double* __restrict__ acc_h,
+                                               const double* __restrict__ fsum, double* fcoef,
+                                               double* __restrict__ vflux,
+                                               double* __restrict__ hflux, int nvi, int nint, int bid, int nblk) {
+    if (st->stopped) return;
+    const int g = bid * 256 + threadIdx.x;
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    fstate_lane(c, st->dt, W, acc_v, acc_h, fsum, fcoef, fg, g);
+    double s2[FMAXW_N];
+    fmaxw_sep_lane(c, u1, v1, wx, wy, acc_v, acc_h, fcoef, fg, g, s2);
+    fslope_t_lane(c, acc_v, acc_h, s2, fcoef, vflux, hflux, fg, g);
+}
+#ifndef HUGKS_BODIES_ONLY
 __global__ __launch_bounds__(256) void k_fface(Consts c, const DevState* __restrict__ st,
                                                const double* __restrict__ W,
                                                const double* __restrict__ u1,
@@ -676,15 +712,9 @@ __global__ __launch_bounds__(256) void k_fface(Consts c, const DevState* __restr
                                                const double* __restrict__ fsum, double* fcoef,
                                                double* __restrict__ vflux,
                                                double* __restrict__ hflux, int nvi, int nint) {
-    if (st->stopped) return;
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= nint) return;
-    const FaceGeo fg = face_geo(c, g, nvi);
-    fstate_lane(c, st->dt, W, acc_v, acc_h, fsum, fcoef, fg, g);
-    double s2[FMAXW_N];
-    fmaxw_sep_lane(c, u1, v1, wx, wy, acc_v, acc_h, fcoef, fg, g, s2);
-    fslope_t_lane(c, acc_v, acc_h, s2, fcoef, vflux, hflux, fg, g);
+    k_fface_body(c, st, W, u1, v1, wx, wy, acc_v, acc_h, fsum, fcoef, vflux, hflux, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
 }
+#endif
 
 // ------------------------------------------------------------------ K5
 template <int DIR, typename T, bool ST>
@@ -766,16 +796,16 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
 }
 
 template <typename T, bool ST>
-__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
+__device__ __forceinline__ void k_fnode_body(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
     const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
     const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ acc_v,
     const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
-    T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint) {
+    T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint, int bid, int nblk) {
     if (st->stopped) return;
     const int g = __builtin_amdgcn_readfirstlane(
-        (int)xcd_block(blockIdx.x, gridDim.x) * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+        (int)xcd_block(bid, nblk) * SPLIT_FPW + (int)(threadIdx.x / WAVE));
     if (g >= nint) return;
     const FaceGeo fg = face_geo(c, g, nvi);
     const double dt = st->dt;
@@ -790,6 +820,16 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
         fnode_face<1, T, ST>(c, fc, fn, ft, up, unode, vnode, dt,
                              hfh + (size_t)fg.face * c.ldv, hfb + (size_t)fg.face * c.ldv);
     }
+}
+template <typename T, bool ST>
+__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
+    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ acc_v,
+    const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
+    T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint) {
+    k_fnode_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, acc_v, acc_h, fcoef, vfh, vfb, hfh, hfb, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace hugks

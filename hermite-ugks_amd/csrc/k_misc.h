@@ -33,6 +33,7 @@ __device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
 
 // Marks the start of a host call: later kernels run again even if a
 // previous call ended on an exit condition.
+#ifndef HUGKS_BODIES_ONLY
 __global__ void k_begin(DevState* st, double t_stop) {
     if (threadIdx.x == 0) {
         st->stopped = 0;
@@ -41,6 +42,7 @@ __global__ void k_begin(DevState* st, double t_stop) {
         st->t_stop = t_stop;
     }
 }
+#endif
 
 // Signal rate of one cell for the time step (timestep, 2d/src/solver.f90:
 // 13-61 and 1d/src/solver.f90:16-47).
@@ -71,6 +73,7 @@ constexpr int PRED_LD = 9;     // per-partial slots of the finalize pass
 static_assert(PRED_NB * (1 + PRED_LD) <= RED_SCRATCH, "reduction scratch");
 
 // First pass of the time step: the largest signal rate per workgroup.
+#ifndef HUGKS_BODIES_ONLY
 __global__ __launch_bounds__(PRED_NT) void k_rate_part(Consts c, int dim,
                                                        const DevState* __restrict__ st,
                                                        const double* __restrict__ W,
@@ -84,14 +87,15 @@ __global__ __launch_bounds__(PRED_NT) void k_rate_part(Consts c, int dim,
     block_max<1, PRED_NT>(t, lds);
     if (threadIdx.x == 0) red[blockIdx.x] = t[0];
 }
+#endif
 
 // timestep: dt from the largest rate, over the nred partials of
 // k_rate_part or, with nred = 0, straight over the cells (small grids: one
 // launch instead of two).  The 1D loop tests simtime >= maxtime before the
 // step (1d/src/main.f90:84), so that test is made here.
-__global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState* st,
+__device__ __forceinline__ void k_timestep_body(Consts c, int dim, DevState* st,
                                                      const double* __restrict__ W,
-                                                     const double* __restrict__ red, int nred) {
+                                                     const double* __restrict__ red, int nred, int bid, int nblk) {
     __shared__ double lds[RED_NT / WAVE];
     if (st->stopped) return;
     if (dim == 1 && st->simtime >= c.maxtime) {
@@ -117,10 +121,18 @@ __global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState
         st->dt = dt;
     }
 }
+#ifndef HUGKS_BODIES_ONLY
+__global__ __launch_bounds__(RED_NT) void k_timestep(Consts c, int dim, DevState* st,
+                                                     const double* __restrict__ W,
+                                                     const double* __restrict__ red, int nred) {
+    k_timestep_body(c, dim, st, W, red, nred, (int)blockIdx.x, (int)gridDim.x);
+}
+#endif
 
 // First pass of the finalisation: per workgroup, the sums of the per-cell
 // residual partials (fixed order) and, in 1D, the maxima of the steady
 // residuals.
+#ifndef HUGKS_BODIES_ONLY
 __global__ __launch_bounds__(PRED_NT) void k_part_sum(Consts c, int dim,
                                                       const DevState* __restrict__ st,
                                                       const double* __restrict__ part,
@@ -148,15 +160,16 @@ __global__ __launch_bounds__(PRED_NT) void k_part_sum(Consts c, int dim,
         for (int q = 0; q < 3; ++q) r[6 + q] = mx[q];
     }
 }
+#endif
 
 // Residual and loop state after update(): 2d/src/solver.f90:393 with
 // 2d/src/main.f90:82,102-104; 1d/src/solver.f90:262 with 1d/src/main.f90:
 // 106-107.  Partials per cell: 2D (dw^2 x4, |rho|); 1D (dw^2 x3, |w| x3,
 // steady residual x3), reduced by k_part_sum; one workgroup sums its
 // partials in index order.
-__global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState* st,
+__device__ __forceinline__ void k_finalize_body(Consts c, int dim, DevState* st,
                                                      const double* __restrict__ part,
-                                                     const double* __restrict__ red, int nred) {
+                                                     const double* __restrict__ red, int nred, int bid, int nblk) {
     __shared__ double lds[9 * (RED_NT / WAVE)];
     if (st->stopped) return;
     const int ncell = c.nx * c.ny;
@@ -232,9 +245,17 @@ __global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState
         }
     }
 }
+#ifndef HUGKS_BODIES_ONLY
+__global__ __launch_bounds__(RED_NT) void k_finalize(Consts c, int dim, DevState* st,
+                                                     const double* __restrict__ part,
+                                                     const double* __restrict__ red, int nred) {
+    k_finalize_body(c, dim, st, part, red, nred, (int)blockIdx.x, (int)gridDim.x);
+}
+#endif
 
 // Reduced Maxwellian initial state, 2d/src/initialization.f90:227-235
 // (1d/src/initialization.f90:185-192) with W = get_conserved(prim0).
+#ifndef HUGKS_BODIES_ONLY
 __global__ void k_init_f(Consts c, int dim, const double* __restrict__ prim0,
                          const double* __restrict__ u1, const double* __restrict__ v1,
                          double* __restrict__ W, void* h, void* b, int fp64) {
@@ -275,6 +296,7 @@ __global__ void k_init_f(Consts c, int dim, const double* __restrict__ prim0,
         ((float*)b)[idx] = (float)B;
     }
 }
+#endif
 
 // Heat flux per cell for output: get_heat_flux(h, b, uspace, vspace, prim)
 // (2d/src/gastheory2d.f90:121-129; 1d/src/gastheory.f90:98-104).

@@ -53,6 +53,38 @@ __host__ __device__ inline int mf_threads(int nn, bool fp64) {
     const int w256 = (nn + 255) / 256 * 256 - nn, w320 = (nn + 319) / 320 * 320 - nn;
     return fp64 && w320 < w256 ? 320 : 256;
 }
+
+// Compiled instances (nodes per thread, threads, nv or 0 for any nv): the
+// common Gauss-Hermite grids per storage type, then the run-time kernels.
+#define HUGKS_UPDM_F32(X) X(1, 256, 16) X(2, 256, 20) X(3, 256, 24) X(4, 256, 28) \
+    X(4, 256, 32) X(7, 256, 40)
+#define HUGKS_UPDM_F64(X) X(1, 256, 16) X(2, 256, 20) X(2, 320, 24) X(3, 320, 28) \
+    X(4, 256, 32) X(5, 320, 40)
+#define HUGKS_UPDM_ANY(X) X(1, 256, 0) X(2, 256, 0) X(3, 256, 0) X(4, 256, 0) X(5, 256, 0) \
+    X(6, 256, 0) X(7, 256, 0) X(8, 256, 0) X(1, 320, 0) X(2, 320, 0) X(3, 320, 0)        \
+    X(4, 320, 0) X(5, 320, 0) X(6, 320, 0) X(7, 320, 0)
+
+// The instance a case runs: (nodes per thread, threads, compiled-in nv).
+struct UpdKey {
+    int ept, nt, nvc;
+};
+__host__ inline UpdKey update_key(const Consts& c, bool fp64) {
+    UpdKey k;
+    k.nt = mf_threads(c.nn, fp64);
+    k.ept = (c.nn + k.nt - 1) / k.nt;
+    k.nvc = (c.nv == 16 || c.nv == 20 || c.nv == 24 || c.nv == 28 || c.nv == 32 || c.nv == 40)
+                ? c.nv : 0;
+    bool listed = false;
+#define HUGKS_UPDM_HAS(E, N, V) listed = listed || (k.ept == E && k.nt == N && k.nvc == V);
+    if (fp64) {
+        HUGKS_UPDM_F64(HUGKS_UPDM_HAS)
+    } else {
+        HUGKS_UPDM_F32(HUGKS_UPDM_HAS)
+    }
+#undef HUGKS_UPDM_HAS
+    if (!listed) k.nvc = 0;   // the run-time kernel
+    return k;
+}
 constexpr int MF_NB = 16;    // Hermite basis rows He_0..He_15
 
 // r! for r = 0..15 (exact in double)
@@ -127,15 +159,15 @@ enum : int {
     CP_IDEN = 19, CP_ITAU = 20, CP_ITAUO = 21,
 };
 
-__global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict__ st,
+__device__ __forceinline__ void k_cell_pre_body(Consts c, DevState* __restrict__ st,
                                                   double* __restrict__ W,
                                                   double* __restrict__ part,
                                                   const double* __restrict__ vflux,
                                                   const double* __restrict__ hflux,
                                                   const double* __restrict__ acc_cell,
-                                                  double* __restrict__ cp) {
+                                                  double* __restrict__ cp, int bid, int nblk) {
     if (st->stopped) return;
-    const int cell = blockIdx.x * 256 + threadIdx.x;
+    const int cell = bid * 256 + threadIdx.x;
     if (cell >= c.nx * c.ny) return;
     const int nx = c.nx;
     const int i = cell % nx, j = cell / nx;
@@ -190,6 +222,17 @@ __global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict
         !isfinite(pn[3]) || !(pn[0] > 0.0))
         atomicOr(&st->flags, 1);
 }
+#ifndef HUGKS_BODIES_ONLY
+__global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict__ st,
+                                                  double* __restrict__ W,
+                                                  double* __restrict__ part,
+                                                  const double* __restrict__ vflux,
+                                                  const double* __restrict__ hflux,
+                                                  const double* __restrict__ acc_cell,
+                                                  double* __restrict__ cp) {
+    k_cell_pre_body(c, st, W, part, vflux, hflux, acc_cell, cp, (int)blockIdx.x, (int)gridDim.x);
+}
+#endif
 
 // Shakhov-BGK update of one cell per workgroup with the Hermite force on
 // MFMA; the cell's scalars come from k_cell_pre.
@@ -197,12 +240,12 @@ __global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict
 // Gauss-Hermite sizes), so that the LDS layout and the tile loops fold;
 // NVC = 0: any nv, read at run time.
 template <typename T, int EPT, int NT, int NVC>
-__global__ __launch_bounds__(NT) void k_update2d_mfma(
+__device__ __forceinline__ void k_update2d_mfma_body(
     Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb,
     const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ wnode_d,
-    const double* __restrict__ u1, const double* __restrict__ v1) {
+    const double* __restrict__ u1, const double* __restrict__ v1, int bid, int nblk) {
     using A = typename Arith<T>::A;
     using M = Mfma<T>;
     using V = typename M::V;
@@ -210,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
     __shared__ double red[4 * (NT / WAVE)];
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    const int cell = tiled_cell((int)xcd_block(blockIdx.x, gridDim.x), c.nx, c.ny);
+    const int cell = tiled_cell((int)xcd_block(bid, nblk), c.nx, c.ny);
     const int nx = c.nx, nm = c.nm;
     const int nv = NVC ? NVC : c.nv, nn = NVC ? NVC * NVC : c.nn;
     const int np = mf_np(nv);
@@ -443,6 +486,15 @@ __global__ __launch_bounds__(NT) void k_update2d_mfma(
             __builtin_nontemporal_store((T)bn, bo_p + k);
         }
     }
+}
+template <typename T, int EPT, int NT, int NVC>
+__global__ __launch_bounds__(NT) void k_update2d_mfma(
+    Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
+    T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
+    const T* __restrict__ hfh, const T* __restrict__ hfb,
+    const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ wnode_d,
+    const double* __restrict__ u1, const double* __restrict__ v1) {
+    k_update2d_mfma_body<T, EPT, NT, NVC>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode, wnode_d, u1, v1, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace hugks

@@ -1,0 +1,81 @@
+// Host launchers of the grouped batch kernels (k_group.h).
+// the kernel bodies only: the single-case kernels belong to the other
+// launch files
+#define HUGKS_BODIES_ONLY
+#include "k_group.h"
+
+namespace hugks {
+
+void group_blocks(const Launch2D& L, int* slope, int* fsum, int* fface, int* fnode, int* wall,
+                  int* cpre) {
+    const Consts& c = L.c;
+    int nvf, nint;
+    face_counts(c, nvf, nint);
+    const int ncell = c.nx * c.ny;
+    *slope = (ncell + SLOPE_CPW - 1) / SLOPE_CPW;
+    *fsum = *fnode = (nint + SPLIT_FPW - 1) / SPLIT_FPW;
+    *fface = (nint + 255) / 256;
+    *wall = ((int)L.nwall + 3) / 4;
+    *cpre = (ncell + 255) / 256;
+}
+
+void group_update_key(const Launch2D& L, int* ept, int* nt, int* nvc) {
+    const UpdKey k = update_key(L.c, L.fp64);
+    *ept = k.ept;
+    *nt = k.nt;
+    *nvc = k.nvc;
+}
+
+template <typename T>
+static hipError_t group_update_t(const GroupUpd& u, const Launch2D* Ls, hipStream_t s,
+                                 bool prepare) {
+#define HUGKS_GUPD_CASE(E, N, V)                                                            \
+    if (u.ept == E && u.nt == N && u.nvc == V) {                                            \
+        if (prepare)                                                                        \
+            return hipFuncSetAttribute((const void*)kg_update<T, E, N, V>,                  \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,          \
+                                       (int)u.lds);                                         \
+        hipLaunchKernelGGL((kg_update<T, E, N, V>), dim3(u.blocks), dim3(N), u.lds, s,      \
+                           u.tab, Ls);                                                      \
+        return hipGetLastError();                                                           \
+    }
+    if (sizeof(T) == 4) {
+        HUGKS_UPDM_F32(HUGKS_GUPD_CASE)
+    } else {
+        HUGKS_UPDM_F64(HUGKS_GUPD_CASE)
+    }
+    HUGKS_UPDM_ANY(HUGKS_GUPD_CASE)
+#undef HUGKS_GUPD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t prepare_group_update(bool fp64, const GroupUpd& u) {
+    return fp64 ? group_update_t<double>(u, nullptr, nullptr, true)
+                : group_update_t<float>(u, nullptr, nullptr, true);
+}
+
+template <typename T>
+static hipError_t group_step_t(const GroupPlan& P, hipStream_t s) {
+    hipLaunchKernelGGL((kg_timestep), dim3(P.g_ts), dim3(RED_NT), 0, s, P.t_ts, P.Ls);
+    hipLaunchKernelGGL((kg_slope<T>), dim3(P.g_slope), dim3(WAVE * SLOPE_CPW), 0, s, P.t_slope,
+                       P.Ls);
+    hipLaunchKernelGGL((kg_fsum<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s, P.t_fsum, P.Ls);
+    hipLaunchKernelGGL((kg_fface), dim3(P.g_fface), dim3(256), 0, s, P.t_fface, P.Ls);
+    hipLaunchKernelGGL((kg_fnode<T>), dim3(P.g_fnode), dim3(WAVE * SPLIT_FPW), 0, s, P.t_fnode,
+                       P.Ls);
+    if (P.g_wall > 0)
+        hipLaunchKernelGGL((kg_wall<T>), dim3(P.g_wall), dim3(256), 0, s, P.t_wall, P.Ls);
+    hipLaunchKernelGGL((kg_cell_pre), dim3(P.g_cpre), dim3(256), 0, s, P.t_cpre, P.Ls);
+    for (int q = 0; q < P.nupd; ++q) {
+        hipError_t e = group_update_t<T>(P.upd[q], P.Ls, s, false);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((kg_finalize), dim3(P.g_fin), dim3(RED_NT), 0, s, P.t_fin, P.Ls);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_step(const GroupPlan& P, hipStream_t s) {
+    return P.fp64 ? group_step_t<double>(P, s) : group_step_t<float>(P, s);
+}
+
+}  // namespace hugks

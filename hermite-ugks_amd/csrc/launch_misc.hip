@@ -6,7 +6,6 @@ namespace hugks {
 // Workgroups of the first reduction pass; 0 (one single-workgroup pass)
 // for grids a single workgroup reduces quickly, where a second launch
 // would cost more than it saves.
-constexpr int PRED_MIN_CELLS = 16384;
 static int pred_blocks(const Consts& c) {
     if (c.nx * c.ny < PRED_MIN_CELLS) return 0;
     const int n = (c.nx * c.ny + PRED_NT - 1) / PRED_NT;

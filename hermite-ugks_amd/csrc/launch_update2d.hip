@@ -55,13 +55,17 @@ size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64) {
 template <typename T>
 static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool prepare) {
     const Consts& c = L.c;
-    const int nt = mf_threads(c.nn, sizeof(T) == 8);
-    const int ept = (c.nn + nt - 1) / nt;
+    const UpdKey key = update_key(c, sizeof(T) == 8);
     const unsigned ncell = (unsigned)(c.nx * c.ny);
     const size_t lds = update2d_mfma_lds_bytes(c, sizeof(T) == 8);
     using A = typename Arith<T>::A;
+    if (!prepare) {
+        HookScope hs(L, "cell_pre");
+        hipLaunchKernelGGL((k_cell_pre), dim3((ncell + 255) / 256), dim3(256), 0, s, c, L.st, L.W,
+                           L.part, L.vflux, L.hflux, L.acc_cell, L.cellpre);
+    }
 #define HUGKS_UPDM_CASE(E, N, V)                                                                \
-    if (ept == E && nt == N && nvc == V) {                                                      \
+    if (key.ept == E && key.nt == N && key.nvc == V) {                                          \
         if (prepare)                                                                            \
             return hipFuncSetAttribute((const void*)k_update2d_mfma<T, E, N, V>,                \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
@@ -71,45 +75,12 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool pr
             (const T*)L.hfh, (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.u1, L.v1);        \
         return hipGetLastError();                                                               \
     }
-    if (!prepare) {
-        HookScope hs(L, "cell_pre");
-        hipLaunchKernelGGL((k_cell_pre), dim3((ncell + 255) / 256), dim3(256), 0, s, c, L.st, L.W,
-                           L.part, L.vflux, L.hflux, L.acc_cell, L.cellpre);
-    }
-    // compiled-in grid sizes (EPT and threads as mf_threads chooses them)
-    int nvc = (c.nv == 16 || c.nv == 20 || c.nv == 24 || c.nv == 28 || c.nv == 32 ||
-                     c.nv == 40) ? c.nv : 0;
     if (sizeof(T) == 4) {
-        HUGKS_UPDM_CASE(1, 256, 16)
-        HUGKS_UPDM_CASE(2, 256, 20)
-        HUGKS_UPDM_CASE(3, 256, 24)
-        HUGKS_UPDM_CASE(4, 256, 28)
-        HUGKS_UPDM_CASE(4, 256, 32)
-        HUGKS_UPDM_CASE(7, 256, 40)
+        HUGKS_UPDM_F32(HUGKS_UPDM_CASE)
     } else {
-        HUGKS_UPDM_CASE(1, 256, 16)
-        HUGKS_UPDM_CASE(2, 256, 20)
-        HUGKS_UPDM_CASE(2, 320, 24)
-        HUGKS_UPDM_CASE(3, 320, 28)
-        HUGKS_UPDM_CASE(4, 256, 32)
-        HUGKS_UPDM_CASE(5, 320, 40)
+        HUGKS_UPDM_F64(HUGKS_UPDM_CASE)
     }
-    nvc = 0;   // any other size (or shape) on the run-time kernel
-    HUGKS_UPDM_CASE(1, 256, 0)
-    HUGKS_UPDM_CASE(2, 256, 0)
-    HUGKS_UPDM_CASE(3, 256, 0)
-    HUGKS_UPDM_CASE(4, 256, 0)
-    HUGKS_UPDM_CASE(5, 256, 0)
-    HUGKS_UPDM_CASE(6, 256, 0)
-    HUGKS_UPDM_CASE(7, 256, 0)
-    HUGKS_UPDM_CASE(8, 256, 0)
-    HUGKS_UPDM_CASE(1, 320, 0)
-    HUGKS_UPDM_CASE(2, 320, 0)
-    HUGKS_UPDM_CASE(3, 320, 0)
-    HUGKS_UPDM_CASE(4, 320, 0)
-    HUGKS_UPDM_CASE(5, 320, 0)
-    HUGKS_UPDM_CASE(6, 320, 0)
-    HUGKS_UPDM_CASE(7, 320, 0)
+    HUGKS_UPDM_ANY(HUGKS_UPDM_CASE)
 #undef HUGKS_UPDM_CASE
     return hipErrorInvalidValue;
 }

@@ -177,6 +177,11 @@ module hugks_capi
       type(c_ptr), value :: batch
     end function hugks_batch_destroy
 
+    integer(c_int32_t) function hugks_batch_grouped(batch) bind(c, name='hugks_batch_grouped')
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+    end function hugks_batch_grouped
+
     integer(c_int) function hugks_batch_handle(batch, i, handle) bind(c, name='hugks_batch_handle')
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: batch

@@ -67,7 +67,8 @@ EXPORTS = [
     "hugks_checkpoint_write", "hugks_checkpoint_read", "hugks_kernel_times",
     "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
     "hugks_set_schedule", "hugks_get_schedule", "hugks_tune", "hugks_batch_create",
-    "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_handle",
+    "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_grouped",
+    "hugks_batch_handle",
     "hugks_batch_advance",
 ]
 ARR_SHX, ARR_SHY, ARR_SBX, ARR_SBY, ARR_VFH, ARR_VFB, ARR_HFH, ARR_HFB, ARR_VFLUX, ARR_HFLUX = range(10)
@@ -116,6 +117,8 @@ def load(path=LIBPATH):
     lib.hugks_batch_last_error.restype = C.c_char_p
     lib.hugks_batch_size.argtypes = [H]
     lib.hugks_batch_size.restype = C.c_int32
+    lib.hugks_batch_grouped.argtypes = [H]
+    lib.hugks_batch_grouped.restype = C.c_int32
     lib.hugks_batch_handle.argtypes = [H, C.c_int32, C.POINTER(H)]
     lib.hugks_batch_advance.argtypes = [H, C.c_int32, C.POINTER(StepInfo)]
     _lib = lib
@@ -331,6 +334,11 @@ class Batch:
     def _check(self, rc):
         if rc != OK:
             raise HugksError(rc, self.lib.hugks_batch_last_error(self.b).decode())
+
+    @property
+    def grouped(self):
+        """True when one grouped launch per kernel type runs every case."""
+        return bool(self.lib.hugks_batch_grouped(self.b))
 
     def advance(self, max_steps):
         infos = (StepInfo * len(self.cases))()

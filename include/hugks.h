@@ -246,6 +246,11 @@ int hugks_batch_create(int32_t n, const hugks_desc *descs, int32_t flags,
 int hugks_batch_destroy(hugks_batch *b);
 const char *hugks_batch_last_error(const hugks_batch *b);
 int32_t hugks_batch_size(const hugks_batch *b);
+/* 1 when a step of the batch is one grouped launch per kernel type (all
+ * cases small 2D cases of one storage type with factoring weights and an
+ * MFMA-capable grid; they then run the split flux and the MFMA update),
+ * 0 when each case's kernels are launched on their own. */
+int32_t hugks_batch_grouped(const hugks_batch *b);
 int hugks_batch_handle(hugks_batch *b, int32_t i, hugks_handle **h);
 /* max_steps replays (<= 0: until every case stopped); infos[n] receives
  * each case's loop state. */

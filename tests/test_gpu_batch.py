@@ -92,3 +92,78 @@ def test_batch_cases_stop_on_their_own(have_gpu):
         assert inf.stop_reason == capi.STOP_MAXTIME
         assert inf.steps == alone.steps and inf.iter == alone.iter
         assert inf.simtime == alone.simtime
+
+
+def grouped_cases(prec):
+    return [
+        cases.build_2d(poiseuille(nv=12, ny=10), prec),
+        cases.build_2d(poiseuille(nv=16, ny=14, kn=0.5, phi=0.1, tw=(1.0, 1.2)), prec),
+        cases.build_2d(rt(nv=14), prec, rt_variant="cartesian", seed=4),
+        cases.build_2d(poiseuille(nv=20, ny=8, nm=8), prec),
+        cases.build_2d(rt(nv=24, nm=8), prec, rt_variant="cartesian", seed=9),
+        cases.build_2d(poiseuille(nv=13, ny=6, nm=6), prec),
+    ]
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED_FP32])
+def test_grouped_batch_equals_cases_stepped_alone(have_gpu, prec):
+    """Cases of one storage type run as one grouped launch per kernel type;
+    every case must match, bitwise and on every replay, the same case
+    stepped alone with the schedule the batch gave it."""
+    cs = grouped_cases(prec)
+    with capi.Batch(cs) as b:
+        print("grouped", b.grouped, "prec", prec, flush=True)
+        assert b.grouped
+        singles = []
+        for c, s in zip(cs, b.solvers):
+            x = capi.Solver(c)
+            x.set_schedule(*s.schedule())
+            singles.append(x)
+        for rep, n in enumerate((2, 1, 3)):
+            infos = b.advance(n)
+            assert [i.steps for i in infos] == [n] * len(cs)
+            for k, s in enumerate(singles):
+                s.advance(n)
+                wa = s.macro(with_q=False)
+                wb = b.solvers[k].macro(with_q=False)
+                assert np.array_equal(wa, wb), (rep, k, np.max(np.abs(wa - wb)))
+                ha, _ = s.get_f()
+                hb, _ = b.solvers[k].get_f()
+                assert np.array_equal(ha, hb), (rep, k)
+        for s in singles:
+            s.close()
+
+
+def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
+    """A grouped mixed batch against the oracle; with HUGKS_BATCH_GROUPED=0
+    the same batch runs case by case and gives the same bits; cases with
+    different maxtime stop on their own."""
+    cs = grouped_cases(capi.MIXED_FP32)
+    with capi.Batch(cs) as b:
+        assert b.grouped
+        b.advance(5)
+        wg = [s.macro(with_q=False) for s in b.solvers]
+    monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
+    with capi.Batch(cs) as b:
+        assert not b.grouped
+        for s in b.solvers:
+            s.set_schedule(capi.FLUX_SPLIT, capi.UPDATE_MFMA)
+        b.advance(5)
+        wl = [s.macro(with_q=False) for s in b.solvers]
+    monkeypatch.delenv("HUGKS_BATCH_GROUPED")
+    for c, a, x in zip(cs, wg, wl):
+        assert np.array_equal(a, x)
+        o = oracle_py.Oracle(c.with_precision(capi.FP64))
+        o.step(5)
+        err = field_errors(a, o.get()[0])
+        print("nv", c.desc.nv, "err", err, flush=True)
+        assert np.all(err <= 1e-5), err
+    ts = [cases.build_2d(poiseuille(nv=10, ny=8, maxtime=t), capi.FP64) for t in (0.02, 0.05, 0.035)]
+    with capi.Batch(ts) as b:
+        assert b.grouped
+        infos = b.advance(0)
+    for c, inf in zip(ts, infos):
+        with capi.Solver(c) as s:
+            alone = s.advance(0)
+        assert inf.stop_reason == capi.STOP_MAXTIME
+        assert inf.steps == alone.steps and inf.simtime == alone.simtime
