@@ -117,7 +117,8 @@ __global__ __launch_bounds__(256) void kg_cell_pre(const int* __restrict__ tab,
 }
 
 template <typename T, int EPT, int NT, int NVC>
-__global__ __launch_bounds__(NT) void kg_update(const int* __restrict__ tab,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+void kg_update(const int* __restrict__ tab,
                                                 const Launch2D* __restrict__ Ls) {
     using A = typename Arith<T>::A;
     const GroupIdx g = group_index(tab);

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_cell_pre(Consts c, DevState* __restrict
 // NVC = 0: any nv, read at run time.
 template <typename T, int EPT, int NT, int NVC>
 __device__ __forceinline__ void k_update2d_mfma_body(
-    Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
+    const Consts& c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb,
     const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ wnode_d,
@@ -459,6 +459,18 @@ __device__ __forceinline__ void k_update2d_mfma_body(
     const T* fb_u = hfb + (size_t)hu * c.ldv;
     T* ho_p = h + (size_t)cell * c.ldv;
     T* bo_p = b + (size_t)cell * c.ldv;
+    // the face-flux divergence of every node first: all its loads are in
+    // flight before the node arithmetic waits on the first of them
+    A dvh[EPT], dvb[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int k = tid + e * NT;
+        dvh[e] = dvb[e] = (A)0;
+        if (k < nn) {
+            dvh[e] = ((A)fh_l[k] - (A)fh_r[k] + (A)fh_d[k] - (A)fh_u[k]) * iarea;
+            dvb[e] = ((A)fb_l[k] - (A)fb_r[k] + (A)fb_d[k] - (A)fb_u[k]) * iarea;
+        }
+    }
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = tid + e * NT;
@@ -476,8 +488,7 @@ __device__ __forceinline__ void k_update2d_mfma_body(
             const double Bpn = kfn * fma(2.0 * cqn, Hn, Hpn);
             const A om = (A)(rx[MR_OM * nv] * ry[MR_OM * nv]);
             const double gh = (double)(om * (A)F[k]), gb = (double)(om * (A)F[nn + k]);
-            const double dfh = (double)(((A)fh_l[k] - (A)fh_r[k] + (A)fh_d[k] - (A)fh_u[k]) * iarea);
-            const double dfb = (double)(((A)fb_l[k] - (A)fb_r[k] + (A)fb_d[k] - (A)fb_u[k]) * iarea);
+            const double dfh = (double)dvh[e], dfb = (double)dvb[e];
             const double ho = (double)rh[e], bo = (double)rb[e];
             const double hn = (ho + dfh + a_n * Hpn + a_o * (Hpo - ho) + gh) * iden;
             const double bn = (bo + dfb + a_n * Bpn + a_o * (Bpo - bo) + gb) * iden;
@@ -487,8 +498,11 @@ __device__ __forceinline__ void k_update2d_mfma_body(
         }
     }
 }
+// FP32 storage: at least five wavefronts per SIMD (the LDS of a cell allows
+// five workgroups per CU at nv = 40).
 template <typename T, int EPT, int NT, int NVC>
-__global__ __launch_bounds__(NT) void k_update2d_mfma(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+void k_update2d_mfma(
     Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb,
