@@ -246,7 +246,8 @@ __global__ __launch_bounds__(256) void k_flux_wall(
     const typename Arith<T>::A* __restrict__ vnode, const typename Arith<T>::A* __restrict__ wnode,
     T* __restrict__ vfh, T* __restrict__ vfb, double* __restrict__ vflux, T* __restrict__ hfh,
     T* __restrict__ hfb, double* __restrict__ hflux, int nwall) {
-    k_flux_wall_body<T>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, wnode, vfh, vfb, vflux, hfh, hfb, hflux, nwall, (int)blockIdx.x, (int)gridDim.x);
+    k_flux_wall_body<T>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, wnode, vfh, vfb, vflux, hfh,
+        hfb, hflux, nwall, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace hugks

@@ -38,6 +38,10 @@
 
 namespace hugks {
 
+// The single-case kernels of this header are defined only where
+// HUGKS_BODIES_ONLY is not set: the grouped-launch file (launch_group.hip)
+// takes the device bodies alone, so no kernel is defined in two files.
+
 constexpr int FSUM_N = 37;      // K1 sums per face
 constexpr int FSUM_LD = FACE_SUM_LD;
 constexpr int FCOEF_LD = FACE_COEF_LD;  // K2/K4 coefficients per face
@@ -312,7 +316,8 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
     const double* __restrict__ vnd, const double* __restrict__ wnd,
     const double* __restrict__ acc_v, const double* __restrict__ acc_h,
     double* __restrict__ fsum, int nvi, int nint) {
-    k_fsum_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd, acc_v, acc_h, fsum, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
+    k_fsum_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd, acc_v, acc_h,
+        fsum, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ------------------------------------------------------------------ K2
@@ -690,7 +695,8 @@ __device__ __forceinline__ void k_fface_body(Consts c, const DevState* __restric
                                                const double* __restrict__ acc_h,
                                                const double* __restrict__ fsum, double* fcoef,
                                                double* __restrict__ vflux,
-                                               double* __restrict__ hflux, int nvi, int nint, int bid, int nblk) {
+                                               double* __restrict__ hflux, int nvi, int nint,
+                                               int bid, int nblk) {
     if (st->stopped) return;
     const int g = bid * 256 + threadIdx.x;
     if (g >= nint) return;
@@ -712,7 +718,8 @@ __global__ __launch_bounds__(256) void k_fface(Consts c, const DevState* __restr
                                                const double* __restrict__ fsum, double* fcoef,
                                                double* __restrict__ vflux,
                                                double* __restrict__ hflux, int nvi, int nint) {
-    k_fface_body(c, st, W, u1, v1, wx, wy, acc_v, acc_h, fsum, fcoef, vflux, hflux, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
+    k_fface_body(c, st, W, u1, v1, wx, wy, acc_v, acc_h, fsum, fcoef, vflux, hflux, nvi, nint,
+        (int)blockIdx.x, (int)gridDim.x);
 }
 #endif
 
@@ -802,7 +809,8 @@ __device__ __forceinline__ void k_fnode_body(
     const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
     const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ acc_v,
     const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
-    T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint, int bid, int nblk) {
+    T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint, int bid,
+    int nblk) {
     if (st->stopped) return;
     const int g = __builtin_amdgcn_readfirstlane(
         (int)xcd_block(bid, nblk) * SPLIT_FPW + (int)(threadIdx.x / WAVE));
@@ -829,7 +837,8 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
     const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ acc_v,
     const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
     T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint) {
-    k_fnode_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, acc_v, acc_h, fcoef, vfh, vfb, hfh, hfb, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
+    k_fnode_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, acc_v, acc_h, fcoef, vfh,
+        vfb, hfh, hfb, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace hugks

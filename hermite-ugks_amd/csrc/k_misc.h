@@ -6,6 +6,10 @@
 
 namespace hugks {
 
+// The single-case kernels of this header are defined only where
+// HUGKS_BODIES_ONLY is not set: the grouped-launch file (launch_group.hip)
+// takes the device bodies alone, so no kernel is defined in two files.
+
 constexpr int RED_NT = 1024;
 
 template <int NV, int NT = RED_NT>
@@ -95,7 +99,8 @@ __global__ __launch_bounds__(PRED_NT) void k_rate_part(Consts c, int dim,
 // step (1d/src/main.f90:84), so that test is made here.
 __device__ __forceinline__ void k_timestep_body(Consts c, int dim, DevState* st,
                                                      const double* __restrict__ W,
-                                                     const double* __restrict__ red, int nred, int bid, int nblk) {
+                                                     const double* __restrict__ red, int nred,
+                                                     int bid, int nblk) {
     __shared__ double lds[RED_NT / WAVE];
     if (st->stopped) return;
     if (dim == 1 && st->simtime >= c.maxtime) {
@@ -169,7 +174,8 @@ __global__ __launch_bounds__(PRED_NT) void k_part_sum(Consts c, int dim,
 // partials in index order.
 __device__ __forceinline__ void k_finalize_body(Consts c, int dim, DevState* st,
                                                      const double* __restrict__ part,
-                                                     const double* __restrict__ red, int nred, int bid, int nblk) {
+                                                     const double* __restrict__ red, int nred,
+                                                     int bid, int nblk) {
     __shared__ double lds[9 * (RED_NT / WAVE)];
     if (st->stopped) return;
     const int ncell = c.nx * c.ny;

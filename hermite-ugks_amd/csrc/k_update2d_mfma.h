@@ -24,6 +24,10 @@
 
 namespace hugks {
 
+// The single-case kernels of this header are defined only where
+// HUGKS_BODIES_ONLY is not set: the grouped-launch file (launch_group.hip)
+// takes the device bodies alone, so no kernel is defined in two files.
+
 typedef float mf_f4 __attribute__((vector_size(4 * sizeof(float))));
 typedef double mf_d4 __attribute__((vector_size(4 * sizeof(double))));
 
@@ -508,7 +512,8 @@ void k_update2d_mfma(
     const T* __restrict__ hfh, const T* __restrict__ hfb,
     const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ wnode_d,
     const double* __restrict__ u1, const double* __restrict__ v1) {
-    k_update2d_mfma_body<T, EPT, NT, NVC>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode, wnode_d, u1, v1, (int)blockIdx.x, (int)gridDim.x);
+    k_update2d_mfma_body<T, EPT, NT, NVC>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode, wnode_d, u1,
+        v1, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace hugks
