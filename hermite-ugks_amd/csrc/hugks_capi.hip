@@ -893,8 +893,8 @@ int hugks_tune(const hugks_desc* d, int32_t* flux_kernel, int32_t* update_kernel
 struct hugks_batch {
     std::vector<hugks_handle*> cases;
     bool grouped = false;       // one grouped launch per kernel type (k_group.h)
-    GroupPlan plan{};
-    void* gmem = nullptr;       // tables and per-case arguments of the grouped step
+    std::vector<GroupPlan> plans;   // one per storage type present
+    std::vector<void*> gmem;    // per-case arguments and the plans' tables
     hipStream_t main = nullptr;
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> joins;
@@ -913,7 +913,7 @@ static void batch_release(hugks_batch* b) {
     if (b->fork) hipEventDestroy(b->fork);
     for (hipStream_t s : b->lanes) hipStreamDestroy(s);
     if (b->main) hipStreamDestroy(b->main);
-    if (b->gmem) hipFree(b->gmem);
+    for (void* p : b->gmem) hipFree(p);
     for (hugks_handle* h : b->cases) hugks_destroy(h);
     delete b;
 }
@@ -926,25 +926,26 @@ static int batch_fail(hugks_batch* b, int code, const std::string& msg) {
 
 extern "C" {
 
-// A batch runs grouped when every case is a small 2D case of one storage
-// type whose step is the split flux with factoring weights and the MFMA
-// update: then one launch per kernel type covers all cases.
+// A batch runs grouped when every case is a small 2D case whose step can be
+// the split flux with factoring weights and the MFMA update: then one launch
+// per kernel type and storage type covers all cases.
 static bool batch_groupable(const hugks_batch* b) {
     if (const char* g = getenv("HUGKS_BATCH_GROUPED"))
         if (strcmp(g, "0") == 0) return false;
     for (const hugks_handle* h : b->cases) {
         const Launch2D& L = h->L;
-        if (h->dim != 2 || L.fp64 != b->cases[0]->L.fp64 || !L.wx ||
-            !update2d_mfma_supported(L.c) || L.c.nx * L.c.ny >= PRED_MIN_CELLS)
+        if (h->dim != 2 || !L.wx || !update2d_mfma_supported(L.c) ||
+            L.c.nx * L.c.ny >= PRED_MIN_CELLS)
             return false;
     }
     return true;
 }
 
-// Tables of the grouped launches ([n, start[0..n], case[0..n-1]] each) and
-// the per-case arguments, in one device allocation.
-static hipError_t batch_group_plan(hugks_batch* b) {
-    const int n = (int)b->cases.size();
+// Plan of the grouped launches of the cases ids (one storage type): tables
+// ([n, start[0..n], case[0..n-1]] each, case = index into Ls) in one device
+// allocation.
+static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& ids,
+                                      const Launch2D* Ls) {
     std::vector<std::vector<int>> tabs;
     std::vector<int> blocks;
     auto table = [&](const std::vector<std::pair<int, int>>& cs) {   // (case, workgroups)
@@ -964,9 +965,8 @@ static hipError_t batch_group_plan(hugks_batch* b) {
     std::vector<std::pair<int, int>> one, sl, fs, ff, fn, wl, cp;
     std::map<std::tuple<int, int, int>, std::vector<std::pair<int, int>>> upd;
     std::map<std::tuple<int, int, int>, size_t> upd_lds;
-    for (int i = 0; i < n; ++i) {
+    for (int i : ids) {
         hugks_handle* h = b->cases[i];
-        hugks_set_schedule(h, HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA);
         int s1, s2, s3, s4, s5, s6, e, t, v;
         group_blocks(h->L, &s1, &s2, &s3, &s4, &s5, &s6);
         one.push_back({i, 1});
@@ -982,34 +982,29 @@ static hipError_t batch_group_plan(hugks_batch* b) {
         const size_t lds = update2d_mfma_lds_bytes(h->L.c, h->fp64);
         upd_lds[key] = std::max(upd_lds[key], lds);
     }
-    if (upd.size() > 16) return hipErrorInvalidValue;
-    GroupPlan& P = b->plan;
-    P.fp64 = b->cases[0]->fp64;
+    GroupPlan P{};
+    if (upd.size() > sizeof(P.upd) / sizeof(P.upd[0])) return hipErrorInvalidValue;
+    P.fp64 = b->cases[ids[0]]->fp64;
     const int k_ts = table(one), k_sl = table(sl), k_fs = table(fs), k_ff = table(ff),
               k_fn = table(fn), k_wl = wl.empty() ? -1 : table(wl), k_cp = table(cp);
     std::vector<int> k_up;
     for (auto& kv : upd) k_up.push_back(table(kv.second));
-    // one allocation: per-case Launch2D, then the tables
     size_t ints = 0;
     std::vector<size_t> off;
     for (auto& t : tabs) {
         off.push_back(ints);
         ints += t.size();
     }
-    const size_t lbytes = (n * sizeof(Launch2D) + 255) / 256 * 256;
-    hipError_t e = hipMalloc(&b->gmem, lbytes + ints * sizeof(int));
+    void* tmem = nullptr;
+    hipError_t e = hipMalloc(&tmem, ints * sizeof(int));
     if (e != hipSuccess) return e;
-    std::vector<Launch2D> Ls(n);
-    for (int i = 0; i < n; ++i) Ls[i] = b->cases[i]->L;
+    b->gmem.push_back(tmem);
     std::vector<int> flat;
     for (auto& t : tabs) flat.insert(flat.end(), t.begin(), t.end());
-    e = hipMemcpy(b->gmem, Ls.data(), n * sizeof(Launch2D), hipMemcpyHostToDevice);
-    char* tb = static_cast<char*>(b->gmem) + lbytes;
-    if (e == hipSuccess)
-        e = hipMemcpy(tb, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
+    e = hipMemcpy(tmem, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    auto tp = [&](int k) { return reinterpret_cast<const int*>(tb) + off[k]; };
-    P.Ls = static_cast<const Launch2D*>(b->gmem);
+    auto tp = [&](int k) { return static_cast<const int*>(tmem) + off[k]; };
+    P.Ls = Ls;
     P.t_ts = P.t_fin = tp(k_ts);
     P.g_ts = P.g_fin = blocks[k_ts];
     P.t_slope = tp(k_sl); P.g_slope = blocks[k_sl];
@@ -1033,7 +1028,28 @@ static hipError_t batch_group_plan(hugks_batch* b) {
         e = prepare_group_update(P.fp64, u);
         if (e != hipSuccess) return e;
     }
+    b->plans.push_back(P);
     return hipSuccess;
+}
+
+// Per-case arguments to device memory, then one plan per storage type.
+static hipError_t batch_group_plan(hugks_batch* b) {
+    const int n = (int)b->cases.size();
+    void* lmem = nullptr;
+    hipError_t e = hipMalloc(&lmem, n * sizeof(Launch2D));
+    if (e != hipSuccess) return e;
+    b->gmem.push_back(lmem);
+    std::vector<Launch2D> Ls(n);
+    std::vector<int> ids[2];
+    for (int i = 0; i < n; ++i) {
+        hugks_set_schedule(b->cases[i], HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA);
+        Ls[i] = b->cases[i]->L;
+        ids[b->cases[i]->fp64 ? 1 : 0].push_back(i);
+    }
+    e = hipMemcpy(lmem, Ls.data(), n * sizeof(Launch2D), hipMemcpyHostToDevice);
+    for (int t = 0; t < 2 && e == hipSuccess; ++t)
+        if (!ids[t].empty()) e = batch_group_plan_of(b, ids[t], static_cast<const Launch2D*>(lmem));
+    return e;
 }
 
 int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_batch** out) {
@@ -1041,15 +1057,6 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     *out = nullptr;
     hugks_batch* b = new hugks_batch();
     for (int32_t i = 0; i < n; ++i) {
-        int fk = -1, uk = -1;
-        if (flags & HUGKS_BATCH_AUTOTUNE) {
-            int rc = tune_case(&descs[i], &fk, &uk, nullptr);
-            if (rc != HUGKS_OK) {
-                std::string m = "case " + std::to_string(i) + ": " + g_create_error;
-                batch_release(b);
-                return batch_fail(nullptr, rc, m);
-            }
-        }
         hugks_handle* h = nullptr;
         int rc = hugks_create(&descs[i], &h);
         if (rc != HUGKS_OK) {
@@ -1058,7 +1065,19 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
             return batch_fail(nullptr, rc, m);
         }
         b->cases.push_back(h);
-        if (fk >= 0) hugks_set_schedule(h, fk, uk);
+    }
+    // a grouped batch runs its own schedule (split flux, MFMA update); the
+    // tuner picks the schedules of a batch stepped case by case
+    const bool groupable = batch_groupable(b);
+    for (int32_t i = 0; i < n && (flags & HUGKS_BATCH_AUTOTUNE) && !groupable; ++i) {
+        int fk = -1, uk = -1;
+        int rc = tune_case(&descs[i], &fk, &uk, nullptr);
+        if (rc != HUGKS_OK) {
+            std::string m = "case " + std::to_string(i) + ": " + g_create_error;
+            batch_release(b);
+            return batch_fail(nullptr, rc, m);
+        }
+        hugks_set_schedule(b->cases[i], fk, uk);
     }
     // one step of every case, fanned out over a few streams and captured
     // into one graph: a replay advances the whole batch by one step
@@ -1078,11 +1097,12 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     // cases finish their own set-up on their streams first
     for (hugks_handle* h : b->cases)
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    b->grouped = e == hipSuccess && batch_groupable(b);
+    b->grouped = e == hipSuccess && groupable;
     if (b->grouped) e = batch_group_plan(b);
     if (e == hipSuccess) e = hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal);
     if (b->grouped) {
-        if (e == hipSuccess) e = launch_group_step(b->plan, b->main);
+        for (size_t q = 0; q < b->plans.size() && e == hipSuccess; ++q)
+            e = launch_group_step(b->plans[q], b->main);
     } else {
         if (e == hipSuccess) e = hipEventRecord(b->fork, b->main);
         for (int l = 0; l < nl && e == hipSuccess; ++l)

@@ -36,11 +36,16 @@ def batch_cases():
     ]
 
 
-@pytest.mark.parametrize("autotune", [False, True])
-def test_batch_equals_cases_stepped_alone(have_gpu, autotune):
+@pytest.mark.parametrize("autotune,grouped", [(False, True), (False, False), (True, False)])
+def test_batch_equals_cases_stepped_alone(have_gpu, monkeypatch, autotune, grouped):
+    """FP64 and FP32 cases in one batch: grouped (one plan per storage type)
+    or stepped case by case over lane streams, with or without the tuner."""
     cs = batch_cases()
-    print(f"batch of {len(cs)} autotune={autotune}", flush=True)
+    print(f"batch of {len(cs)} autotune={autotune} grouped={grouped}", flush=True)
+    if not grouped:
+        monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
     with capi.Batch(cs, autotune=autotune) as b:
+        assert b.grouped == grouped
         scheds = [s.schedule() for s in b.solvers]
         print("schedules", scheds, flush=True)
         singles = []
@@ -64,9 +69,11 @@ def test_batch_equals_cases_stepped_alone(have_gpu, autotune):
             s.close()
 
 
-def test_batch_autotuned_parity_with_oracle(have_gpu):
+def test_batch_autotuned_parity_with_oracle(have_gpu, monkeypatch):
     cs = batch_cases()
+    monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
     with capi.Batch(cs, autotune=True) as b:
+        assert not b.grouped
         b.advance(6)
         ws = [s.macro(with_q=False) for s in b.solvers]
     for c, w in zip(cs, ws):
