@@ -176,6 +176,34 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
 // keeps the first half, the upper lane the second (zero-padded when M is
 // odd).  On the way back pos becomes the value index the lane ends up
 // holding, or -1 where its path ran through padding.
+// The exchanges across 32 and 16 lanes use the gfx950 lane swaps:
+// v_permlane32_swap(a, b) leaves a = [a lanes 0-31, b lanes 0-31] and
+// b = [a lanes 32-63, b lanes 32-63], v_permlane16_swap does the same per
+// pair of 16-lane rows, so a + b is the stage's sum on every lane (a
+// partner's lo value on the lower lane of a pair, hi on the upper) with no
+// selects and no LDS round trip; the same additions as the shuffle form.
+template <int O>
+__device__ __forceinline__ double swap_sum(double lo_v, double hi_v) {
+    static_assert(O == 32 || O == 16, "lane swaps exist for 32 and 16");
+    const unsigned long long a = __builtin_bit_cast(unsigned long long, lo_v);
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, hi_v);
+    const unsigned a0 = (unsigned)a, a1 = (unsigned)(a >> 32);
+    const unsigned b0 = (unsigned)b, b1 = (unsigned)(b >> 32);
+    unsigned r00, r01, r10, r11;
+    if constexpr (O == 32) {
+        const auto w0 = __builtin_amdgcn_permlane32_swap(a0, b0, false, false);
+        const auto w1 = __builtin_amdgcn_permlane32_swap(a1, b1, false, false);
+        r00 = w0[0]; r01 = w0[1]; r10 = w1[0]; r11 = w1[1];
+    } else {
+        const auto w0 = __builtin_amdgcn_permlane16_swap(a0, b0, false, false);
+        const auto w1 = __builtin_amdgcn_permlane16_swap(a1, b1, false, false);
+        r00 = w0[0]; r01 = w0[1]; r10 = w1[0]; r11 = w1[1];
+    }
+    const double x = __builtin_bit_cast(double, (unsigned long long)r00 | ((unsigned long long)r10 << 32));
+    const double y = __builtin_bit_cast(double, (unsigned long long)r01 | ((unsigned long long)r11 << 32));
+    return x + y;
+}
+
 template <int M, int O>
 __device__ __forceinline__ double wave_rs_stage(double* x, int lane, int& pos) {
     constexpr int M2 = (M + 1) / 2;
@@ -184,9 +212,13 @@ __device__ __forceinline__ double wave_rs_stage(double* x, int lane, int& pos) {
     for (int t = 0; t < M2; ++t) {
         const double lo_v = x[t];
         const double hi_v = (t + M2 < M) ? x[t + M2] : 0.0;
-        const double keep = hi ? hi_v : lo_v;
-        const double send = hi ? lo_v : hi_v;
-        x[t] = keep + __shfl_xor(send, O, WAVE);
+        if constexpr (O == 32 || O == 16) {
+            x[t] = swap_sum<O>(lo_v, hi_v);
+        } else {
+            const double keep = hi ? hi_v : lo_v;
+            const double send = hi ? lo_v : hi_v;
+            x[t] = keep + __shfl_xor(send, O, WAVE);
+        }
     }
     double r;
     if constexpr (O > 1) {

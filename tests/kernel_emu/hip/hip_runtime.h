@@ -190,6 +190,39 @@ inline T __shfl(T v, int lane, int width = 64) {
     return r;
 }
 
+// v_permlane32_swap / v_permlane16_swap (gfx950): returns {new vdst, new
+// src0}.  32: lanes 32-63 of vdst trade places with lanes 0-31 of src0;
+// 16: the odd 16-lane rows of vdst with the even rows of src0.
+struct PlSwap {
+    unsigned v[2];
+    unsigned operator[](int i) const { return v[i]; }
+};
+namespace emu {
+inline PlSwap permlane_swap(unsigned vd, unsigned s0, unsigned w) {
+    const unsigned t = threadIdx.x, l = t & 63u;
+    std::memcpy(emu::xbuf + 8 * t, &vd, 4);
+    std::memcpy(emu::xbuf + 8 * t + 4, &s0, 4);
+    emu::wave_barrier();
+    const bool upper = (l & w) != 0;
+    PlSwap r;
+    if (upper) {
+        std::memcpy(&r.v[0], emu::xbuf + 8 * (t - w) + 4, 4);   // src0 of the lower row
+        r.v[1] = s0;
+    } else {
+        r.v[0] = vd;
+        std::memcpy(&r.v[1], emu::xbuf + 8 * (t + w), 4);       // vdst of the upper row
+    }
+    emu::wave_barrier();
+    return r;
+}
+}  // namespace emu
+inline PlSwap __builtin_amdgcn_permlane32_swap(unsigned vd, unsigned s0, bool, bool) {
+    return emu::permlane_swap(vd, s0, 32);
+}
+inline PlSwap __builtin_amdgcn_permlane16_swap(unsigned vd, unsigned s0, bool, bool) {
+    return emu::permlane_swap(vd, s0, 16);
+}
+
 // 16x16x4 matrix tiles, lane layouts as on gfx950: A(i,k) from lane
 // 16k+i, B(k,j) from lane 16k+j; result element v of lane l is
 // C(4*(l/16)+v, l%16) for f32 and C(l/16+4*v, l%16) for f64.
