@@ -750,15 +750,29 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
     const A mxi2 = (A)((c.K * c.K + 2.0 * c.K) / (4.0 * p3 * p3));
     const A csA = (A)(c.shk * (1.0 - c.prandtl) * p3 * p3 / p0);
     const A q0 = (A)fc[FC_Q], q1 = (A)fc[FC_Q + 1];
+    // per-face factors: with P(a) = a0 + a1 vn + a2 vt + (a3/2) c^2, the
+    // Maxwellian-slope terms are g_h = H P(a) + (a3/2) B and
+    // g_b = B P(a) + (a3/2) mxi2 H; the node flux is vn times a sum of
+    // products of per-node coefficients and these terms
     const A flenA = (A)flen;
-    A a_L[4], a_R[4], a_b[4], a_T[4];
+    A sL[5], sR[5], pb[5], pT[5];   // a0, a1, a2, a3/2, a3 mxi2/2
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        a_L[q] = (A)fc[FC_AL + q];
-        a_R[q] = (A)fc[FC_AR + q];
-        a_b[q] = (A)fc[FC_AB + q];
-        a_T[q] = (A)fc[FC_AT + q];
+    for (int q = 0; q < 3; ++q) {
+        sL[q] = (A)fc[FC_AL + q];
+        sR[q] = (A)fc[FC_AR + q];
+        pb[q] = (A)fc[FC_AB + q];
+        pT[q] = (A)fc[FC_AT + q];
     }
+    sL[3] = (A)(0.5 * fc[FC_AL + 3]);
+    sR[3] = (A)(0.5 * fc[FC_AR + 3]);
+    pb[3] = (A)(0.5 * fc[FC_AB + 3]);
+    pT[3] = (A)(0.5 * fc[FC_AT + 3]);
+    sL[4] = sL[3] * mxi2;
+    sR[4] = sR[3] * mxi2;
+    pb[4] = pb[3] * mxi2;
+    pT[4] = pT[3] * mxi2;
+    const A tl = (A)2 * lam, K5 = K - (A)5, K3 = K - (A)3;
+    const A m6fn = mt6 * fnA, m6ft = tsign * (mt6 * ftA), m5t = tsign * mt5;
 #pragma unroll 2
     for (int k = lane; k < c.nn; k += WAVE) {
         const A u = unode[k], v = vnode[k];
@@ -769,36 +783,35 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
         const A sgn = d ? hlen : -hlen;
         const A hr = hu + sgn * sn;
         const A br = bu + sgn * bn;
-        const A st = tsign * stR, bt = tsign * btR;
         const A cu = vn - U, cv = vt - V;
         const A cp2 = cu * cu + cv * cv;
         const A c2 = vn * vn + vt * vt;
         const A H = rho_l_pi * fexp(-lam * cp2);
         const A Bm = H * kfac;
         const A cq = csA * (cu * q0 + cv * q1);
-        const A lam2 = (A)2 * lam * cp2;
-        const A Hp = cq * (lam2 + K - (A)5) * H;
-        const A Bp = cq * (lam2 + K - (A)3) * Bm;
-        const A eh = c2 * H + Bm, eb = c2 * Bm + mxi2 * H;
-        const A u0 = d ? a_L[0] : a_R[0], u1 = d ? a_L[1] : a_R[1];
-        const A u2 = d ? a_L[2] : a_R[2], u3 = d ? a_L[3] : a_R[3];
-        const A gh = u0 * H + u1 * vn * H + u2 * vt * H + (A)0.5 * u3 * eh;
-        const A gb = u0 * Bm + u1 * vn * Bm + u2 * vt * Bm + (A)0.5 * u3 * eb;
-        const A gbh = a_b[0] * H + a_b[1] * vn * H + a_b[2] * vt * H + (A)0.5 * a_b[3] * eh;
-        const A gbb = a_b[0] * Bm + a_b[1] * vn * Bm + a_b[2] * vt * Bm + (A)0.5 * a_b[3] * eb;
-        const A gTh = a_T[0] * H + a_T[1] * vn * H + a_T[2] * vt * H + (A)0.5 * a_T[3] * eh;
-        const A gTb = a_T[0] * Bm + a_T[1] * vn * Bm + a_T[2] * vt * Bm + (A)0.5 * a_T[3] * eb;
-        A xh = mt1 * vn * (H + Hp) + mt2 * vn * vn * gh + mt2 * vn * vt * gbh + mt3 * vn * gTh +
-               mt4 * vn * hr - mt5 * vn * vn * sn - mt5 * vn * vt * st;
-        A xb = mt1 * vn * (Bm + Bp) + mt2 * vn * vn * gb + mt2 * vn * vt * gbb + mt3 * vn * gTb +
-               mt4 * vn * br - mt5 * vn * vn * bn - mt5 * vn * vt * bt;
-        xh = xh - tm2fn * vn * gh + mt6 * fnA * vn * sn;
-        xb = xb - tm2fn * vn * gb + mt6 * fnA * vn * bn;
-        xh = xh - tm2ft * vn * gbh + mt6 * ftA * vn * st;
-        xb = xb - tm2ft * vn * gbb + mt6 * ftA * vn * bt;
+        const A l2 = tl * cp2;
+        const A Hp = cq * (l2 + K5) * H;
+        const A Bp = cq * (l2 + K3) * Bm;
+        const A s0 = d ? sL[0] : sR[0], s1 = d ? sL[1] : sR[1], s2 = d ? sL[2] : sR[2];
+        const A s3 = d ? sL[3] : sR[3], s4 = d ? sL[4] : sR[4];
+        const A Pu = s0 + s1 * vn + s2 * vt + s3 * c2;
+        const A Pb = pb[0] + pb[1] * vn + pb[2] * vt + pb[3] * c2;
+        const A PT = pT[0] + pT[1] * vn + pT[2] * vt + pT[3] * c2;
+        const A gh = H * Pu + s3 * Bm, gb = Bm * Pu + s4 * H;
+        const A gbh = H * Pb + pb[3] * Bm, gbb = Bm * Pb + pb[4] * H;
+        const A gTh = H * PT + pT[3] * Bm, gTb = Bm * PT + pT[4] * H;
+        // coefficients of g, g_b, the normal and the tangential slope
+        // (flux.f90:207-236 collected by term)
+        const A al = mt2 * vn - tm2fn, be = mt2 * vt - tm2ft;
+        const A ga = m6fn - mt5 * vn, de = m6ft - m5t * vt;
+        const A fv = flenA * vn;
+        const A xh = mt1 * (H + Hp) + al * gh + be * gbh + mt3 * gTh + mt4 * hr + ga * sn +
+                     de * stR;
+        const A xb = mt1 * (Bm + Bp) + al * gb + be * gbb + mt3 * gTb + mt4 * br + ga * bn +
+                     de * btR;
         // streamed out: read back only by the cell update
-        __builtin_nontemporal_store((T)(flenA * xh), ofh + k);
-        __builtin_nontemporal_store((T)(flenA * xb), ofb + k);
+        __builtin_nontemporal_store((T)(fv * xh), ofh + k);
+        __builtin_nontemporal_store((T)(fv * xb), ofb + k);
     }
 }
 
