@@ -215,7 +215,6 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
     const int lane = threadIdx.x & (WAVE - 1);
     const double len = DIR == 0 ? c.dx : c.dy;
     const double hl = 0.5 * len;
-    const double tsign = DIR == 0 ? 1.0 : -1.0;
     const A vs = (A)vshift;
     double s[FSUM_N];
 #pragma unroll
@@ -231,7 +230,8 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
         const double sn = (double)snA, bn = (double)bnA;
         const double hf = (double)hu + sgn * sn;
         const double bf = (double)bu + sgn * bn;
-        const double st = tsign * (double)stA, bt = tsign * (double)btA;
+        // tangential slopes: the orientation sign is applied to their sums
+        const double st = (double)stA, bt = (double)btA;
         const double u = und[k], v = vnd[k], w = wnd[k];
         const double vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
         const double a = w * hf, e = w * sn, f = w * st, g = w * bf, p = w * bn, r = w * bt;
@@ -240,7 +240,6 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
         s[0] += a;
         s[1] += vn * a;
         s[2] += vt * a;
-        s[3] += c2 * a;
         s[4] += g;
         s[5] += f;
         s[6] += vn * f;
@@ -255,7 +254,6 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
         s[15] += vnvt * a;
         s[16] += vn2vt * e;
         s[17] += vt2vn * f;
-        s[18] += vnc2 * a;
         s[19] += vn * g;
         s[20] += (vn2 * c2) * e;
         s[21] += vn2 * p;
@@ -275,6 +273,15 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
         s[34] += vt2vn * a;
         s[35] += (vt2 * vt) * a;
         s[36] += vt * g;
+    }
+    // c^2 = vn^2 + vt^2: sum w c^2 hf and sum w vn c^2 hf from their parts
+    s[3] = s[12] + s[31];
+    s[18] = s[32] + s[34];
+    if (DIR == 1) {
+        // horizontal faces take the tangential slopes with the opposite sign
+        constexpr int tq[13] = {5, 6, 7, 8, 9, 11, 14, 17, 22, 23, 28, 29, 30};
+#pragma unroll
+        for (int t = 0; t < 13; ++t) s[tq[t]] = -s[tq[t]];
     }
     int q;
     const double x = wave_reduce_scatter<FSUM_N>(s, q);
