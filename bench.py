@@ -303,7 +303,22 @@ def bench_c4(a):
         tf, wf, _ = run_batch(f64)
         errs = np.array([field_errors(x, y) for x, y in zip(wm, wf)])
         errs_th = np.array([field_errors(x, y, thermal=True) for x, y in zip(wm, wf)])
-        out["fp64"] = {"value": N * a.steps / tf, "ms_per_step": 1e3 * tf / a.steps}
+        # the plain FP64 path: every case on its own, one after the other,
+        # with the library's default schedule
+        t_p64 = 0.0
+        for c in f64:
+            s = capi.Solver(c)
+            if a.warmup > 0:
+                s.advance(a.warmup)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.advance(a.steps)
+            torch.cuda.synchronize()
+            t_p64 += time.perf_counter() - t0
+            s.close()
+        out["fp64"] = {"value": N * a.steps / tf, "ms_per_step": 1e3 * tf / a.steps,
+                       "plain_sequential_value": N * a.steps / t_p64}
+        out["batched_mixed_over_plain_fp64"] = value * t_p64 / (N * a.steps)
         out["mixed_vs_fp64"] = {
             "speedup": value / out["fp64"]["value"],
             "max_rel_err_conserved": errs.max(axis=0).tolist(),
