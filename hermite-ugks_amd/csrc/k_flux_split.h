@@ -219,7 +219,10 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
     double s[FSUM_N];
 #pragma unroll
     for (int q = 0; q < FSUM_N; ++q) s[q] = 0.0;
-    constexpr int UNR = ST ? 1 : 2;   // the stencil holds ten loads per node
+    // nodes per iteration: one for FP32 storage (118 VGPRs, four waves per
+    // SIMD: 4.55 -> 4.38 ms at 512^2 cells, GH40) and for the stencil (ten
+    // loads per node), two for FP64 (6.52 ms against 6.68 with one)
+    constexpr int UNR = (ST || sizeof(T) == 4) ? 1 : 2;
 #pragma unroll UNR
     for (int k = lane; k < c.nn; k += WAVE) {
         // upwind choice in the node arithmetic, as the fused kernel makes it
@@ -780,7 +783,9 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
     pT[4] = pT[3] * mxi2;
     const A tl = (A)2 * lam, K5 = K - (A)5, K3 = K - (A)3;
     const A m6fn = mt6 * fnA, m6ft = tsign * (mt6 * ftA), m5t = tsign * mt5;
-#pragma unroll 2
+    // one node per iteration (62 VGPRs for FP32 storage: 4.55 -> 4.32 ms at
+    // 512^2 cells, GH40; FP64 unchanged)
+#pragma unroll 1
     for (int k = lane; k < c.nn; k += WAVE) {
         const A u = unode[k], v = vnode[k];
         const A vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
