@@ -56,7 +56,8 @@ extern "C" {
 #define HUGKS_FP64           0  /* everything FP64 (parity path)            */
 #define HUGKS_MIXED_FP32     1  /* FP32 f/flux, FP64 moments and W,
                                    FP32 Hermite contractions                 */
-#define HUGKS_MIXED_BF16X3   2  /* as MIXED_FP32, Hermite as bf16x3 MFMA     */
+#define HUGKS_MIXED_BF16X3   2  /* reserved for bf16x3 Hermite contractions;
+                                   runs as HUGKS_MIXED_FP32 today            */
 
 /* boundary kind per side (x-lo, x-hi, y-lo, y-hi) */
 #define HUGKS_BC_PERIODIC    0  /* 2d/src/boundary.f90:16-53                 */
