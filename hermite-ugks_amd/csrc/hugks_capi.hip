@@ -162,11 +162,8 @@ static hipError_t enqueue_step_on(hugks_handle* h, hipStream_t s, bool timed) {
 
 static hipError_t enqueue_step(hugks_handle* h) { return enqueue_step_on(h, h->stream, true); }
 
-static int read_state(hugks_handle* h, hugks_step_info* info) {
-    DevState s;
-    hipError_t e = hipMemcpyAsync(&s, h->L.st, sizeof(s), hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return hip_fail(h, e, "reading device state");
+// A case's step info and status from a host copy of its device state.
+static int state_info(hugks_handle* h, const DevState& s, hugks_step_info* info) {
     if (info) {
         memset(info, 0, sizeof(*info));
         info->steps = s.steps;
@@ -182,6 +179,13 @@ static int read_state(hugks_handle* h, hugks_step_info* info) {
     if (s.flags & 1) return fail(h, HUGKS_E_NUMERIC, "non-finite state or T <= 0 on the device");
     if (s.flags & 2) return fail(h, HUGKS_E_WALL, "zero outgoing wall mass coefficient");
     return HUGKS_OK;
+}
+static int read_state(hugks_handle* h, hugks_step_info* info) {
+    DevState s;
+    hipError_t e = hipMemcpyAsync(&s, h->L.st, sizeof(s), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "reading device state");
+    return state_info(h, s, info);
 }
 
 extern "C" {
@@ -895,6 +899,9 @@ struct hugks_batch {
     bool grouped = false;       // one grouped launch per kernel type (k_group.h)
     std::vector<GroupPlan> plans;   // one per storage type present
     std::vector<void*> gmem;    // per-case arguments and the plans' tables
+    DevState** d_sts = nullptr;     // the cases' device states (device array)
+    DevState* d_gather = nullptr;   // their copies side by side (device)
+    DevState* h_gather = nullptr;   // and on the host (pinned)
     hipStream_t main = nullptr;
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> joins;
@@ -914,6 +921,9 @@ static void batch_release(hugks_batch* b) {
     for (hipStream_t s : b->lanes) hipStreamDestroy(s);
     if (b->main) hipStreamDestroy(b->main);
     for (void* p : b->gmem) hipFree(p);
+    if (b->d_sts) hipFree(b->d_sts);
+    if (b->d_gather) hipFree(b->d_gather);
+    if (b->h_gather) hipHostFree(b->h_gather);
     for (hugks_handle* h : b->cases) hugks_destroy(h);
     delete b;
 }
@@ -1082,6 +1092,15 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     // one step of every case, fanned out over a few streams and captured
     // into one graph: a replay advances the whole batch by one step
     hipError_t e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking);
+    // the cases' states: reset and read back for all cases at once
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_sts, n * sizeof(DevState*));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_gather, n * sizeof(DevState));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_gather, n * sizeof(DevState), 0);
+    if (e == hipSuccess) {
+        std::vector<DevState*> sts(n);
+        for (int32_t i = 0; i < n; ++i) sts[i] = b->cases[i]->L.st;
+        e = hipMemcpy(b->d_sts, sts.data(), n * sizeof(DevState*), hipMemcpyHostToDevice);
+    }
     int lanes = 8;
     if (const char* ls = getenv("HUGKS_BATCH_LANES")) lanes = atoi(ls) > 0 ? atoi(ls) : lanes;
     const int nl = n < lanes ? n : lanes;
@@ -1153,32 +1172,35 @@ int hugks_batch_handle(hugks_batch* b, int32_t i, hugks_handle** h) {
 
 int hugks_batch_advance(hugks_batch* b, int32_t max_steps, hugks_step_info* infos) {
     if (!b) return batch_fail(nullptr, HUGKS_E_ARG, "null batch");
-    const size_t n = b->cases.size();
-    hipError_t e = hipSuccess;
-    for (size_t i = 0; i < n && e == hipSuccess; ++i)
-        e = launch_begin(b->cases[i]->L.st, INFINITY, b->main);
+    const int n = (int)b->cases.size();
+    // every case's state to the host: one gather kernel and one copy
+    auto fetch = [&]() {
+        hipError_t r = launch_gather_states(b->d_sts, n, b->d_gather, b->main);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(b->h_gather, b->d_gather, n * sizeof(DevState),
+                               hipMemcpyDeviceToHost, b->main);
+        if (r == hipSuccess) r = hipStreamSynchronize(b->main);
+        return r;
+    };
+    hipError_t e = launch_begin_all(b->d_sts, n, INFINITY, b->main);
     if (max_steps > 0) {
         for (int32_t s = 0; s < max_steps && e == hipSuccess; ++s) e = hipGraphLaunch(b->exec, b->main);
     } else {
         // no cap: replay until every case met an exit condition
-        std::vector<DevState> st(n);
         for (;;) {
             for (int s = 0; s < 32 && e == hipSuccess; ++s) e = hipGraphLaunch(b->exec, b->main);
-            for (size_t i = 0; i < n && e == hipSuccess; ++i)
-                e = hipMemcpyAsync(&st[i], b->cases[i]->L.st, sizeof(DevState),
-                                   hipMemcpyDeviceToHost, b->main);
-            if (e == hipSuccess) e = hipStreamSynchronize(b->main);
+            if (e == hipSuccess) e = fetch();
             if (e != hipSuccess) break;
             bool all = true;
-            for (size_t i = 0; i < n; ++i) all = all && (st[i].stopped || st[i].flags);
+            for (int i = 0; i < n; ++i) all = all && (b->h_gather[i].stopped || b->h_gather[i].flags);
             if (all) break;
         }
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(b->main);
+    if (e == hipSuccess) e = fetch();
     if (e != hipSuccess) return batch_fail(b, HUGKS_E_DEVICE, std::string("batch advance: ") + hipGetErrorString(e));
     int first = HUGKS_OK;
-    for (size_t i = 0; i < n; ++i) {
-        int rc = read_state(b->cases[i], infos ? &infos[i] : nullptr);
+    for (int i = 0; i < n; ++i) {
+        int rc = state_info(b->cases[i], b->h_gather[i], infos ? &infos[i] : nullptr);
         if (rc != HUGKS_OK && first == HUGKS_OK) {
             first = rc;
             b->err = "case " + std::to_string(i) + ": " + b->cases[i]->err;

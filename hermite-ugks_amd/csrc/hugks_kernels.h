@@ -65,6 +65,9 @@ struct HookScope {
 };
 
 hipError_t launch_begin(DevState* st, double t_stop, hipStream_t s);
+// batch forms: the states of n cases (device array of pointers)
+hipError_t launch_begin_all(DevState* const* sts, int n, double t_stop, hipStream_t s);
+hipError_t launch_gather_states(DevState* const* sts, int n, DevState* out, hipStream_t s);
 hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s);
 hipError_t launch_flux2d(const Launch2D& L, hipStream_t s);
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s);

@@ -46,6 +46,23 @@ __global__ void k_begin(DevState* st, double t_stop) {
         st->t_stop = t_stop;
     }
 }
+// k_begin for every case of a batch, one thread per case.
+__global__ void k_begin_all(DevState* const* __restrict__ sts, int n, double t_stop) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        DevState* st = sts[i];
+        st->stopped = 0;
+        st->stop_reason = 0;
+        st->steps = 0;
+        st->t_stop = t_stop;
+    }
+}
+// The states of a batch's cases side by side, for one copy to the host.
+__global__ void k_gather_states(DevState* const* __restrict__ sts, int n,
+                                DevState* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *sts[i];
+}
 #endif
 
 // Signal rate of one cell for the time step (timestep, 2d/src/solver.f90:

@@ -26,6 +26,16 @@ hipError_t launch_begin(DevState* st, double t_stop, hipStream_t s) {
     hipLaunchKernelGGL((k_begin), dim3(1), dim3(64), 0, s, st, t_stop);
     return hipGetLastError();
 }
+hipError_t launch_begin_all(DevState* const* sts, int n, double t_stop, hipStream_t s) {
+    hipLaunchKernelGGL((k_begin_all), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sts, n,
+                       t_stop);
+    return hipGetLastError();
+}
+hipError_t launch_gather_states(DevState* const* sts, int n, DevState* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_gather_states), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sts,
+                       n, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_timestep(const Launch2D& L, int dim, hipStream_t s) {
     HookScope hs(L, "timestep");
