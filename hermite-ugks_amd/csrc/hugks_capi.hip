@@ -339,6 +339,8 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     ALLOC(un_d, ldv * sizeof(double));
     ALLOC(vn_d, ldv * sizeof(double));
     ALLOC(wn_d, ldv * sizeof(double));
+    double* n4_d;
+    ALLOC(n4_d, ldv * 4 * sizeof(double));
     ALLOC(u1, d->nv * sizeof(double));
     ALLOC(v1, d->nv * sizeof(double));
     ALLOC(prim0_d, ncell * (dim + 2) * sizeof(double));
@@ -358,10 +360,19 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     hipMemcpy(un_d, un.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(vn_d, vn.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(wn_d, wn.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
+    {
+        std::vector<double> n4(4 * (size_t)ldv, 0.0);
+        for (int k = 0; k < ldv; ++k) {
+            n4[4 * (size_t)k] = un[k];
+            n4[4 * (size_t)k + 1] = vn[k];
+            n4[4 * (size_t)k + 2] = wn[k];
+        }
+        hipMemcpy(n4_d, n4.data(), 4 * (size_t)ldv * sizeof(double), hipMemcpyHostToDevice);
+    }
     hipMemcpy(u1, d->u, d->nv * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(v1, vtab.data(), d->nv * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(prim0_d, d->prim0, ncell * (dim + 2) * sizeof(double), hipMemcpyHostToDevice);
-    L.unode_d = un_d; L.vnode_d = vn_d; L.wnode_d = wn_d; L.u1 = u1; L.v1 = v1;
+    L.unode_d = un_d; L.vnode_d = vn_d; L.wnode_d = wn_d; L.u1 = u1; L.v1 = v1; L.node4 = n4_d;
     if (h->fp64) {
         L.unode = un_d; L.vnode = vn_d; L.wnode = wn_d;
     } else {

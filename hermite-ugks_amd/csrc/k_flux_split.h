@@ -204,13 +204,33 @@ __device__ __forceinline__ Upwind<T, ST> make_upwind(const Consts& c, const Face
 }
 
 // ------------------------------------------------------------------ K1
+// FP64 velocity tables of K1: node k's u, v and weight, from 32-byte
+// records {u, v, w, 0} on the FP32-storage path (two loads per node
+// instead of three: 4.37 -> 4.20 ms at 512^2 cells, GH40) and from three
+// arrays on the FP64 path (the records measured 6.5 -> 7.5 ms there).
+struct alignas(32) NodeRec {
+    double u, v, w, pad;
+};
+struct NodeTabs {
+    const double *u, *v, *w;   // per node
+    const double* rec;         // NodeRec per node
+    template <typename T>
+    __device__ __forceinline__ void get(int k, double& uk, double& vk, double& wk) const {
+        if constexpr (sizeof(T) == 4) {
+            const NodeRec r = *reinterpret_cast<const NodeRec*>(rec + 4 * (size_t)k);
+            uk = r.u;
+            vk = r.v;
+            wk = r.w;
+        } else {
+            uk = u[k];
+            vk = v[k];
+            wk = w[k];
+        }
+    }
+};
 template <int DIR, typename T, bool ST>
 __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const Upwind<T, ST>& up,
-                                          const typename Arith<T>::A* __restrict__ unode,
-                                          const typename Arith<T>::A* __restrict__ vnode,
-                                          const double* __restrict__ und,
-                                          const double* __restrict__ vnd,
-                                          const double* __restrict__ wnd, double* __restrict__ out) {
+                                          const NodeTabs& nt, double* __restrict__ out) {
     using A = typename Arith<T>::A;
     const int lane = threadIdx.x & (WAVE - 1);
     const double len = DIR == 0 ? c.dx : c.dy;
@@ -225,8 +245,10 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
     constexpr int UNR = (ST || sizeof(T) == 4) ? 1 : 2;
 #pragma unroll UNR
     for (int k = lane; k < c.nn; k += WAVE) {
+        double u, v, w;
+        nt.get<T>(k, u, v, w);
         // upwind choice in the node arithmetic, as the fused kernel makes it
-        const int d = upwind_delta((DIR == 0 ? unode[k] : vnode[k]) - vs);
+        const int d = upwind_delta((A)(DIR == 0 ? u : v) - vs);
         A hu, bu, snA, stA, bnA, btA;
         up.get(d, k, hu, bu, snA, stA, bnA, btA);
         const double sgn = d ? hl : -hl;
@@ -235,7 +257,6 @@ __device__ __forceinline__ void fsum_face(const Consts& c, double vshift, const 
         const double bf = (double)bu + sgn * bn;
         // tangential slopes: the orientation sign is applied to their sums
         const double st = (double)stA, bt = (double)btA;
-        const double u = und[k], v = vnd[k], w = wnd[k];
         const double vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
         const double a = w * hf, e = w * sn, f = w * st, g = w * bf, p = w * bn, r = w * bt;
         const double vn2 = vn * vn, vt2 = vt * vt, c2 = vn2 + vt2, vnvt = vn * vt;
@@ -295,9 +316,8 @@ template <typename T, bool ST>
 __device__ __forceinline__ void k_fsum_body(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
-    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ und,
-    const double* __restrict__ vnd, const double* __restrict__ wnd,
+    const T* __restrict__ sby, const double* __restrict__ und, const double* __restrict__ vnd,
+    const double* __restrict__ wnd, const double* __restrict__ node4,
     const double* __restrict__ acc_v, const double* __restrict__ acc_h,
     double* __restrict__ fsum, int nvi, int nint, int bid, int nblk) {
     if (st->stopped) return;
@@ -307,27 +327,25 @@ __device__ __forceinline__ void k_fsum_body(
     const FaceGeo fg = face_geo(c, g, nvi);
     const double dt = st->dt;
     const Upwind<T, ST> up = make_upwind<T, ST>(c, fg, h, b, shx, shy, sbx, sby);
+    const NodeTabs nt{und, vnd, wnd, node4};
     if (fg.dir == 0) {
         const double vshift = 0.5 * acc_v[2 * fg.face] * dt;
-        fsum_face<0, T, ST>(c, vshift, up, unode, vnode, und, vnd, wnd,
-                            fsum + (size_t)g * FSUM_LD);
+        fsum_face<0, T, ST>(c, vshift, up, nt, fsum + (size_t)g * FSUM_LD);
     } else {
         const double vshift = 0.5 * acc_h[2 * fg.face] * dt;
-        fsum_face<1, T, ST>(c, vshift, up, unode, vnode, und, vnd, wnd,
-                            fsum + (size_t)g * FSUM_LD);
+        fsum_face<1, T, ST>(c, vshift, up, nt, fsum + (size_t)g * FSUM_LD);
     }
 }
 template <typename T, bool ST>
 __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
-    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ und,
-    const double* __restrict__ vnd, const double* __restrict__ wnd,
+    const T* __restrict__ sby, const double* __restrict__ und, const double* __restrict__ vnd,
+    const double* __restrict__ wnd, const double* __restrict__ node4,
     const double* __restrict__ acc_v, const double* __restrict__ acc_h,
     double* __restrict__ fsum, int nvi, int nint) {
-    k_fsum_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, und, vnd, wnd, acc_v, acc_h,
-        fsum, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
+    k_fsum_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, und, vnd, wnd, node4, acc_v, acc_h, fsum,
+        nvi, nint, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ------------------------------------------------------------------ K2

@@ -67,9 +67,9 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void kg_fsum(const int* __restric
     int nvf, nint;
     face_counts(L.c, nvf, nint);
     k_fsum_body<T, false>(L.c, L.st, (const T*)L.h, (const T*)L.b, (const T*)L.shx,
-                          (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby,
-                          (const A*)L.unode, (const A*)L.vnode, L.unode_d, L.vnode_d, L.wnode_d,
-                          L.acc_v, L.acc_h, L.fsum, nvf, nint, g.bid, g.nblk);
+                          (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby, L.unode_d,
+                          L.vnode_d, L.wnode_d, L.node4, L.acc_v, L.acc_h, L.fsum, nvf, nint,
+                          g.bid, g.nblk);
 }
 
 __global__ __launch_bounds__(256) void kg_fface(const int* __restrict__ tab,

@@ -42,8 +42,8 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
             auto kf = stn ? k_fsum<T, true> : k_fsum<T, false>;
             hipLaunchKernelGGL(kf, dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
                                (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
-                               (const T*)L.sbx, (const T*)L.sby, un, vn, L.unode_d, L.vnode_d,
-                               L.wnode_d, L.acc_v, L.acc_h, L.fsum, nvf, nint);
+                               (const T*)L.sbx, (const T*)L.sby, L.unode_d, L.vnode_d,
+                               L.wnode_d, L.node4, L.acc_v, L.acc_h, L.fsum, nvf, nint);
         }
         if (L.wx) {
             // tensor-product quadrature: face state, Maxwellian sums and
