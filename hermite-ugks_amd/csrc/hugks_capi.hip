@@ -373,6 +373,21 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     hipMemcpy(v1, vtab.data(), d->nv * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(prim0_d, d->prim0, ncell * (dim + 2) * sizeof(double), hipMemcpyHostToDevice);
     L.unode_d = un_d; L.vnode_d = vn_d; L.wnode_d = wn_d; L.u1 = u1; L.v1 = v1; L.node4 = n4_d;
+    {
+        void* uv;
+        const size_t es_a = h->fp64 ? sizeof(double) : sizeof(float);
+        ALLOC(uv, 2 * (size_t)ldv * es_a);
+        if (h->fp64) {
+            std::vector<double> t(2 * (size_t)ldv);
+            for (int k = 0; k < ldv; ++k) { t[2 * k] = un[k]; t[2 * k + 1] = vn[k]; }
+            hipMemcpy(uv, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
+        } else {
+            std::vector<float> t(2 * (size_t)ldv);
+            for (int k = 0; k < ldv; ++k) { t[2 * k] = (float)un[k]; t[2 * k + 1] = (float)vn[k]; }
+            hipMemcpy(uv, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+        }
+        L.uvnode = uv;
+    }
     if (h->fp64) {
         L.unode = un_d; L.vnode = vn_d; L.wnode = wn_d;
     } else {

@@ -24,6 +24,7 @@ struct Launch2D {
     const void *unode, *vnode, *wnode;       // per-node tables in the arithmetic type
     const double *unode_d, *vnode_d, *wnode_d;  // FP64 tables (moments)
     const double* node4;     // per node {u, v, w, 0} (FP64), 32-byte records
+    const void* uvnode;      // per node {u, v} in the arithmetic type
     const double *u1, *v1;                   // 1D node lists [nv] (double)
     const double *wx, *wy;   // weight factors w[i + nv j] = wx[i] wy[j]; null if the
                              // weights do not factor

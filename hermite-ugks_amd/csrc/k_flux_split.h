@@ -755,8 +755,7 @@ __global__ __launch_bounds__(256) void k_fface(Consts c, const DevState* __restr
 template <int DIR, typename T, bool ST>
 __device__ __forceinline__ void fnode_face(const Consts& c, const double* __restrict__ fc,
                                            double fn, double ft, const Upwind<T, ST>& up,
-                                           const typename Arith<T>::A* __restrict__ unode,
-                                           const typename Arith<T>::A* __restrict__ vnode,
+                                           const typename Arith<T>::A* __restrict__ uvnode,
                                            double dt, T* __restrict__ ofh, T* __restrict__ ofb) {
     using A = typename Arith<T>::A;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -805,7 +804,10 @@ __device__ __forceinline__ void fnode_face(const Consts& c, const double* __rest
     // 512^2 cells, GH40; FP64 unchanged)
 #pragma unroll 1
     for (int k = lane; k < c.nn; k += WAVE) {
-        const A u = unode[k], v = vnode[k];
+        // node k's (u, v) in the node arithmetic, one 2-element load
+        struct alignas(2 * sizeof(A)) UV { A u, v; };
+        const UV uv = *reinterpret_cast<const UV*>(uvnode + 2 * k);
+        const A u = uv.u, v = uv.v;
         const A vn = DIR == 0 ? u : v, vt = DIR == 0 ? v : -u;
         const int d = upwind_delta(vn - vshift);
         A hu, bu, sn, stR, bn, btR;
@@ -849,8 +851,8 @@ template <typename T, bool ST>
 __device__ __forceinline__ void k_fnode_body(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
-    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ acc_v,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ uvnode,
+    const double* __restrict__ acc_v,
     const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
     T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint, int bid,
     int nblk) {
@@ -864,11 +866,11 @@ __device__ __forceinline__ void k_fnode_body(
     const Upwind<T, ST> up = make_upwind<T, ST>(c, fg, h, b, shx, shy, sbx, sby);
     if (fg.dir == 0) {
         const double fn = acc_v[2 * fg.face], ft = acc_v[2 * fg.face + 1];
-        fnode_face<0, T, ST>(c, fc, fn, ft, up, unode, vnode, dt,
+        fnode_face<0, T, ST>(c, fc, fn, ft, up, uvnode, dt,
                              vfh + (size_t)fg.face * c.ldv, vfb + (size_t)fg.face * c.ldv);
     } else {
         const double fn = acc_h[2 * fg.face], ft = acc_h[2 * fg.face + 1];
-        fnode_face<1, T, ST>(c, fc, fn, ft, up, unode, vnode, dt,
+        fnode_face<1, T, ST>(c, fc, fn, ft, up, uvnode, dt,
                              hfh + (size_t)fg.face * c.ldv, hfb + (size_t)fg.face * c.ldv);
     }
 }
@@ -876,11 +878,11 @@ template <typename T, bool ST>
 __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ unode,
-    const typename Arith<T>::A* __restrict__ vnode, const double* __restrict__ acc_v,
+    const T* __restrict__ sby, const typename Arith<T>::A* __restrict__ uvnode,
+    const double* __restrict__ acc_v,
     const double* __restrict__ acc_h, const double* __restrict__ fcoef, T* __restrict__ vfh,
     T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint) {
-    k_fnode_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, unode, vnode, acc_v, acc_h, fcoef, vfh,
+    k_fnode_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, uvnode, acc_v, acc_h, fcoef, vfh,
         vfb, hfh, hfb, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
 }
 

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void kg_fnode(const int* __restri
     face_counts(L.c, nvf, nint);
     k_fnode_body<T, false>(L.c, L.st, (const T*)L.h, (const T*)L.b, (const T*)L.shx,
                            (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby,
-                           (const A*)L.unode, (const A*)L.vnode, L.acc_v, L.acc_h, L.fcoef,
+                           (const A*)L.uvnode, L.acc_v, L.acc_h, L.fcoef,
                            (T*)L.vfh, (T*)L.vfb, (T*)L.hfh, (T*)L.hfb, nvf, nint, g.bid, g.nblk);
 }
 

@@ -75,8 +75,9 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
             auto kn = stn ? k_fnode<T, true> : k_fnode<T, false>;
             hipLaunchKernelGGL(kn, dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
                                (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
-                               (const T*)L.sbx, (const T*)L.sby, un, vn, L.acc_v, L.acc_h,
-                               L.fcoef, (T*)L.vfh, (T*)L.vfb, (T*)L.hfh, (T*)L.hfb, nvf, nint);
+                               (const T*)L.sbx, (const T*)L.sby, (const A*)L.uvnode, L.acc_v,
+                               L.acc_h, L.fcoef, (T*)L.vfh, (T*)L.vfb, (T*)L.hfh, (T*)L.hfb, nvf,
+                               nint);
         }
     } else if (nvf > 0) {
         HookScope hs(L, "flux_vface");
