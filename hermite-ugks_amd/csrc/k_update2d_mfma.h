@@ -349,7 +349,7 @@ __device__ __forceinline__ void k_update2d_mfma_body(
             q4[1] += cu * wb;
             q4[2] += cv * c2 * wh;
             q4[3] += cv * wb;
-            const A wa = wnode[k];
+            const A wa = (A)wt;   // the A-type weight table holds (A)wnode_d
             G[jv * ldG + iu] = (T)(wa * (A)rh[e]);
             G[GS + jv * ldG + iu] = (T)(wa * (A)rb[e]);
         }
