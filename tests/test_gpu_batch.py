@@ -85,12 +85,16 @@ def test_batch_autotuned_parity_with_oracle(have_gpu, monkeypatch):
         assert np.all(err <= tol), err
 
 
-def test_batch_cases_stop_on_their_own(have_gpu):
+@pytest.mark.parametrize("grouped", [True, False])
+def test_batch_cases_stop_on_their_own(have_gpu, monkeypatch, grouped):
     """Cases with different maxtime: the uncapped advance runs until every
     case has stopped; each stops at its own maxtime after the same number
-    of steps as when run alone."""
+    of steps as when run alone (grouped launches and per-case graph)."""
     cs = [cases.build_2d(poiseuille(nv=10, ny=8, maxtime=t), capi.FP64) for t in (0.02, 0.05, 0.035)]
+    if not grouped:
+        monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
     with capi.Batch(cs) as b:
+        assert b.grouped == grouped
         infos = b.advance(0)
     for c, inf in zip(cs, infos):
         with capi.Solver(c) as s:
