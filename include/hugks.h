@@ -172,7 +172,7 @@ int hugks_step(hugks_handle *h, hugks_step_info *info);
 int hugks_advance(hugks_handle *h, int32_t max_steps, hugks_step_info *info);
 
 /* As hugks_advance, and also stops after the first step that ends with
- * simtime >= t_stop: the driver's output points (2d/src/main.f90:88-99,
+ * simtime >= t_stop: the main program's output points (2d/src/main.f90:88-99,
  * write_macro at time2/time3/time4) without a per-step host round trip. */
 int hugks_advance_until(hugks_handle *h, int32_t max_steps, double t_stop,
                         hugks_step_info *info);

@@ -1,8 +1,8 @@
 """ctypes binding of the C oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke()
-and by the CPU-baseline leg of bench.py, as the checker.  The product does
-not import it.
+and by the CPU-baseline leg of bench.py, to check results against.  The
+product does not import it.
 """
 import ctypes as C
 import os
