@@ -92,20 +92,22 @@ def test_rt_512_properties(have_gpu):
     assert np.all(err <= 1e-5), err
 
 
-def test_arrays_over_4gib_match_narrow_channel(have_gpu):
+@pytest.mark.parametrize("prec", [capi.MIXED_FP32, capi.FP64])
+def test_arrays_over_4gib_match_narrow_channel(have_gpu, prec):
     """Distribution arrays past 4 GiB take the 64-bit addressing of the face
     kernels (32-bit buffer offsets cover smaller ones).  A channel flow that
-    is uniform along x, 11000 x 64 cells at GH40 in FP32 storage (4.5 GB per
-    array), must give in every column exactly what the same channel 4 cells
-    wide gives (x-face fluxes cancel exactly between equal cells)."""
+    is uniform along x, 11000 x 64 cells at GH40 (4.5 GB per array in FP32
+    storage, 9 GB in FP64), must give in every column exactly what the same
+    channel 4 cells wide gives (x-face fluxes cancel exactly between equal
+    cells)."""
     p = dict(nx=4, ny=64, xlength=0.1, ylength=1.0, nv=40, vmax=5.0, Tref=1.0,
              quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=8, cfl=0.8,
              maxtime=200.0, case_id=5, knudsen=0.1, phi=0.05, TwL=1.0, TwR=1.2)
     steps = 3
-    narrow = cases.build_2d(p, capi.MIXED_FP32)
+    narrow = cases.build_2d(p, prec)
     _, _, wn = gpu_run(narrow, steps)
     wide_p = dict(p, nx=11000, xlength=0.1 * 11000 / 4)
-    wide = cases.build_2d(wide_p, capi.MIXED_FP32)
+    wide = cases.build_2d(wide_p, prec)
     assert wide.ncell * wide.desc.nv ** 2 * 4 > 2 ** 32
     _, _, ww = gpu_run(wide, steps)
     wn = wn.reshape(64, 4, -1)
