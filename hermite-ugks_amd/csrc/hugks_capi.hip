@@ -922,7 +922,8 @@ int hugks_tune(const hugks_desc* d, int32_t* flux_kernel, int32_t* update_kernel
 
 struct hugks_batch {
     std::vector<hugks_handle*> cases;
-    bool grouped = false;       // one grouped launch per kernel type (k_group.h)
+    bool grouped = false;       // some cases run in grouped launches (k_group.h)
+    std::vector<char> in_group; // per case: stepped by the grouped launches
     std::vector<GroupPlan> plans;   // one per storage type present
     std::vector<void*> gmem;    // per-case arguments and the plans' tables
     DevState** d_sts = nullptr;     // the cases' device states (device array)
@@ -962,19 +963,26 @@ static int batch_fail(hugks_batch* b, int code, const std::string& msg) {
 
 extern "C" {
 
-// A batch runs grouped when every case is a small 2D case whose step can be
-// the split flux with factoring weights and the MFMA update: then one launch
-// per kernel type and storage type covers all cases.
-static bool batch_groupable(const hugks_batch* b) {
-    if (const char* g = getenv("HUGKS_BATCH_GROUPED"))
-        if (strcmp(g, "0") == 0) return false;
-    for (const hugks_handle* h : b->cases) {
-        const Launch2D& L = h->L;
-        if (h->dim != 2 || !L.wx || !update2d_mfma_supported(L.c) ||
-            L.c.nx * L.c.ny >= PRED_MIN_CELLS)
-            return false;
+// The small 2D cases whose step can be the split flux with factoring
+// weights and the MFMA update run in grouped launches (one launch per
+// kernel type and storage type for all of them) when there are at least two;
+// the other cases keep their own launches on the lane streams.
+static bool case_groupable(const hugks_handle* h) {
+    const Launch2D& L = h->L;
+    return h->dim == 2 && L.wx && update2d_mfma_supported(L.c) &&
+           L.c.nx * L.c.ny < PRED_MIN_CELLS;
+}
+static std::vector<char> batch_groupable(const hugks_batch* b) {
+    std::vector<char> g(b->cases.size(), 0);
+    if (const char* e = getenv("HUGKS_BATCH_GROUPED"))
+        if (strcmp(e, "0") == 0) return g;
+    int count = 0;
+    for (size_t i = 0; i < b->cases.size(); ++i) {
+        g[i] = case_groupable(b->cases[i]) ? 1 : 0;
+        count += g[i];
     }
-    return true;
+    if (count < 2) std::fill(g.begin(), g.end(), 0);
+    return g;
 }
 
 // Plan of the grouped launches of the cases ids (one storage type): tables
@@ -1078,9 +1086,11 @@ static hipError_t batch_group_plan(hugks_batch* b) {
     std::vector<Launch2D> Ls(n);
     std::vector<int> ids[2];
     for (int i = 0; i < n; ++i) {
-        hugks_set_schedule(b->cases[i], HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA);
+        if (b->in_group[i]) {
+            hugks_set_schedule(b->cases[i], HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA);
+            ids[b->cases[i]->fp64 ? 1 : 0].push_back(i);
+        }
         Ls[i] = b->cases[i]->L;
-        ids[b->cases[i]->fp64 ? 1 : 0].push_back(i);
     }
     e = hipMemcpy(lmem, Ls.data(), n * sizeof(Launch2D), hipMemcpyHostToDevice);
     for (int t = 0; t < 2 && e == hipSuccess; ++t)
@@ -1102,10 +1112,11 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
         }
         b->cases.push_back(h);
     }
-    // a grouped batch runs its own schedule (split flux, MFMA update); the
-    // tuner picks the schedules of a batch stepped case by case
-    const bool groupable = batch_groupable(b);
-    for (int32_t i = 0; i < n && (flags & HUGKS_BATCH_AUTOTUNE) && !groupable; ++i) {
+    // grouped cases run their own schedule (split flux, MFMA update); the
+    // tuner picks the schedules of the cases stepped on their own
+    b->in_group = batch_groupable(b);
+    for (int32_t i = 0; i < n && (flags & HUGKS_BATCH_AUTOTUNE); ++i) {
+        if (b->in_group[i]) continue;
         int fk = -1, uk = -1;
         int rc = tune_case(&descs[i], &fk, &uk, nullptr);
         if (rc != HUGKS_OK) {
@@ -1142,22 +1153,27 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     // cases finish their own set-up on their streams first
     for (hugks_handle* h : b->cases)
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    b->grouped = e == hipSuccess && groupable;
+    int nsep = 0;   // cases with their own launches
+    for (int32_t i = 0; i < n; ++i) nsep += b->in_group[i] ? 0 : 1;
+    b->grouped = e == hipSuccess && nsep < n;
     if (b->grouped) e = batch_group_plan(b);
     if (e == hipSuccess) e = hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal);
-    if (b->grouped) {
-        for (size_t q = 0; q < b->plans.size() && e == hipSuccess; ++q)
-            e = launch_group_step(b->plans[q], b->main);
-    } else {
+    // cases with their own launches on the lanes, forked from the batch
+    // stream; the grouped launches on the batch stream beside them
+    const int nlu = nsep < nl ? nsep : nl;
+    if (nlu > 0) {
         if (e == hipSuccess) e = hipEventRecord(b->fork, b->main);
-        for (int l = 0; l < nl && e == hipSuccess; ++l)
+        for (int l = 0; l < nlu && e == hipSuccess; ++l)
             e = hipStreamWaitEvent(b->lanes[l], b->fork, 0);
+        int q = 0;
         for (int32_t i = 0; i < n && e == hipSuccess; ++i)
-            e = enqueue_step_on(b->cases[i], b->lanes[i % nl], false);
-        for (int l = 0; l < nl && e == hipSuccess; ++l) {
-            e = hipEventRecord(b->joins[l], b->lanes[l]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->joins[l], 0);
-        }
+            if (!b->in_group[i]) e = enqueue_step_on(b->cases[i], b->lanes[q++ % nlu], false);
+    }
+    for (size_t q = 0; q < b->plans.size() && e == hipSuccess; ++q)
+        e = launch_group_step(b->plans[q], b->main);
+    for (int l = 0; l < nlu && e == hipSuccess; ++l) {
+        e = hipEventRecord(b->joins[l], b->lanes[l]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->joins[l], 0);
     }
     if (e == hipSuccess) {
         e = hipStreamEndCapture(b->main, &b->graph);
@@ -1187,7 +1203,12 @@ const char* hugks_batch_last_error(const hugks_batch* b) {
 
 int32_t hugks_batch_size(const hugks_batch* b) { return b ? (int32_t)b->cases.size() : 0; }
 
-int32_t hugks_batch_grouped(const hugks_batch* b) { return b && b->grouped ? 1 : 0; }
+int32_t hugks_batch_grouped(const hugks_batch* b) {
+    if (!b) return 0;
+    int32_t k = 0;
+    for (char g : b->in_group) k += g ? 1 : 0;
+    return k;
+}
 
 int hugks_batch_handle(hugks_batch* b, int32_t i, hugks_handle** h) {
     if (!b || !h || i < 0 || i >= (int32_t)b->cases.size())

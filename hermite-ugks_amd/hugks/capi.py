@@ -337,8 +337,13 @@ class Batch:
 
     @property
     def grouped(self):
-        """True when one grouped launch per kernel type runs every case."""
-        return bool(self.lib.hugks_batch_grouped(self.b))
+        """True when grouped launches (one per kernel type) run every case."""
+        return self.grouped_cases == len(self.solvers)
+
+    @property
+    def grouped_cases(self):
+        """Number of cases stepped by the grouped launches."""
+        return int(self.lib.hugks_batch_grouped(self.b))
 
     def advance(self, max_steps):
         infos = (StepInfo * len(self.cases))()

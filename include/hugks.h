@@ -236,9 +236,9 @@ int hugks_tune(const hugks_desc *d, int32_t *flux_kernel, int32_t *update_kernel
  * velocity sizes).  The reference runs each case as its own process; here
  * one step of every case is captured into one HIP graph, whose replay
  * advances the whole batch by a step, each case stopping on its own exit
- * conditions.  With HUGKS_BATCH_AUTOTUNE each case of a batch that is not
- * grouped (hugks_batch_grouped) gets the schedule hugks_tune picks for its
- * shape.  hugks_batch_handle gives the case's
+ * conditions.  With HUGKS_BATCH_AUTOTUNE each case that is not in the
+ * grouped launches (hugks_batch_grouped) gets the schedule hugks_tune picks
+ * for its shape.  hugks_batch_handle gives the case's
  * handle for hugks_get_macro and the other per-case calls (not for
  * hugks_step/advance while the batch is in use). */
 #define HUGKS_BATCH_AUTOTUNE 1
@@ -248,10 +248,11 @@ int hugks_batch_create(int32_t n, const hugks_desc *descs, int32_t flags,
 int hugks_batch_destroy(hugks_batch *b);
 const char *hugks_batch_last_error(const hugks_batch *b);
 int32_t hugks_batch_size(const hugks_batch *b);
-/* 1 when a step of the batch is one grouped launch per kernel type and
- * storage type (all cases small 2D cases with factoring weights and an
- * MFMA-capable grid; they then run the split flux and the MFMA update),
- * 0 when each case's kernels are launched on their own. */
+/* Number of cases stepped by grouped launches: one launch per kernel type
+ * and storage type covers all small 2D cases with factoring weights and an
+ * MFMA-capable grid (at least two of them; they then run the split flux
+ * and the MFMA update); the other cases' kernels are launched on their own
+ * beside them.  0 when every case runs on its own. */
 int32_t hugks_batch_grouped(const hugks_batch *b);
 int hugks_batch_handle(hugks_batch *b, int32_t i, hugks_handle **h);
 /* max_steps replays (<= 0: until every case stopped); infos[n] receives

@@ -178,3 +178,31 @@ def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
             alone = s.advance(0)
         assert inf.stop_reason == capi.STOP_MAXTIME
         assert inf.steps == alone.steps and inf.simtime == alone.simtime
+
+
+def test_small_cases_grouped_beside_a_large_one(have_gpu):
+    """A batch with one case too large for the grouped launches: the small
+    cases run grouped, the large one with its own launches beside them, and
+    every case matches, bitwise and on every replay, the same case stepped
+    alone with the schedule the batch gave it."""
+    cs = grouped_cases(capi.MIXED_FP32) + [
+        cases.build_2d(rt(nx=130, ny=130, nv=12, nm=6), capi.MIXED_FP32, rt_variant="cartesian",
+                       seed=5)]
+    with capi.Batch(cs) as b:
+        print("grouped cases", b.grouped_cases, "of", len(cs), flush=True)
+        assert b.grouped_cases == len(cs) - 1 and not b.grouped
+        singles = []
+        for c, s in zip(cs, b.solvers):
+            x = capi.Solver(c)
+            x.set_schedule(*s.schedule())
+            singles.append(x)
+        for rep, n in enumerate((2, 1)):
+            infos = b.advance(n)
+            assert [i.steps for i in infos] == [n] * len(cs)
+            for k, s in enumerate(singles):
+                s.advance(n)
+                wa = s.macro(with_q=False)
+                wb = b.solvers[k].macro(with_q=False)
+                assert np.array_equal(wa, wb), (rep, k, np.max(np.abs(wa - wb)))
+        for s in singles:
+            s.close()
