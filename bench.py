@@ -240,8 +240,9 @@ def c4_params(seed=7, n=64):
 
 def bench_c4(a):
     """Aggregate updates/s of the batched sweep: all cases advanced by one
-    graph replay per step (autotuned schedules), against the same cases
-    stepped one after the other, and the mixed path against FP64 per case."""
+    graph replay per step (grouped launches for the small cases, tuned
+    schedules for the others), against the same cases stepped one after the
+    other, and the mixed path against FP64 per case."""
     import torch
     from metrics import field_errors
     plist = c4_params()
@@ -260,10 +261,12 @@ def bench_c4(a):
             raise RuntimeError("a batch case stopped inside the timed region")
         ws = [s.macro(with_q=False) for s in b.solvers]
         scheds = [s.schedule() for s in b.solvers]
+        run_batch.grouped = b.grouped_cases
         b.close()
         return t1 - t0, ws, scheds
 
     tm, wm, scheds = run_batch(mixed)
+    grouped = run_batch.grouped
     value = N * a.steps / tm
     # the same cases one after the other (eager launches), for comparison;
     # per-kernel times from the HIP-event hooks give the roofline
@@ -339,6 +342,7 @@ def bench_c4(a):
         "sequential_value": N * a.steps / t_seq,
         "batch_over_sequential": t_seq / tm,
         "schedules": {f"flux{f}_update{u}": n for (f, u), n in Counter(scheds).items()},
+        "grouped_cases": grouped,
     }
     line.update(out)
     print(json.dumps(line))
