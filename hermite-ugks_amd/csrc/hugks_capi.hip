@@ -50,7 +50,42 @@ struct hugks_handle {
     hipEvent_t open_ev = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t ncell = 0;
+    uint64_t run_hash = 0;   // of the inputs that define the run (checkpoints)
 };
+
+// FNV-1a, 64-bit, over byte ranges.
+static uint64_t fnv1a(uint64_t hsh, const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        hsh ^= c[i];
+        hsh *= 1099511628211ull;
+    }
+    return hsh;
+}
+
+// Hash of what defines a run besides its shape: velocity nodes, weights,
+// accelerations, Hermite order and the physical constants and boundary
+// conditions.  Loop controls (cfl, maxtime, residual and steady-state
+// stops) are left out: a restored run may continue under other ones.
+static uint64_t run_hash_of(const hugks_desc* d, size_t ncell) {
+    uint64_t r = 1469598103934665603ull;
+    const int nn = d->dim == 2 ? d->nv * d->nv : d->nv;
+    r = fnv1a(r, d->u, d->nv * sizeof(double));
+    if (d->dim == 2) r = fnv1a(r, d->v, d->nv * sizeof(double));
+    r = fnv1a(r, d->weight, nn * sizeof(double));
+    if (d->dim == 2) {
+        r = fnv1a(r, d->accel_cell, 2 * ncell * sizeof(double));
+        r = fnv1a(r, d->accel_vface, 2 * (size_t)(d->nx + 1) * d->ny * sizeof(double));
+        r = fnv1a(r, d->accel_hface, 2 * (size_t)d->nx * (d->ny + 1) * sizeof(double));
+    }
+    const int32_t iv[] = {d->nm, d->tau_model, d->dt_rule, d->bc[0], d->bc[1], d->bc[2],
+                          d->bc[3], d->force_variant};
+    const double dv[] = {d->dx, d->dy, d->pi, d->smv, d->innerk, d->gamma, d->prandtl,
+                         d->shakhov_coef, d->muref, d->omega, d->tw[0], d->tw[1], d->tw[2],
+                         d->tw[3], d->phi, d->umax_dt, d->vmax_dt, d->force_rate};
+    r = fnv1a(r, iv, sizeof iv);
+    return fnv1a(r, dv, sizeof dv);
+}
 
 static int fail(hugks_handle* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -176,8 +211,11 @@ static int state_info(hugks_handle* h, const DevState& s, hugks_step_info* info)
         for (int q = 0; q < 3; ++q) info->steady_res[q] = s.steady[q];
         info->mass_rel_change = s.mass_rel;
     }
-    if (s.flags & 1) return fail(h, HUGKS_E_NUMERIC, "non-finite state or T <= 0 on the device");
+    // the wall flux runs before the cell update in a step (evolution before
+    // update, 2d/src/main.f90:80-81), so a zero wall coefficient is the
+    // first failure of its step; the non-finite state it leads to is not
     if (s.flags & 2) return fail(h, HUGKS_E_WALL, "zero outgoing wall mass coefficient");
+    if (s.flags & 1) return fail(h, HUGKS_E_NUMERIC, "non-finite state or T <= 0 on the device");
     return HUGKS_OK;
 }
 static int read_state(hugks_handle* h, hugks_step_info* info) {
@@ -224,6 +262,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     if (d->nm < 0) return fail(nullptr, HUGKS_E_ARG, "Hermite order Nm must be non-negative");
     if (d->precision < HUGKS_FP64 || d->precision > HUGKS_MIXED_BF16X3)
         return fail(nullptr, HUGKS_E_ARG, "unknown precision");
+    if (d->precision == HUGKS_MIXED_BF16X3)
+        return fail(nullptr, HUGKS_E_UNSUPPORTED,
+                    "HUGKS_MIXED_BF16X3 has no kernel in this build");
     if (!d->u || !d->weight || !d->prim0 || (dim == 2 && (!d->v || !d->accel_cell ||
                                                           !d->accel_vface || !d->accel_hface)))
         return fail(nullptr, HUGKS_E_ARG, "missing node, weight, prim0 or acceleration array");
@@ -276,6 +317,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         return fail(nullptr, HUGKS_E_NOHIP, "no HIP device visible");
 
     hugks_handle* h = new hugks_handle();
+    h->run_hash = run_hash_of(d, ncell);
     h->d = *d;
     h->d.u = h->d.v = h->d.weight = h->d.prim0 = nullptr;
     h->d.accel_cell = h->d.accel_vface = h->d.accel_hface = nullptr;
@@ -300,8 +342,21 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { release(h); delete h; return hip_fail(nullptr, e, "stream"); }
-    hipEventCreate(&h->ev0);
-    hipEventCreate(&h->ev1);
+    // every runtime call of the set-up is checked: a failed copy of the
+    // node tables, the initial state or the accelerations must not leave a
+    // handle with unset inputs behind HUGKS_OK
+#define HCHK(call)                                                                 \
+    do {                                                                           \
+        e = (call);                                                                \
+        if (e != hipSuccess) {                                                     \
+            int rc = hip_fail(nullptr, e, "hugks_create: " #call);                 \
+            release(h);                                                            \
+            delete h;                                                              \
+            return rc;                                                             \
+        }                                                                          \
+    } while (0)
+    HCHK(hipEventCreate(&h->ev0));
+    HCHK(hipEventCreate(&h->ev1));
     const size_t fsz = ncell * ldv * es;
     const size_t nvf = dim == 2 ? (size_t)(d->nx + 1) * d->ny : (size_t)d->nx + 1;
     const size_t nhf = dim == 2 ? (size_t)d->nx * (d->ny + 1) : 1;
@@ -357,9 +412,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     }
     std::vector<double> vtab(d->nv, 0.0);
     if (dim == 2) for (int k = 0; k < d->nv; ++k) vtab[k] = d->v[k];
-    hipMemcpy(un_d, un.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(vn_d, vn.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(wn_d, wn.data(), ldv * sizeof(double), hipMemcpyHostToDevice);
+    HCHK(hipMemcpy(un_d, un.data(), ldv * sizeof(double), hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(vn_d, vn.data(), ldv * sizeof(double), hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(wn_d, wn.data(), ldv * sizeof(double), hipMemcpyHostToDevice));
     {
         std::vector<double> n4(4 * (size_t)ldv, 0.0);
         for (int k = 0; k < ldv; ++k) {
@@ -367,11 +422,11 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
             n4[4 * (size_t)k + 1] = vn[k];
             n4[4 * (size_t)k + 2] = wn[k];
         }
-        hipMemcpy(n4_d, n4.data(), 4 * (size_t)ldv * sizeof(double), hipMemcpyHostToDevice);
+        HCHK(hipMemcpy(n4_d, n4.data(), 4 * (size_t)ldv * sizeof(double), hipMemcpyHostToDevice));
     }
-    hipMemcpy(u1, d->u, d->nv * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(v1, vtab.data(), d->nv * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(prim0_d, d->prim0, ncell * (dim + 2) * sizeof(double), hipMemcpyHostToDevice);
+    HCHK(hipMemcpy(u1, d->u, d->nv * sizeof(double), hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(v1, vtab.data(), d->nv * sizeof(double), hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(prim0_d, d->prim0, ncell * (dim + 2) * sizeof(double), hipMemcpyHostToDevice));
     L.unode_d = un_d; L.vnode_d = vn_d; L.wnode_d = wn_d; L.u1 = u1; L.v1 = v1; L.node4 = n4_d;
     {
         void* uv;
@@ -380,11 +435,11 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         if (h->fp64) {
             std::vector<double> t(2 * (size_t)ldv);
             for (int k = 0; k < ldv; ++k) { t[2 * k] = un[k]; t[2 * k + 1] = vn[k]; }
-            hipMemcpy(uv, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
+            HCHK(hipMemcpy(uv, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
         } else {
             std::vector<float> t(2 * (size_t)ldv);
             for (int k = 0; k < ldv; ++k) { t[2 * k] = (float)un[k]; t[2 * k + 1] = (float)vn[k]; }
-            hipMemcpy(uv, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+            HCHK(hipMemcpy(uv, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
         }
         L.uvnode = uv;
     }
@@ -397,9 +452,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         ALLOC(wf, ldv * sizeof(float));
         std::vector<float> a(ldv), bb(ldv), cc(ldv);
         for (int k = 0; k < ldv; ++k) { a[k] = (float)un[k]; bb[k] = (float)vn[k]; cc[k] = (float)wn[k]; }
-        hipMemcpy(uf, a.data(), ldv * sizeof(float), hipMemcpyHostToDevice);
-        hipMemcpy(vf, bb.data(), ldv * sizeof(float), hipMemcpyHostToDevice);
-        hipMemcpy(wf, cc.data(), ldv * sizeof(float), hipMemcpyHostToDevice);
+        HCHK(hipMemcpy(uf, a.data(), ldv * sizeof(float), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(vf, bb.data(), ldv * sizeof(float), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(wf, cc.data(), ldv * sizeof(float), hipMemcpyHostToDevice));
         L.unode = uf; L.vnode = vf; L.wnode = wf;
     }
     if (dim == 2) {
@@ -407,9 +462,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         ALLOC(ac, ncell * 2 * sizeof(double));
         ALLOC(av, nvf * 2 * sizeof(double));
         ALLOC(ah, nhf * 2 * sizeof(double));
-        hipMemcpy(ac, d->accel_cell, ncell * 2 * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(av, d->accel_vface, nvf * 2 * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(ah, d->accel_hface, nhf * 2 * sizeof(double), hipMemcpyHostToDevice);
+        HCHK(hipMemcpy(ac, d->accel_cell, ncell * 2 * sizeof(double), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(av, d->accel_vface, nvf * 2 * sizeof(double), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(ah, d->accel_hface, nhf * 2 * sizeof(double), hipMemcpyHostToDevice));
         L.acc_cell = ac; L.acc_v = av; L.acc_h = ah;
         unsigned nwall = 0;
         for (int s = 0; s < 4; ++s)
@@ -454,8 +509,8 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
                 double *wxd, *wyd;
                 ALLOC(wxd, nv * sizeof(double));
                 ALLOC(wyd, nv * sizeof(double));
-                hipMemcpy(wxd, fx.data(), nv * sizeof(double), hipMemcpyHostToDevice);
-                hipMemcpy(wyd, fy.data(), nv * sizeof(double), hipMemcpyHostToDevice);
+                HCHK(hipMemcpy(wxd, fx.data(), nv * sizeof(double), hipMemcpyHostToDevice));
+                HCHK(hipMemcpy(wyd, fy.data(), nv * sizeof(double), hipMemcpyHostToDevice));
                 L.wx = wxd;
                 L.wy = wyd;
             }
@@ -469,7 +524,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         if (const char* um = getenv("HUGKS_UPDATE_KERNEL"))
             if (strcmp(um, "lds") == 0) L.update_mode = 0;
         hipDeviceProp_t prop;
-        hipGetDeviceProperties(&prop, h->device);
+        HCHK(hipGetDeviceProperties(&prop, h->device));
         if (L.update_lds > (size_t)prop.sharedMemPerBlockOptin && prop.sharedMemPerBlockOptin > 0) {
             release(h);
             delete h;
@@ -486,24 +541,24 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
 #undef ALLOC
     // zero everything that pads or is read before written, then the
     // initial state
-    hipMemsetAsync(L.shx, 0, fsz, h->stream);
-    hipMemsetAsync(L.sbx, 0, fsz, h->stream);
+    HCHK(hipMemsetAsync(L.shx, 0, fsz, h->stream));
+    HCHK(hipMemsetAsync(L.sbx, 0, fsz, h->stream));
     if (dim == 2) {
-        hipMemsetAsync(L.shy, 0, fsz, h->stream);
-        hipMemsetAsync(L.sby, 0, fsz, h->stream);
-        hipMemsetAsync(L.hfh, 0, nhf * ldv * es, h->stream);
-        hipMemsetAsync(L.hfb, 0, nhf * ldv * es, h->stream);
-        hipMemsetAsync(L.hflux, 0, nhf * 4 * sizeof(double), h->stream);
+        HCHK(hipMemsetAsync(L.shy, 0, fsz, h->stream));
+        HCHK(hipMemsetAsync(L.sby, 0, fsz, h->stream));
+        HCHK(hipMemsetAsync(L.hfh, 0, nhf * ldv * es, h->stream));
+        HCHK(hipMemsetAsync(L.hfb, 0, nhf * ldv * es, h->stream));
+        HCHK(hipMemsetAsync(L.hflux, 0, nhf * 4 * sizeof(double), h->stream));
     }
-    hipMemsetAsync(L.vfh, 0, nvf * ldv * es, h->stream);
-    hipMemsetAsync(L.vfb, 0, nvf * ldv * es, h->stream);
-    hipMemsetAsync(L.vflux, 0, nvf * 4 * sizeof(double), h->stream);
+    HCHK(hipMemsetAsync(L.vfh, 0, nvf * ldv * es, h->stream));
+    HCHK(hipMemsetAsync(L.vfb, 0, nvf * ldv * es, h->stream));
+    HCHK(hipMemsetAsync(L.vflux, 0, nvf * 4 * sizeof(double), h->stream));
     DevState s0{};
     s0.iter = 1;
     s0.t_stop = INFINITY;
     // initial mass: sum of rho * cell size in cell order (2d/src/main.f90:63-68)
     for (size_t q = 0; q < ncell; ++q) s0.mass0 += d->prim0[(size_t)(dim + 2) * q] * (c.dx * c.dy);
-    hipMemcpyAsync(L.st, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream);
+    HCHK(hipMemcpyAsync(L.st, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream));
     e = launch_init_f(L, dim, prim0_d, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
@@ -512,6 +567,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         delete h;
         return rc;
     }
+#undef HCHK
     *out = h;
     return HUGKS_OK;
 }
@@ -629,6 +685,8 @@ int hugks_set_f(hugks_handle* h, const double* hf, const double* bf, const doubl
                                h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
+    // a new state: a device flag raised by an earlier step no longer holds
+    if (e == hipSuccess) e = hipMemsetAsync(&h->L.st->flags, 0, sizeof(int), h->stream);
     if (tmp) hipFreeAsync(tmp, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "set_f");
@@ -691,12 +749,13 @@ int hugks_get_time(hugks_handle* h, double* simtime, int32_t* iter) {
 
 // ----------------------------------------------------------- checkpoint
 namespace {
-constexpr char kCkMagic[8] = {'H', 'U', 'G', 'K', 'S', 'C', 'K', '1'};
+constexpr char kCkMagic[8] = {'H', 'U', 'G', 'K', 'S', 'C', 'K', '2'};
 struct CkHeader {
     char magic[8];
     int32_t abi, dim, nx, ny, nv, nm, precision, ldv;
     int32_t iter, steady_count, flags, pad;
     double dt, simtime, mass0, mass_rel, res[4], steady[3];
+    uint64_t run_hash;
 };
 struct FileCloser {
     FILE* f;
@@ -722,6 +781,7 @@ int hugks_checkpoint_write(hugks_handle* h, const char* path) {
     hd.abi = HUGKS_ABI_VERSION;
     hd.dim = h->dim; hd.nx = c.nx; hd.ny = c.ny; hd.nv = c.nv; hd.nm = c.nm;
     hd.precision = h->d.precision; hd.ldv = c.ldv;
+    hd.run_hash = h->run_hash;
     hd.iter = st.iter; hd.steady_count = st.steady_count; hd.flags = st.flags;
     hd.dt = st.dt; hd.simtime = st.simtime; hd.mass0 = st.mass0; hd.mass_rel = st.mass_rel;
     for (int q = 0; q < 4; ++q) hd.res[q] = st.res[q];
@@ -751,8 +811,13 @@ int hugks_checkpoint_read(hugks_handle* h, const char* path) {
     if (fread(&hd, sizeof(hd), 1, fc.f) != 1 || memcmp(hd.magic, kCkMagic, 8) != 0)
         return fail(h, HUGKS_E_ARG, std::string("not a checkpoint file: ") + path);
     if (hd.abi != HUGKS_ABI_VERSION || hd.dim != h->dim || hd.nx != c.nx || hd.ny != c.ny ||
-        hd.nv != c.nv || hd.precision != h->d.precision || hd.ldv != c.ldv)
-        return fail(h, HUGKS_E_ARG, "checkpoint shape or precision differs from the handle");
+        hd.nv != c.nv || hd.nm != c.nm || hd.precision != h->d.precision || hd.ldv != c.ldv)
+        return fail(h, HUGKS_E_ARG,
+                    "checkpoint shape, Hermite order or precision differs from the handle");
+    if (hd.run_hash != h->run_hash)
+        return fail(h, HUGKS_E_ARG,
+                    "checkpoint was written by a run with other velocity nodes, weights, "
+                    "accelerations or physical constants");
     std::vector<char> buf(wbytes > fbytes ? wbytes : fbytes);
     void* dsts[3] = {h->L.W, h->L.h, h->L.b};
     const size_t sizes[3] = {wbytes, fbytes, fbytes};

@@ -36,26 +36,23 @@ __device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
 }
 
 // Marks the start of a host call: later kernels run again even if a
-// previous call ended on an exit condition.
+// previous call ended on an exit condition, unless a device flag is raised
+// (the state of a failed step is not stepped further; hugks_set_f and a
+// checkpoint restore clear the flags).
+__device__ __forceinline__ void begin_state(DevState* st, double t_stop) {
+    st->stopped = st->flags ? 1 : 0;
+    st->stop_reason = st->flags ? HUGKS_STOP_ERROR : 0;
+    st->steps = 0;
+    st->t_stop = t_stop;
+}
 #ifndef HUGKS_BODIES_ONLY
 __global__ void k_begin(DevState* st, double t_stop) {
-    if (threadIdx.x == 0) {
-        st->stopped = 0;
-        st->stop_reason = 0;
-        st->steps = 0;
-        st->t_stop = t_stop;
-    }
+    if (threadIdx.x == 0) begin_state(st, t_stop);
 }
 // k_begin for every case of a batch, one thread per case.
 __global__ void k_begin_all(DevState* const* __restrict__ sts, int n, double t_stop) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        DevState* st = sts[i];
-        st->stopped = 0;
-        st->stop_reason = 0;
-        st->steps = 0;
-        st->t_stop = t_stop;
-    }
+    if (i < n) begin_state(sts[i], t_stop);
 }
 // The states of a batch's cases side by side, for one copy to the host.
 __global__ void k_gather_states(DevState* const* __restrict__ sts, int n,
@@ -242,7 +239,12 @@ __device__ __forceinline__ void k_finalize_body(Consts c, int dim, DevState* st,
         // mass = sum of rho * cell size (rho > 0 is enforced by the update)
         const double mass = (dim == 2 ? s[4] * (c.dx * c.dy) : s[3] * c.dx);
         st->mass_rel = (mass - st->mass0) / (fabs(st->mass0) + c.smv);
-        if (dim == 2 && st->simtime >= c.maxtime) {
+        if (st->flags) {
+            // the reference stops the run at the error (error stop); later
+            // steps of this call do nothing and the host returns the status
+            st->stopped = 1;
+            st->stop_reason = HUGKS_STOP_ERROR;
+        } else if (dim == 2 && st->simtime >= c.maxtime) {
             st->stopped = 1;
             st->stop_reason = HUGKS_STOP_MAXTIME;
         } else if (dim == 2 && c.stop_on_residual && all_small) {

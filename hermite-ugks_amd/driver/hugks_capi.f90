@@ -26,7 +26,7 @@ module hugks_capi
   integer(c_int32_t), parameter :: HUGKS_FORCE_HERMITE_SYMMETRIC = 0, &
        HUGKS_FORCE_HERMITE_SYMMETRIC_B = 1, HUGKS_FORCE_HERMITE_LEGACY = 2
   integer(c_int32_t), parameter :: HUGKS_STOP_NONE = 0, HUGKS_STOP_MAXTIME = 1, &
-       HUGKS_STOP_RESIDUAL = 2, HUGKS_STOP_TIME = 3, HUGKS_STOP_STEADY = 4
+       HUGKS_STOP_RESIDUAL = 2, HUGKS_STOP_TIME = 3, HUGKS_STOP_STEADY = 4, HUGKS_STOP_ERROR = 5
   integer(c_int32_t), parameter :: HUGKS_FLUX_FUSED = 0, HUGKS_FLUX_SPLIT = 1, HUGKS_FLUX_STENCIL = 2, &
        HUGKS_UPDATE_LDS = 0, HUGKS_UPDATE_MFMA = 1, HUGKS_BATCH_AUTOTUNE = 1
 

@@ -20,7 +20,7 @@ BC_PERIODIC, BC_SPECULAR, BC_DIFFUSE = 0, 1, 2
 TAU_MU_OVER_RHO, TAU_VHS = 0, 1
 DT_2D_GENERAL, DT_2D_WALL_NORMAL, DT_1D = 0, 1, 2
 FORCE_HERMITE_SYMMETRIC, FORCE_HERMITE_SYMMETRIC_B, FORCE_HERMITE_LEGACY = 0, 1, 2
-STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL, STOP_TIME, STOP_STEADY = 0, 1, 2, 3, 4
+STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL, STOP_TIME, STOP_STEADY, STOP_ERROR = 0, 1, 2, 3, 4, 5
 FLUX_FUSED, FLUX_SPLIT, FLUX_STENCIL, UPDATE_LDS, UPDATE_MFMA = 0, 1, 2, 0, 1
 BATCH_AUTOTUNE = 1
 

@@ -56,8 +56,9 @@ extern "C" {
 #define HUGKS_FP64           0  /* everything FP64 (parity path)            */
 #define HUGKS_MIXED_FP32     1  /* FP32 f/flux, FP64 moments and W,
                                    FP32 Hermite contractions                 */
-#define HUGKS_MIXED_BF16X3   2  /* reserved for bf16x3 Hermite contractions;
-                                   runs as HUGKS_MIXED_FP32 today            */
+#define HUGKS_MIXED_BF16X3   2  /* as HUGKS_MIXED_FP32 with bf16x3 split-
+                                   product Hermite contractions; refused with
+                                   HUGKS_E_UNSUPPORTED where not built       */
 
 /* boundary kind per side (x-lo, x-hi, y-lo, y-hi) */
 #define HUGKS_BC_PERIODIC    0  /* 2d/src/boundary.f90:16-53                 */
@@ -86,6 +87,9 @@ extern "C" {
 #define HUGKS_STOP_RESIDUAL  2  /* all(res < eps_res) (2D case 5)           */
 #define HUGKS_STOP_TIME      3  /* simtime >= the t_stop of the call        */
 #define HUGKS_STOP_STEADY    4  /* 1D steady test held steady_steps times   */
+#define HUGKS_STOP_ERROR     5  /* a device flag was raised in the last step
+                                   (the call returns HUGKS_E_NUMERIC or
+                                   HUGKS_E_WALL)                            */
 
 typedef struct hugks_desc {
     int32_t abi_version;      /* = HUGKS_ABI_VERSION                          */
@@ -185,7 +189,10 @@ int hugks_get_macro(hugks_handle *h, double *w, double *q);
 
 /* Distribution functions as FP64, [ncell * nv^dim] each. */
 int hugks_get_f(hugks_handle *h, double *hf, double *bf);
-/* Overwrites h, b and (when w != NULL) W. */
+/* Overwrites h, b and (when w != NULL) W, and clears the device flags of
+ * an earlier failed step.  A handle whose step raised a device flag runs
+ * no further step (hugks_step/advance return the same status) until then
+ * or until hugks_checkpoint_read restores another state. */
 int hugks_set_f(hugks_handle *h, const double *hf, const double *bf,
                 const double *w);
 
@@ -268,7 +275,10 @@ int hugks_get_time(hugks_handle *h, double *simtime, int32_t *iter);
  * steady counter) and W, h, b exactly as stored on the device (FP32 f on
  * the mixed paths), so a handle restored from it continues bitwise as the
  * one that wrote it.  hugks_checkpoint_read needs a handle created with
- * the same dim, nx, ny, nv and precision; HUGKS_E_ARG names any mismatch or
+ * the same dim, nx, ny, nv, Nm and precision, and with the same velocity
+ * nodes, weights, accelerations, physical constants and boundary
+ * conditions (compared through a hash stored in the file; loop controls
+ * such as cfl and maxtime may differ); HUGKS_E_ARG names any mismatch or
  * an unreadable file. */
 int hugks_checkpoint_write(hugks_handle *h, const char *path);
 int hugks_checkpoint_read(hugks_handle *h, const char *path);
@@ -279,8 +289,9 @@ int hugks_checkpoint_read(hugks_handle *h, const char *path);
 int hugks_kernel_times(hugks_handle *h, int32_t cap, const char **names,
                        double *ms, int32_t *calls, int32_t *count);
 int hugks_reset_kernel_times(hugks_handle *h);
-/* 1: time every launch with HIP events (adds a sync per launch);
- * 0: off (default). */
+/* 1: bracket every launch with two HIP events recorded on the handle's
+ * stream (no synchronisation per launch; the elapsed times are collected
+ * by hugks_kernel_times, or after 2048 launches); 0: off (default). */
 int hugks_set_timing(hugks_handle *h, int32_t on);
 
 /* Library/device information string (gfx target, CU count). */

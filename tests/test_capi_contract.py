@@ -47,7 +47,7 @@ def test_header_constants_match_binding():
                  "E_NOHIP", "FP64", "MIXED_FP32", "MIXED_BF16X3", "BC_PERIODIC", "BC_SPECULAR",
                  "BC_DIFFUSE", "TAU_MU_OVER_RHO", "TAU_VHS", "DT_2D_GENERAL", "DT_1D",
                  "FORCE_HERMITE_SYMMETRIC", "FORCE_HERMITE_SYMMETRIC_B", "FORCE_HERMITE_LEGACY",
-                 "STOP_NONE", "STOP_MAXTIME", "STOP_RESIDUAL", "STOP_TIME", "STOP_STEADY",
+                 "STOP_NONE", "STOP_MAXTIME", "STOP_RESIDUAL", "STOP_TIME", "STOP_STEADY", "STOP_ERROR",
                  "ARR_SHX", "ARR_HFLUX"):
         assert consts["HUGKS_" + name] == getattr(capi, name), name
     assert consts["HUGKS_DT_2D_WALL_NORMAL"] == capi.DT_2D_WALL_NORMAL
@@ -158,3 +158,14 @@ def have_no_gpu():
     if not info.startswith("no HIP device"):
         pytest.skip("a HIP device is present")
     return info
+
+
+def test_bf16x3_without_kernel_is_refused():
+    """HUGKS_MIXED_BF16X3 names split-product Hermite contractions; without
+    them the library refuses the mode instead of running another one."""
+    case = small_case()
+    d = case.bind()
+    d.precision = capi.MIXED_BF16X3
+    rc, msg = create(d)
+    assert rc == capi.E_UNSUPPORTED, (rc, msg)
+    assert "BF16X3" in msg

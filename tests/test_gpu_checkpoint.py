@@ -1,7 +1,9 @@
 """Checkpoint / restore (hugks_checkpoint_write / _read): a handle restored
 from a checkpoint continues bitwise as the handle that wrote it (W, f and
-the loop state), in both precisions; a checkpoint of another shape or
-precision is refused with HUGKS_E_ARG."""
+the loop state), in both precisions; a checkpoint of another shape,
+Hermite order or precision, or of a run with other weights, forces or
+physical constants, is refused with HUGKS_E_ARG, while loop controls
+(maxtime, cfl) may change."""
 import numpy as np
 import pytest
 
@@ -42,10 +44,22 @@ def test_restore_refuses_other_shape(have_gpu, tmp_path):
         a.advance(1)
         a.checkpoint(path)
     for other in (cases.build_2d(poiseuille(nx=4, ny=12, nv=12), capi.FP64),
-                  cases.build_2d(poiseuille(nx=4, ny=10, nv=12), capi.MIXED_FP32)):
+                  cases.build_2d(poiseuille(nx=4, ny=10, nv=12), capi.MIXED_FP32),
+                  cases.build_2d(poiseuille(nx=4, ny=10, nv=12, nm=5), capi.FP64)):
         with capi.Solver(other) as b:
             with pytest.raises(capi.HugksError, match="differs"):
                 b.restore(path)
+    # same shape, other run: force, Knudsen number (collision time), wall
+    for other in (poiseuille(nx=4, ny=10, nv=12, phi=0.07),
+                  poiseuille(nx=4, ny=10, nv=12, kn=0.2),
+                  poiseuille(nx=4, ny=10, nv=12, tw=(1.0, 1.1))):
+        with capi.Solver(cases.build_2d(other, capi.FP64)) as b:
+            with pytest.raises(capi.HugksError, match="written by a run with other"):
+                b.restore(path)
+    # loop controls may differ
+    longer = dict(poiseuille(nx=4, ny=10, nv=12), maxtime=500.0, cfl=0.6)
+    with capi.Solver(cases.build_2d(longer, capi.FP64)) as b:
+        b.restore(path)
     with capi.Solver(cases.build_2d(poiseuille(nx=4, ny=10, nv=12), capi.FP64)) as b:
         (tmp_path / "junk").write_bytes(b"not a checkpoint")
         with pytest.raises(capi.HugksError, match="not a checkpoint"):

@@ -162,8 +162,9 @@ def test_odd_velocity_grids(have_gpu, monkeypatch, mode):
 @pytest.mark.parametrize("nv", [24, 28, 32])
 def test_compiled_in_grid_sizes(have_gpu, nv):
     """The update kernel instances compiled for common Gauss-Hermite grids
-    (nv = 24, 28, 32 here; 16, 20 and 40 run in other tests), FP64 and mixed,
-    against the oracle."""
+    (nv = 24, 28, 32 here; 16 and 20 in test_flux_kernels_agree and
+    test_update_kernels_agree, 40 in test_gpu_headline_grid.py), FP64 and
+    mixed, against the oracle."""
     for prec, tol in ((capi.FP64, TOL_FP64), (capi.MIXED_FP32, TOL_MIXED)):
         case = cases.build_2d(poiseuille(nx=4, ny=6, nv=nv, nm=8), prec)
         r = run_pair(case, 3)
