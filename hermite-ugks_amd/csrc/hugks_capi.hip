@@ -640,6 +640,73 @@ int hugks_get_macro(hugks_handle* h, double* w, double* q) {
     return HUGKS_OK;
 }
 
+}  // extern "C"
+
+// Staging of hugks_get_macro_noalloc: a file-scope device array instead of
+// a per-call allocation, so that callers that cannot allocate device
+// memory (code under graph capture, tight serving loops) can read the
+// output fields.  Every GPU holds one copy, at its own address, for the
+// life of the process; host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name.  Every k_macro work-group
+// writes all dim entries of its cell, so nothing has to start at zero.
+// Calls that overlap on one GPU share the array and race.
+namespace {
+constexpr size_t kMacroScratchDoubles = HUGKS_MACRO_NOALLOC_MAX;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) double scratch[kMacroScratchDoubles];
+
+// The device address of scratch on device dev, looked up once per device
+// with that device current.
+static hipError_t scratch_on(int dev, double** out) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) {
+            const hipError_t e2 = hipSetDevice(cur);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+extern "C" {
+
+int hugks_get_macro_noalloc(hugks_handle* h, double* w, double* q) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    const size_t ncell = h->ncell;
+    if (q && (size_t)h->dim * ncell > kMacroScratchDoubles)
+        return fail(h, HUGKS_E_UNSUPPORTED,
+                    "hugks_get_macro_noalloc: dim*ncell = " + std::to_string(h->dim * ncell) +
+                        " heat-flux values exceed its static staging of " +
+                        std::to_string(kMacroScratchDoubles) + "; use hugks_get_macro");
+    hipError_t e = hipSuccess;
+    if (w) e = hipMemcpyAsync(w, h->L.W, ncell * (h->dim + 2) * sizeof(double),
+                              hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && q) {
+        double* qd = nullptr;
+        e = scratch_on(h->device, &qd);
+        if (e == hipSuccess) e = launch_macro(h->L, h->dim, qd, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(q, qd, ncell * h->dim * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "get_macro_noalloc");
+    return HUGKS_OK;
+}
+
 int hugks_get_f(hugks_handle* h, double* hf, double* bf) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
     const Consts& c = h->L.c;

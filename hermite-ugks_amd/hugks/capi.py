@@ -63,7 +63,7 @@ class StepInfo(C.Structure):
 EXPORTS = [
     "hugks_create", "hugks_destroy", "hugks_last_error", "hugks_step", "hugks_advance",
     "hugks_advance_until",
-    "hugks_get_macro", "hugks_get_f", "hugks_set_f", "hugks_get_time",
+    "hugks_get_macro", "hugks_get_macro_noalloc", "hugks_get_f", "hugks_set_f", "hugks_get_time",
     "hugks_checkpoint_write", "hugks_checkpoint_read", "hugks_kernel_times",
     "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
     "hugks_set_schedule", "hugks_get_schedule", "hugks_tune", "hugks_batch_create",
@@ -96,6 +96,7 @@ def load(path=LIBPATH):
     lib.hugks_advance.argtypes = [H, C.c_int32, C.POINTER(StepInfo)]
     lib.hugks_advance_until.argtypes = [H, C.c_int32, C.c_double, C.POINTER(StepInfo)]
     lib.hugks_get_macro.argtypes = [H, _dp, _dp]
+    lib.hugks_get_macro_noalloc.argtypes = [H, _dp, _dp]
     lib.hugks_get_f.argtypes = [H, _dp, _dp]
     lib.hugks_set_f.argtypes = [H, _dp, _dp, _dp]
     lib.hugks_get_time.argtypes = [H, _dp, C.POINTER(C.c_int32)]
@@ -229,6 +230,15 @@ class Solver:
         w = np.empty((self.case.ncell, d.dim + 2))
         q = np.empty((self.case.ncell, d.dim)) if with_q else None
         self._check(self.lib.hugks_get_macro(self.h, _ptr(w), _ptr(q)))
+        return (w, q) if with_q else w
+
+    def macro_noalloc(self, with_q=True):
+        """As macro(), through hugks_get_macro_noalloc (no device allocation;
+        q staged in the library's static device array)."""
+        d = self.case.desc
+        w = np.empty((self.case.ncell, d.dim + 2))
+        q = np.empty((self.case.ncell, d.dim)) if with_q else None
+        self._check(self.lib.hugks_get_macro_noalloc(self.h, _ptr(w), _ptr(q)))
         return (w, q) if with_q else w
 
     def get_f(self):

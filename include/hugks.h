@@ -187,6 +187,19 @@ int hugks_advance_until(hugks_handle *h, int32_t max_steps, double t_stop,
  * get_heat_flux(h, b, prim(w)).  Either pointer may be NULL. */
 int hugks_get_macro(hugks_handle *h, double *w, double *q);
 
+/* As hugks_get_macro, allocating nothing: no hipMalloc, hipFree or
+ * allocating library call.  This entry point has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  q is staged in a file-scope device array of the library of
+ * HUGKS_MACRO_NOALLOC_MAX doubles (8 MiB, held on each GPU it runs on until
+ * the process ends): a case with dim*ncell above that is refused with
+ * HUGKS_E_UNSUPPORTED (up to 524288 cells in 2D).  All calls on one GPU
+ * share the array: calls that overlap on one GPU (other host threads,
+ * other handles' streams) race. */
+#define HUGKS_MACRO_NOALLOC_MAX (1 << 20)
+int hugks_get_macro_noalloc(hugks_handle *h, double *w, double *q);
+
 /* Distribution functions as FP64, [ncell * nv^dim] each. */
 int hugks_get_f(hugks_handle *h, double *hf, double *bf);
 /* Overwrites h, b and (when w != NULL) W, and clears the device flags of
