@@ -28,6 +28,92 @@ struct KernelTimer {
     int calls;
 };
 
+// Per-kernel timing: every launch is bracketed by two HIP events recorded
+// on the launch stream (no synchronisation); elapsed times are collected
+// when the host asks for them or when many records are pending.
+struct KTimes {
+    hipStream_t stream = nullptr;
+    std::vector<KernelTimer> timers;
+    struct Rec {
+        const char* name;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t open_ev = nullptr;
+};
+
+void kt_harvest(KTimes& t) {
+    if (t.pending.empty()) return;
+    (void)hipStreamSynchronize(t.stream);
+    for (auto& r : t.pending) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.e0, r.e1);
+        bool found = false;
+        for (auto& k : t.timers)
+            if (k.name == r.name) {
+                k.ms += ms;
+                k.calls += 1;
+                found = true;
+                break;
+            }
+        if (!found) t.timers.push_back({r.name, (double)ms, 1});
+        t.pool.push_back(r.e0);
+        t.pool.push_back(r.e1);
+    }
+    t.pending.clear();
+}
+
+hipEvent_t kt_event(KTimes& t) {
+    if (t.pool.empty()) {
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    hipEvent_t e = t.pool.back();
+    t.pool.pop_back();
+    return e;
+}
+
+void kt_hook(void* ctx, int phase, const char* name) {
+    KTimes& t = *static_cast<KTimes*>(ctx);
+    if (phase == 0) {
+        if (t.pending.size() >= 4096) kt_harvest(t);
+        t.open_ev = kt_event(t);
+        (void)hipEventRecord(t.open_ev, t.stream);
+    } else {
+        hipEvent_t e1 = kt_event(t);
+        (void)hipEventRecord(e1, t.stream);
+        t.pending.push_back({name, t.open_ev, e1});
+    }
+}
+
+void kt_release(KTimes& t) {
+    for (auto& r : t.pending) {
+        (void)hipEventDestroy(r.e0);
+        (void)hipEventDestroy(r.e1);
+    }
+    t.pending.clear();
+    for (hipEvent_t e : t.pool) (void)hipEventDestroy(e);
+    t.pool.clear();
+}
+
+int kt_report(KTimes& t, int32_t cap, const char** names, double* ms, int32_t* calls,
+              int32_t* count) {
+    kt_harvest(t);
+    int n = 0;
+    for (auto& k : t.timers) {
+        if (n < cap) {
+            if (names) names[n] = k.name;
+            if (ms) ms[n] = k.ms;
+            if (calls) calls[n] = k.calls;
+        }
+        ++n;
+    }
+    if (count) *count = n;
+    return HUGKS_OK;
+}
+
 }  // namespace
 
 struct hugks_handle {
@@ -40,14 +126,7 @@ struct hugks_handle {
     std::vector<void*> allocs;
     std::string err;
     bool timing = false;
-    std::vector<KernelTimer> timers;
-    struct Rec {
-        const char* name;
-        hipEvent_t e0, e1;
-    };
-    std::vector<Rec> pending;
-    std::vector<hipEvent_t> pool;
-    hipEvent_t open_ev = nullptr;
+    KTimes kt;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t ncell = 0;
     uint64_t run_hash = 0;   // of the inputs that define the run (checkpoints)
@@ -110,68 +189,14 @@ static hipError_t dalloc(hugks_handle* h, P** p, size_t bytes) {
 
 static void release(hugks_handle* h) {
     if (!h) return;
-    if (h->stream) hipStreamSynchronize(h->stream);
-    for (void* p : h->allocs) hipFree(p);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
-    if (h->ev0) hipEventDestroy(h->ev0);
-    if (h->ev1) hipEventDestroy(h->ev1);
-    for (auto& r : h->pending) {
-        hipEventDestroy(r.e0);
-        hipEventDestroy(r.e1);
-    }
-    h->pending.clear();
-    for (hipEvent_t e : h->pool) hipEventDestroy(e);
-    h->pool.clear();
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    kt_release(h->kt);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     h->stream = nullptr;
-}
-
-// Per-kernel timing: every launch is bracketed by two HIP events recorded
-// on the handle's stream (no synchronisation); elapsed times are collected
-// when the host asks for them or when many records are pending.
-static void harvest(hugks_handle* h) {
-    if (h->pending.empty()) return;
-    hipStreamSynchronize(h->stream);
-    for (auto& r : h->pending) {
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, r.e0, r.e1);
-        bool found = false;
-        for (auto& t : h->timers)
-            if (t.name == r.name) {
-                t.ms += ms;
-                t.calls += 1;
-                found = true;
-                break;
-            }
-        if (!found) h->timers.push_back({r.name, (double)ms, 1});
-        h->pool.push_back(r.e0);
-        h->pool.push_back(r.e1);
-    }
-    h->pending.clear();
-}
-
-static hipEvent_t pool_event(hugks_handle* h) {
-    if (h->pool.empty()) {
-        hipEvent_t e = nullptr;
-        hipEventCreate(&e);
-        return e;
-    }
-    hipEvent_t e = h->pool.back();
-    h->pool.pop_back();
-    return e;
-}
-
-static void timing_hook(void* ctx, int phase, const char* name) {
-    hugks_handle* h = (hugks_handle*)ctx;
-    if (phase == 0) {
-        if (h->pending.size() >= 4096) harvest(h);
-        h->open_ev = pool_event(h);
-        hipEventRecord(h->open_ev, h->stream);
-    } else {
-        hipEvent_t e1 = pool_event(h);
-        hipEventRecord(e1, h->stream);
-        h->pending.push_back({name, h->open_ev, e1});
-    }
 }
 
 // One reference step of h on stream s (the handle's own stream, or a
@@ -179,8 +204,8 @@ static void timing_hook(void* ctx, int phase, const char* name) {
 static hipError_t enqueue_step_on(hugks_handle* h, hipStream_t s, bool timed) {
     hipError_t e;
     Launch2D& L = h->L;
-    L.hook = timed && h->timing ? timing_hook : nullptr;
-    L.hook_ctx = h;
+    L.hook = timed && h->timing ? kt_hook : nullptr;
+    L.hook_ctx = &h->kt;
     e = launch_timestep(L, h->dim, s);
     if (e != hipSuccess) return e;
     if (h->dim == 2) {
@@ -240,7 +265,7 @@ const char* hugks_device_info(void) {
         return info.c_str();
     }
     hipDeviceProp_t p;
-    hipGetDeviceProperties(&p, 0);
+    (void)hipGetDeviceProperties(&p, 0);
     char buf[256];
     snprintf(buf, sizeof buf, "%s %s CUs=%d LDS/CU=%zu KiB HBM=%.1f GiB", p.name, p.gcnArchName,
              p.multiProcessorCount, (size_t)p.maxSharedMemoryPerMultiProcessor / 1024,
@@ -324,7 +349,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     h->dim = dim;
     h->fp64 = d->precision == HUGKS_FP64;
     h->ncell = ncell;
-    hipGetDevice(&h->device);
+    (void)hipGetDevice(&h->device);
     Consts& c = h->L.c;
     c.nx = d->nx; c.ny = d->ny; c.nv = d->nv; c.nn = nn; c.ldv = ldv; c.nm = d->nm;
     for (int s = 0; s < 4; ++s) { c.bc[s] = d->bc[s]; c.tw[s] = d->tw[s]; }
@@ -342,6 +367,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { release(h); delete h; return hip_fail(nullptr, e, "stream"); }
+    h->kt.stream = h->stream;
     // every runtime call of the set-up is checked: a failed copy of the
     // node tables, the initial state or the accelerations must not leave a
     // handle with unset inputs behind HUGKS_OK
@@ -633,7 +659,7 @@ int hugks_get_macro(hugks_handle* h, double* w, double* q) {
         if (e == hipSuccess)
             e = hipMemcpyAsync(q, qd, ncell * h->dim * sizeof(double), hipMemcpyDeviceToHost,
                                h->stream);
-        if (qd) hipFreeAsync(qd, h->stream);
+        if (qd) (void)hipFreeAsync(qd, h->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "get_macro");
@@ -721,7 +747,7 @@ int hugks_get_f(hugks_handle* h, double* hf, double* bf) {
             e = hipMemcpyAsync(dst, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
-    if (tmp) hipFreeAsync(tmp, h->stream);
+    if (tmp) (void)hipFreeAsync(tmp, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "get_f");
     return HUGKS_OK;
@@ -754,7 +780,7 @@ int hugks_set_f(hugks_handle* h, const double* hf, const double* bf, const doubl
     }
     // a new state: a device flag raised by an earlier step no longer holds
     if (e == hipSuccess) e = hipMemsetAsync(&h->L.st->flags, 0, sizeof(int), h->stream);
-    if (tmp) hipFreeAsync(tmp, h->stream);
+    if (tmp) (void)hipFreeAsync(tmp, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "set_f");
     return HUGKS_OK;
@@ -798,7 +824,7 @@ int hugks_debug_fetch(hugks_handle* h, int32_t which, double* out) {
     hipError_t e = hipMallocAsync((void**)&tmp, n * sizeof(double), h->stream);
     if (e == hipSuccess) e = launch_convert(src, h->fp64, tmp, items, c.nn, c.ldv, true, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
-    if (tmp) hipFreeAsync(tmp, h->stream);
+    if (tmp) (void)hipFreeAsync(tmp, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     return e == hipSuccess ? HUGKS_OK : hip_fail(h, e, "debug_fetch");
 }
@@ -912,24 +938,13 @@ int hugks_checkpoint_read(hugks_handle* h, const char* path) {
 int hugks_kernel_times(hugks_handle* h, int32_t cap, const char** names, double* ms,
                        int32_t* calls, int32_t* count) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
-    harvest(h);
-    int n = 0;
-    for (auto& t : h->timers) {
-        if (n < cap) {
-            if (names) names[n] = t.name;
-            if (ms) ms[n] = t.ms;
-            if (calls) calls[n] = t.calls;
-        }
-        ++n;
-    }
-    if (count) *count = n;
-    return HUGKS_OK;
+    return kt_report(h->kt, cap, names, ms, calls, count);
 }
 
 int hugks_reset_kernel_times(hugks_handle* h) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
-    harvest(h);
-    h->timers.clear();
+    kt_harvest(h->kt);
+    h->kt.timers.clear();
     return HUGKS_OK;
 }
 
@@ -1068,21 +1083,24 @@ struct hugks_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string err;
+    bool timing = false;   // eager steps with timed grouped launches
+    KTimes kt;
 };
 
 static void batch_release(hugks_batch* b) {
     if (!b) return;
-    if (b->main) hipStreamSynchronize(b->main);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    for (hipEvent_t e : b->joins) hipEventDestroy(e);
-    if (b->fork) hipEventDestroy(b->fork);
-    for (hipStream_t s : b->lanes) hipStreamDestroy(s);
-    if (b->main) hipStreamDestroy(b->main);
-    for (void* p : b->gmem) hipFree(p);
-    if (b->d_sts) hipFree(b->d_sts);
-    if (b->d_gather) hipFree(b->d_gather);
-    if (b->h_gather) hipHostFree(b->h_gather);
+    if (b->main) (void)hipStreamSynchronize(b->main);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipStream_t s : b->lanes) (void)hipStreamDestroy(s);
+    if (b->main) (void)hipStreamDestroy(b->main);
+    kt_release(b->kt);
+    for (void* p : b->gmem) (void)hipFree(p);
+    if (b->d_sts) (void)hipFree(b->d_sts);
+    if (b->d_gather) (void)hipFree(b->d_gather);
+    if (b->h_gather) (void)hipHostFree(b->h_gather);
     for (hugks_handle* h : b->cases) hugks_destroy(h);
     delete b;
 }
@@ -1261,6 +1279,7 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     // one step of every case, fanned out over a few streams and captured
     // into one graph: a replay advances the whole batch by one step
     hipError_t e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking);
+    b->kt.stream = b->main;
     // the cases' states: reset and read back for all cases at once
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_sts, n * sizeof(DevState*));
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_gather, n * sizeof(DevState));
@@ -1311,8 +1330,8 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
         e = hipStreamEndCapture(b->main, &b->graph);
     } else {
         hipGraph_t g = nullptr;
-        hipStreamEndCapture(b->main, &g);
-        if (g) hipGraphDestroy(g);
+        (void)hipStreamEndCapture(b->main, &g);
+        if (g) (void)hipGraphDestroy(g);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
@@ -1349,6 +1368,42 @@ int hugks_batch_handle(hugks_batch* b, int32_t i, hugks_handle** h) {
     return HUGKS_OK;
 }
 
+// One step of every case: the captured graph, or, with timing on, the same
+// launches issued eagerly on the batch stream, the grouped ones bracketed
+// by timing events (the cases with their own launches run untimed).
+static hipError_t batch_step(hugks_batch* b) {
+    if (!b->timing) return hipGraphLaunch(b->exec, b->main);
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < b->cases.size() && e == hipSuccess; ++i)
+        if (!b->in_group[i]) e = enqueue_step_on(b->cases[i], b->main, false);
+    for (size_t q = 0; q < b->plans.size() && e == hipSuccess; ++q) {
+        GroupPlan P = b->plans[q];
+        P.hook = kt_hook;
+        P.hook_ctx = &b->kt;
+        e = launch_group_step(P, b->main);
+    }
+    return e;
+}
+
+int hugks_batch_set_timing(hugks_batch* b, int32_t on) {
+    if (!b) return batch_fail(nullptr, HUGKS_E_ARG, "null batch");
+    b->timing = on != 0;
+    return HUGKS_OK;
+}
+
+int hugks_batch_kernel_times(hugks_batch* b, int32_t cap, const char** names, double* ms,
+                             int32_t* calls, int32_t* count) {
+    if (!b) return batch_fail(nullptr, HUGKS_E_ARG, "null batch");
+    return kt_report(b->kt, cap, names, ms, calls, count);
+}
+
+int hugks_batch_reset_kernel_times(hugks_batch* b) {
+    if (!b) return batch_fail(nullptr, HUGKS_E_ARG, "null batch");
+    kt_harvest(b->kt);
+    b->kt.timers.clear();
+    return HUGKS_OK;
+}
+
 int hugks_batch_advance(hugks_batch* b, int32_t max_steps, hugks_step_info* infos) {
     if (!b) return batch_fail(nullptr, HUGKS_E_ARG, "null batch");
     const int n = (int)b->cases.size();
@@ -1363,11 +1418,11 @@ int hugks_batch_advance(hugks_batch* b, int32_t max_steps, hugks_step_info* info
     };
     hipError_t e = launch_begin_all(b->d_sts, n, INFINITY, b->main);
     if (max_steps > 0) {
-        for (int32_t s = 0; s < max_steps && e == hipSuccess; ++s) e = hipGraphLaunch(b->exec, b->main);
+        for (int32_t s = 0; s < max_steps && e == hipSuccess; ++s) e = batch_step(b);
     } else {
         // no cap: replay until every case met an exit condition
         for (;;) {
-            for (int s = 0; s < 32 && e == hipSuccess; ++s) e = hipGraphLaunch(b->exec, b->main);
+            for (int s = 0; s < 32 && e == hipSuccess; ++s) e = batch_step(b);
             if (e == hipSuccess) e = fetch();
             if (e != hipSuccess) break;
             bool all = true;

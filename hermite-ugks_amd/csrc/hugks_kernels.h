@@ -54,6 +54,19 @@ constexpr int PRED_MIN_CELLS = 16384;
 // Doubles of the cell-reduction partials (k_misc.h: PRED_NB x (1 + PRED_LD)).
 constexpr int RED_SCRATCH = 1024 * 10;
 
+// Brackets one launch with a hook given by function and context.
+struct HookCall {
+    LaunchHook f;
+    void* ctx;
+    const char* name;
+    HookCall(LaunchHook fn, void* c, const char* n) : f(fn), ctx(c), name(n) {
+        if (f) f(ctx, 0, name);
+    }
+    ~HookCall() {
+        if (f) f(ctx, 1, name);
+    }
+};
+
 // Brackets one launch with the hook.
 struct HookScope {
     const Launch2D& L;
@@ -97,6 +110,8 @@ struct GroupPlan {
     int g_ts, g_slope, g_fsum, g_fface, g_fnode, g_wall, g_cpre, g_fin;
     GroupUpd upd[16];
     int nupd;
+    LaunchHook hook;     // may be null (timing of eager, not captured, launches)
+    void* hook_ctx;
 };
 hipError_t launch_group_step(const GroupPlan& P, hipStream_t s);
 hipError_t prepare_group_update(bool fp64, const GroupUpd& u);

@@ -279,7 +279,7 @@ __global__ __launch_bounds__(64) void k_wall1d(Consts c, DevState* __restrict__ 
     A fh_[EPT], fb_[EPT];
     int dl[EPT];
     double r[2] = {0, 0};
-    const A Hu = (A)pow(lamw / c.pi, 0.5), kf = (A)(c.K / (2.0 * lamw));
+    const double Hud = pow(lamw / c.pi, 0.5), kfd = c.K / (2.0 * lamw);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int k = lane + 64 * e;
@@ -294,8 +294,8 @@ __global__ __launch_bounds__(64) void k_wall1d(Consts c, DevState* __restrict__ 
             fh_[e] = (A)h[o] - hr * (A)sh[o];
             fb_[e] = (A)b[o] - hr * (A)sb[o];
             if (diffuse) {
-                if (!d) r[0] += (double)(wt * u * fh_[e]);
-                else r[1] += (double)(wt * u * Hu * fexp(-(A)lamw * u * u));
+                if (!d) r[0] += (double)wt * (double)u * (double)fh_[e];
+                else r[1] += (double)wt * (double)u * Hud * exp(-lamw * (double)u * (double)u);
             }
         }
     }
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(64) void k_wall1d(Consts c, DevState* __restrict__ 
         }
         rhow = -r[0] / r[1];
     }
-    const A Hc = (A)(rhow * pow(lamw / c.pi, 0.5));
+    const double Hcd = rhow * pow(lamw / c.pi, 0.5);
     double s[3] = {0, 0, 0};
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -315,13 +315,20 @@ __global__ __launch_bounds__(64) void k_wall1d(Consts c, DevState* __restrict__ 
         if (k < c.nn) {
             const A u = unode[k], wt = wnode[k];
             A hf = fh_[e], bf = fb_[e];
+            // conserved-flux sums in FP64 from FP64 per-node values (the
+            // re-emitted mass cancels the incoming mass to FP64 round-off on
+            // the FP32-storage path too); stored fluxes in the storage type
+            const double ud = (double)u, wtd = (double)wt;
+            double hfd = (double)hf, bfd = (double)bf;
             if (diffuse && dl[e]) {
-                hf = Hc * fexp(-(A)lamw * u * u);
-                bf = hf * kf;
+                hfd = Hcd * exp(-lamw * ud * ud);
+                bfd = hfd * kfd;
+                hf = (A)hfd;
+                bf = (A)bfd;
             }
-            s[0] += (double)(wt * u * hf);
-            s[1] += (double)(wt * u * u * hf);
-            s[2] += (double)(wt * u * ((u * u) * hf + bf));
+            s[0] += wtd * ud * hfd;
+            s[1] += wtd * ud * ud * hfd;
+            s[2] += wtd * ud * ((ud * ud) * hfd + bfd);
             fh[(size_t)face * c.ldv + k] = (T)((A)dt * (u * hf));
             fb[(size_t)face * c.ldv + k] = (T)((A)dt * (u * bf));
         }

@@ -185,8 +185,8 @@ __device__ __forceinline__ void k_flux_wall_body(
             A vn, vt, wt, hf, bf;
             int d;
             node(k, vn, vt, wt, d, hf, bf);
-            if (!d) r[0] += (double)(wt * vn * hf);
-            else r[1] += (double)(wt * vn) * exp(-lamw * ((double)vn * vn + (double)vt * vt));
+            if (!d) r[0] += (double)wt * (double)vn * (double)hf;
+            else r[1] += (double)wt * (double)vn * exp(-lamw * ((double)vn * vn + (double)vt * vt));
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -199,24 +199,32 @@ __device__ __forceinline__ void k_flux_wall_body(
         if (R2 == 0.0 && lane == 0) atomicOr(&st->flags, 2);
         rhow = -r[0] / R2;
     }
-    const A Hc = (A)(rhow * (lamw / c.pi)), kf = (A)(c.K / (2.0 * lamw));
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     T* oh = (dir == 0 ? vfh : hfh) + (size_t)face * c.ldv;
     T* ob = (dir == 0 ? vfb : hfb) + (size_t)face * c.ldv;
     const A dtl = (A)(dt * flen);
+    const double Hcd = rhow * (lamw / c.pi), kfd = c.K / (2.0 * lamw);
     for (int k = lane; k < nn; k += WAVE) {
         A vn, vt, wt, hf, bf;
         int d;
         node(k, vn, vt, wt, d, hf, bf);
+        // the conserved-flux sums in FP64 from FP64 per-node values, so that
+        // the re-emitted mass cancels the incoming mass to FP64 round-off
+        // (R1 + rho_w R2 = 0) on the FP32-storage path too; the stored
+        // per-node fluxes are rounded to the storage type
+        const double vnd = (double)vn, vtd = (double)vt, wtd = (double)wt;
+        double hfd = (double)hf, bfd = (double)bf;
         if (diffuse && d) {
-            hf = Hc * fexp(-(A)lamw * (vn * vn + vt * vt));
-            bf = hf * kf;
+            hfd = Hcd * exp(-lamw * (vnd * vnd + vtd * vtd));
+            bfd = hfd * kfd;
+            hf = (A)hfd;
+            bf = (A)bfd;
         }
-        const A wvh = wt * vn * hf;
-        s[0] += (double)wvh;
-        s[1] += (double)(vn * wvh);
-        s[2] += (double)(vt * wvh);
-        s[3] += (double)(wt * vn * ((vn * vn + vt * vt) * hf + bf));
+        const double wvh = wtd * vnd * hfd;
+        s[0] += wvh;
+        s[1] += vnd * wvh;
+        s[2] += vtd * wvh;
+        s[3] += wtd * vnd * ((vnd * vnd + vtd * vtd) * hfd + bfd);
         oh[k] = (T)(dtl * (vn * hf));
         ob[k] = (T)(dtl * (vn * bf));
     }

@@ -56,21 +56,46 @@ hipError_t prepare_group_update(bool fp64, const GroupUpd& u) {
 
 template <typename T>
 static hipError_t group_step_t(const GroupPlan& P, hipStream_t s) {
-    hipLaunchKernelGGL((kg_timestep), dim3(P.g_ts), dim3(RED_NT), 0, s, P.t_ts, P.Ls);
-    hipLaunchKernelGGL((kg_slope<T>), dim3(P.g_slope), dim3(WAVE * SLOPE_CPW), 0, s, P.t_slope,
-                       P.Ls);
-    hipLaunchKernelGGL((kg_fsum<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s, P.t_fsum, P.Ls);
-    hipLaunchKernelGGL((kg_fface), dim3(P.g_fface), dim3(256), 0, s, P.t_fface, P.Ls);
-    hipLaunchKernelGGL((kg_fnode<T>), dim3(P.g_fnode), dim3(WAVE * SPLIT_FPW), 0, s, P.t_fnode,
-                       P.Ls);
-    if (P.g_wall > 0)
+    {
+        HookCall hc(P.hook, P.hook_ctx, "kg_timestep");
+        hipLaunchKernelGGL((kg_timestep), dim3(P.g_ts), dim3(RED_NT), 0, s, P.t_ts, P.Ls);
+    }
+    {
+        HookCall hc(P.hook, P.hook_ctx, "kg_slope");
+        hipLaunchKernelGGL((kg_slope<T>), dim3(P.g_slope), dim3(WAVE * SLOPE_CPW), 0, s,
+                           P.t_slope, P.Ls);
+    }
+    {
+        HookCall hc(P.hook, P.hook_ctx, "kg_fsum");
+        hipLaunchKernelGGL((kg_fsum<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s, P.t_fsum,
+                           P.Ls);
+    }
+    {
+        HookCall hc(P.hook, P.hook_ctx, "kg_fface");
+        hipLaunchKernelGGL((kg_fface), dim3(P.g_fface), dim3(256), 0, s, P.t_fface, P.Ls);
+    }
+    {
+        HookCall hc(P.hook, P.hook_ctx, "kg_fnode");
+        hipLaunchKernelGGL((kg_fnode<T>), dim3(P.g_fnode), dim3(WAVE * SPLIT_FPW), 0, s,
+                           P.t_fnode, P.Ls);
+    }
+    if (P.g_wall > 0) {
+        HookCall hc(P.hook, P.hook_ctx, "kg_wall");
         hipLaunchKernelGGL((kg_wall<T>), dim3(P.g_wall), dim3(256), 0, s, P.t_wall, P.Ls);
-    hipLaunchKernelGGL((kg_cell_pre), dim3(P.g_cpre), dim3(256), 0, s, P.t_cpre, P.Ls);
+    }
+    {
+        HookCall hc(P.hook, P.hook_ctx, "kg_cell_pre");
+        hipLaunchKernelGGL((kg_cell_pre), dim3(P.g_cpre), dim3(256), 0, s, P.t_cpre, P.Ls);
+    }
     for (int q = 0; q < P.nupd; ++q) {
+        HookCall hc(P.hook, P.hook_ctx, "kg_update");
         hipError_t e = group_update_t<T>(P.upd[q], P.Ls, s, false);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((kg_finalize), dim3(P.g_fin), dim3(RED_NT), 0, s, P.t_fin, P.Ls);
+    {
+        HookCall hc(P.hook, P.hook_ctx, "kg_finalize");
+        hipLaunchKernelGGL((kg_finalize), dim3(P.g_fin), dim3(RED_NT), 0, s, P.t_fin, P.Ls);
+    }
     return hipGetLastError();
 }
 

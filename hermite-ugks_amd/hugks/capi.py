@@ -69,7 +69,8 @@ EXPORTS = [
     "hugks_set_schedule", "hugks_get_schedule", "hugks_tune", "hugks_batch_create",
     "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_grouped",
     "hugks_batch_handle",
-    "hugks_batch_advance",
+    "hugks_batch_advance", "hugks_batch_set_timing", "hugks_batch_kernel_times",
+    "hugks_batch_reset_kernel_times",
 ]
 ARR_SHX, ARR_SHY, ARR_SBX, ARR_SBY, ARR_VFH, ARR_VFB, ARR_HFH, ARR_HFB, ARR_VFLUX, ARR_HFLUX = range(10)
 
@@ -122,6 +123,10 @@ def load(path=LIBPATH):
     lib.hugks_batch_grouped.restype = C.c_int32
     lib.hugks_batch_handle.argtypes = [H, C.c_int32, C.POINTER(H)]
     lib.hugks_batch_advance.argtypes = [H, C.c_int32, C.POINTER(StepInfo)]
+    lib.hugks_batch_set_timing.argtypes = [H, C.c_int32]
+    lib.hugks_batch_kernel_times.argtypes = [H, C.c_int32, C.POINTER(C.c_char_p), _dp,
+                                             C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    lib.hugks_batch_reset_kernel_times.argtypes = [H]
     _lib = lib
     return lib
 
@@ -359,6 +364,23 @@ class Batch:
         infos = (StepInfo * len(self.cases))()
         self._check(self.lib.hugks_batch_advance(self.b, int(max_steps), infos))
         return list(infos)
+
+    def set_timing(self, on):
+        """Eager steps with the grouped launches timed (a measurement aid)."""
+        self._check(self.lib.hugks_batch_set_timing(self.b, 1 if on else 0))
+
+    def reset_kernel_times(self):
+        self._check(self.lib.hugks_batch_reset_kernel_times(self.b))
+
+    def kernel_times(self):
+        cap = 32
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        calls = (C.c_int32 * cap)()
+        n = C.c_int32()
+        self._check(self.lib.hugks_batch_kernel_times(self.b, cap, names, ms, calls,
+                                                      C.byref(n)))
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(min(n.value, cap))}
 
     def close(self):
         if self.b:

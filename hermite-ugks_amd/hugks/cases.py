@@ -266,3 +266,25 @@ def from_namelist(path, dim, precision=None):
     if dim == 2:
         return build_2d(params, prec, str(gpu["rt_variant"]), int(gpu["seed"])), gpu
     return build_1d(params, prec), gpu
+
+
+def sweep_c4(seed=7, n=64):
+    """The batched Kn / force sweep: n force-driven channel cases (case 5),
+    nx = 4, Nm = 8, Gauss-Hermite grids; per case Kn log-uniform on
+    [0.01, 1], phi uniform on [0.01, 0.2], ny uniform integer on [32, 128],
+    nv from {16, 20, 24, 28, 32}; every second case is a Fourier-type
+    channel with TwR uniform on [1.1, 1.5] (TwL = 1).  Drawn with numpy's
+    PCG64 from a fixed seed.  Returns the &params dicts."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(n):
+        kn = float(10 ** rng.uniform(-2.0, 0.0))
+        phi = float(rng.uniform(0.01, 0.2))
+        ny = int(rng.integers(32, 129))
+        nv = int(rng.choice([16, 20, 24, 28, 32]))
+        twr = float(rng.uniform(1.1, 1.5)) if k % 2 else 1.0
+        out.append(dict(nx=4, ny=ny, xlength=0.1, ylength=1.0, nv=nv, vmax=5.0, Tref=1.0,
+                        quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=8,
+                        cfl=0.8, maxtime=200.0, case_id=5, knudsen=kn, phi=phi, TwL=1.0,
+                        TwR=twr))
+    return out

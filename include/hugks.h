@@ -279,6 +279,18 @@ int hugks_batch_handle(hugks_batch *b, int32_t i, hugks_handle **h);
  * each case's loop state. */
 int hugks_batch_advance(hugks_batch *b, int32_t max_steps, hugks_step_info *infos);
 
+/* Timing of a batch's grouped launches, a measurement aid: with on = 1,
+ * hugks_batch_advance issues each step's launches eagerly on the batch
+ * stream instead of replaying the graph (the same kernels, the same bits),
+ * bracketing each grouped launch with two HIP events; the cases with their
+ * own launches run untimed.  hugks_batch_kernel_times reports per grouped
+ * kernel (kg_timestep, kg_slope, kg_fsum, kg_fface, kg_fnode, kg_wall,
+ * kg_cell_pre, kg_update, kg_finalize) as hugks_kernel_times does. */
+int hugks_batch_set_timing(hugks_batch *b, int32_t on);
+int hugks_batch_kernel_times(hugks_batch *b, int32_t cap, const char **names, double *ms,
+                             int32_t *calls, int32_t *count);
+int hugks_batch_reset_kernel_times(hugks_batch *b);
+
 /* Loop state (simtime, iter) of the handle. */
 int hugks_get_time(hugks_handle *h, double *simtime, int32_t *iter);
 
@@ -304,7 +316,7 @@ int hugks_kernel_times(hugks_handle *h, int32_t cap, const char **names,
 int hugks_reset_kernel_times(hugks_handle *h);
 /* 1: bracket every launch with two HIP events recorded on the handle's
  * stream (no synchronisation per launch; the elapsed times are collected
- * by hugks_kernel_times, or after 2048 launches); 0: off (default). */
+ * by hugks_kernel_times, or after 4096 launches); 0: off (default). */
 int hugks_set_timing(hugks_handle *h, int32_t on);
 
 /* Library/device information string (gfx target, CU count). */

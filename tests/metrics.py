@@ -13,6 +13,13 @@ force terms, and a one-ulp perturbation of the initial f moves it by
 
 prim_errors(w, wr): per primitive field (rho, U, V, T) on the same idea:
 velocities against the largest speed, T against max T.
+
+prim_errors_thermal(w, wr): as prim_errors with velocities against the
+thermal speed sqrt(max T) (T = 1/lambda, so sqrt(T) is the most probable
+molecular speed sqrt(2RT)); q_error_thermal(q, qr, wr): the heat flux
+against max rho * (max T)^(3/2).  For low-speed flows (Mach 0.01) these
+are the scales on which an FP32 distribution function resolves the
+macroscopic fields.
 """
 import numpy as np
 
@@ -58,3 +65,21 @@ def prim_errors(w, wr, gamma):
 def q_error(q, qr):
     qs = max(np.max(np.sqrt(np.sum(np.asarray(qr) ** 2, axis=1))), 1e-300)
     return np.max(np.abs(np.asarray(q) - np.asarray(qr))) / qs
+
+
+def prim_errors_thermal(w, wr, gamma):
+    """rho and T as prim_errors; velocities against sqrt(max T)."""
+    r, v, t = primitives(w, gamma)
+    rr, vr, tr = primitives(wr, gamma)
+    cth = np.sqrt(np.max(np.abs(tr)))
+    out = [np.max(np.abs(r - rr)) / np.max(np.abs(rr))]
+    for q in range(v.shape[1]):
+        out.append(np.max(np.abs(v[:, q] - vr[:, q])) / cth)
+    out.append(np.max(np.abs(t - tr)) / np.max(np.abs(tr)))
+    return np.array(out)
+
+
+def q_error_thermal(q, qr, wr, gamma):
+    rr, _, tr = primitives(wr, gamma)
+    scale = np.max(np.abs(rr)) * np.max(np.abs(tr)) ** 1.5
+    return np.max(np.abs(np.asarray(q) - np.asarray(qr))) / scale
