@@ -4,7 +4,8 @@ Usage: python hermite-ugks_amd/build_lib.py [--jobs N]
 
 Every translation unit under csrc/ is compiled separately (in parallel)
 and linked into hermite-ugks_amd/libhugks.so.  Packed FP32 instructions
-are disabled for device code.  The Fortran drivers under driver/
+are disabled for device code.  The build fails when the compiler's
+resource-usage remarks report scratch (private) memory for any kernel.  The Fortran drivers under driver/
 (ugks2d_gpu, ugks1d_gpu: the reference's main programs over the C-ABI)
 are then built with amdflang and linked against the library.
 """
@@ -38,9 +39,23 @@ def sources():
 
 def compile_one(src):
     obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r.returncode, r.stderr
+
+
+def scratch_kernels(stderr):
+    """(kernel, bytes per lane) of every kernel whose resource-usage remark
+    reports a private segment (register spills or stack)."""
+    out, name = [], None
+    for ln in stderr.splitlines():
+        if "Function Name:" in ln:
+            name = ln.split("Function Name:")[1].split()[0]
+        elif "ScratchSize [bytes/lane]:" in ln and name:
+            n = int(ln.split("ScratchSize [bytes/lane]:")[1].split()[0])
+            if n > 0:
+                out.append((name, n))
+    return out
 
 
 def build(jobs=None, verbose=False):
@@ -49,19 +64,28 @@ def build(jobs=None, verbose=False):
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
     objs = []
     errors = []
+    spills = []
     with cf.ThreadPoolExecutor(jobs) as ex:
         for src, obj, rc, err in ex.map(compile_one, srcs):
             # keep only real diagnostics (the host pass does not know the
-            # device-only target feature and says so)
+            # device-only target feature and says so; the resource-usage
+            # remarks are read below)
             lines = [ln for ln in err.splitlines()
-                     if "packed-fp32-ops" not in ln and ln.strip()]
+                     if "packed-fp32-ops" not in ln and "kernel-resource-usage" not in ln
+                     and ln.strip()]
             if rc != 0:
                 errors.append(f"{src}:\n" + "\n".join(lines))
             elif verbose and lines:
                 print("\n".join(lines), file=sys.stderr)
+            spills += [(os.path.basename(src), k, n) for k, n in scratch_kernels(err)]
             objs.append(obj)
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
+    # no shipped kernel may use scratch (private) memory: a spill there is
+    # a silent slowdown and a class of fault this library rules out
+    if spills:
+        raise RuntimeError("kernels with scratch memory:\n" +
+                           "\n".join(f"  {f}: {k} {n} bytes/lane" for f, k, n in spills))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -546,11 +546,23 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         ALLOC(L.fmaxw, (nint ? nint : 1) * FACE_MAXW_LD * sizeof(double));
         ALLOC(L.cellpre, ncell * 24 * sizeof(double));
         L.update_lds = update2d_lds_bytes(c, h->fp64);
-        L.update_mode = update2d_mfma_supported(c) ? 1 : 0;
-        if (const char* um = getenv("HUGKS_UPDATE_KERNEL"))
-            if (strcmp(um, "lds") == 0) L.update_mode = 0;
         hipDeviceProp_t prop;
         HCHK(hipGetDeviceProperties(&prop, h->device));
+        L.lds_limit = prop.sharedMemPerBlockOptin > 0 ? (size_t)prop.sharedMemPerBlockOptin
+                                                      : (size_t)prop.sharedMemPerBlock;
+        L.update_nw = update2d_mx_default_waves(c);
+        // default: the MFMA node update, else the MFMA Hermite tiles, else
+        // the LDS kernel
+        L.update_mode = update2d_mx_supported(c, h->fp64, L.lds_limit, L.update_nw) ? 2
+                        : update2d_mfma_supported(c)                               ? 1
+                                                                                   : 0;
+        if (const char* um = getenv("HUGKS_UPDATE_KERNEL")) {
+            if (strcmp(um, "lds") == 0) L.update_mode = 0;
+            if (strcmp(um, "mfma") == 0 && update2d_mfma_supported(c)) L.update_mode = 1;
+            if (strcmp(um, "mfma-node") == 0 &&
+                update2d_mx_supported(c, h->fp64, L.lds_limit, L.update_nw))
+                L.update_mode = 2;
+        }
         if (L.update_lds > (size_t)prop.sharedMemPerBlockOptin && prop.sharedMemPerBlockOptin > 0) {
             release(h);
             delete h;
@@ -960,15 +972,37 @@ int hugks_set_timing(hugks_handle* h, int32_t on) {
 int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kernel) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
     if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_STENCIL ||
-        update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA)
+        update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA_NODE)
         return fail(h, HUGKS_E_ARG, "unknown kernel choice");
     if (h->dim != 2) return HUGKS_OK;   // 1D has one kernel of each kind
     if (update_kernel == HUGKS_UPDATE_MFMA && !update2d_mfma_supported(h->L.c))
         return fail(h, HUGKS_E_UNSUPPORTED, "MFMA update needs nv <= 48 and Nm <= 14");
+    if (update_kernel == HUGKS_UPDATE_MFMA_NODE &&
+        !update2d_mx_supported(h->L.c, h->fp64, h->L.lds_limit, h->L.update_nw))
+        return fail(h, HUGKS_E_UNSUPPORTED,
+                    "MFMA node update needs nv <= 48, Nm <= 14 and its LDS on the device");
     if (flux_kernel == HUGKS_FLUX_STENCIL && !h->L.stencil_ok)
         return fail(h, HUGKS_E_UNSUPPORTED, "stencil flux needs distribution arrays under 4 GiB");
     h->L.flux_mode = flux_kernel;
     h->L.update_mode = update_kernel;
+    return HUGKS_OK;
+}
+
+int hugks_set_update_waves(hugks_handle* h, int32_t waves) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    if (h->dim != 2) return HUGKS_OK;
+    const int nw = waves > 0 ? waves : update2d_mx_default_waves(h->L.c);
+    if (nw > 16 || !update2d_mx_supported(h->L.c, h->fp64, h->L.lds_limit, nw))
+        return fail(h, HUGKS_E_UNSUPPORTED,
+                    "no MFMA node-update kernel with " + std::to_string(nw) +
+                        " wavefronts per cell for this velocity grid");
+    h->L.update_nw = nw;
+    return HUGKS_OK;
+}
+
+int hugks_get_update_waves(hugks_handle* h, int32_t* waves) {
+    if (!h || !waves) return fail(h, HUGKS_E_ARG, "null handle or output");
+    *waves = h->dim == 2 ? h->L.update_nw : 0;
     return HUGKS_OK;
 }
 
@@ -1016,7 +1050,7 @@ static int tune_case(const hugks_desc* d, int* flux_kernel, int* update_kernel, 
     double best = 1e300;
     if (d->dim == 2) {
         for (int fk = HUGKS_FLUX_FUSED; fk <= HUGKS_FLUX_STENCIL; ++fk)
-            for (int uk = HUGKS_UPDATE_LDS; uk <= HUGKS_UPDATE_MFMA; ++uk) {
+            for (int uk = HUGKS_UPDATE_LDS; uk <= HUGKS_UPDATE_MFMA_NODE; ++uk) {
                 if (hugks_set_schedule(h, fk, uk) != HUGKS_OK) continue;
                 hipError_t e = launch_begin(h->L.st, INFINITY, h->stream);
                 for (int s = 0; s < 2 && e == hipSuccess; ++s) e = enqueue_step_on(h, h->stream, false);
@@ -1119,7 +1153,8 @@ extern "C" {
 // the other cases keep their own launches on the lane streams.
 static bool case_groupable(const hugks_handle* h) {
     const Launch2D& L = h->L;
-    return h->dim == 2 && L.wx && update2d_mfma_supported(L.c) &&
+    return h->dim == 2 && L.wx &&
+           update2d_mx_supported(L.c, h->fp64, L.lds_limit, L.update_nw) &&
            L.c.nx * L.c.ny < PRED_MIN_CELLS;
 }
 static std::vector<char> batch_groupable(const hugks_batch* b) {
@@ -1173,7 +1208,7 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
         group_update_key(h->L, &e, &t, &v);
         const auto key = std::make_tuple(e, t, v);
         upd[key].push_back({i, h->L.c.nx * h->L.c.ny});
-        const size_t lds = update2d_mfma_lds_bytes(h->L.c, h->fp64);
+        const size_t lds = group_update_lds(h->L);
         upd_lds[key] = std::max(upd_lds[key], lds);
     }
     GroupPlan P{};
@@ -1212,8 +1247,8 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
     int q = 0;
     for (auto& kv : upd) {
         GroupUpd& u = P.upd[P.nupd++];
-        u.ept = std::get<0>(kv.first);
-        u.nt = std::get<1>(kv.first);
+        u.nw = std::get<0>(kv.first);
+        u.maxp = std::get<1>(kv.first);
         u.nvc = std::get<2>(kv.first);
         u.blocks = blocks[k_up[q]];
         u.tab = tp(k_up[q]);
@@ -1237,7 +1272,7 @@ static hipError_t batch_group_plan(hugks_batch* b) {
     std::vector<int> ids[2];
     for (int i = 0; i < n; ++i) {
         if (b->in_group[i]) {
-            hugks_set_schedule(b->cases[i], HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA);
+            hugks_set_schedule(b->cases[i], HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA_NODE);
             ids[b->cases[i]->fp64 ? 1 : 0].push_back(i);
         }
         Ls[i] = b->cases[i]->L;

@@ -50,6 +50,17 @@ struct DevState {
 };
 
 // ---------------------------------------------------------------- scalars
+// Boundary kind and wall temperature of side s (0..3) by selects, not by
+// a run-time index into the arrays of c: a Consts copied from memory (the
+// grouped kernels read each case's from a table) would otherwise be kept
+// in scratch memory to be indexed.
+__device__ __forceinline__ int bc_of(const Consts& c, int s) {
+    return s == 0 ? c.bc[0] : s == 1 ? c.bc[1] : s == 2 ? c.bc[2] : c.bc[3];
+}
+__device__ __forceinline__ double tw_of(const Consts& c, int s) {
+    return s == 0 ? c.tw[0] : s == 1 ? c.tw[1] : s == 2 ? c.tw[2] : c.tw[3];
+}
+
 // get_primary / get_conserved, 2d/src/gastheory2d.f90:28-46
 __device__ __forceinline__ void prim_from_w(const Consts& c, const double* w, double* p) {
     p[0] = w[0];
@@ -156,12 +167,26 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
             for (int i = 0; i < NV; ++i) lds[wid * NV + i] = v[i];
         }
         __syncthreads();
+        // waves in index order for every value; for large blocks one
+        // wave's partials in registers at a time (the same additions): with
+        // the loop over waves unrolled the compiler keeps all NV x NT/64
+        // partials live (a 1024-thread block with NV = 6 spilled to scratch)
+        if constexpr (NV * (NT / WAVE) <= 32) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double s = lds[i];
+            for (int i = 0; i < NV; ++i) {
+                double s = lds[i];
 #pragma unroll
-            for (int w = 1; w < NT / WAVE; ++w) s += lds[w * NV + i];
-            v[i] = s;
+                for (int w = 1; w < NT / WAVE; ++w) s += lds[w * NV + i];
+                v[i] = s;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v[i] = lds[i];
+#pragma unroll 1
+            for (int w = 1; w < NT / WAVE; ++w) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) v[i] += lds[w * NV + i];
+            }
         }
         __syncthreads();
     } else {
@@ -278,7 +303,7 @@ struct SlopeNb {
 };
 __host__ __device__ __forceinline__ SlopeNb slope_nb(const Consts& c, int cell, int dir) {
     const int i = cell % c.nx, j = cell / c.nx;
-    const bool per = c.bc[2 * dir] == HUGKS_BC_PERIODIC;
+    const bool per = bc_of(c, 2 * dir) == HUGKS_BC_PERIODIC;
     const int n = dir == 0 ? c.nx : c.ny;
     const int p = dir == 0 ? i : j;
     const int step = dir == 0 ? 1 : c.nx;
@@ -335,6 +360,21 @@ __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
     else
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+// Raw buffer store with the non-temporal cache policy (the data is read
+// back only by a later kernel); an offset at or past the resource's size
+// is dropped by the hardware range check.
+template <typename T>
+__device__ __forceinline__ void buf_st_nt(T v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    constexpr int kNonTemporal = 2;
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0,
+                                              kNonTemporal);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(unsigned __attribute__((vector_size(8))), v), r, (int)off, 0,
+            kNonTemporal);
 }
 
 template <typename T> struct Arith;           // per-node arithmetic type

@@ -33,7 +33,10 @@ struct Launch2D {
     int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline,
                          // 2 the pipeline with slopes from the cell stencils
     int stencil_ok;      // distribution arrays under 4 GiB (32-bit byte offsets)
-    int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles
+    int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles,
+                         // 2 MFMA node update (force and collision on tiles)
+    int update_nw;       // wavefronts per cell of update mode 2
+    size_t lds_limit;    // largest dynamic LDS of one workgroup on the device
     double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
     double* cellpre;     // per-cell scalars of the MFMA update [ncell][24]
     double* red;         // partials of the cell reductions [RED_SCRATCH]
@@ -98,7 +101,7 @@ hipError_t launch_convert_in(const double* src, void* dst, bool dst_fp64, size_t
 // Grouped launches of a batch (k_group.h): device tables and grid sizes of
 // one batch step; updates are grouped by kernel instance.
 struct GroupUpd {
-    int ept, nt, nvc;    // the update instance (UpdKey)
+    int nw, maxp, nvc;   // the node-update instance (mx_key)
     int blocks;
     size_t lds;          // largest LDS of the instance's cases
     const int* tab;
@@ -119,14 +122,20 @@ hipError_t prepare_group_update(bool fp64, const GroupUpd& u);
 // fnode, wall, cell_pre; time step and finalisation take one each)
 void group_blocks(const Launch2D& L, int* slope, int* fsum, int* fface, int* fnode, int* wall,
                   int* cpre);
-// update instance of a case as (ept, nt, nvc)
-void group_update_key(const Launch2D& L, int* ept, int* nt, int* nvc);
+// node-update instance of a case as (wavefronts per cell, tile positions
+// per wavefront, compiled-in nv or 0) and its dynamic LDS bytes
+void group_update_key(const Launch2D& L, int* nw, int* maxp, int* nvc);
+size_t group_update_lds(const Launch2D& L);
 
 // 1D path
 hipError_t launch_step1d(const Launch2D& L, hipStream_t s);
 
 size_t update2d_lds_bytes(const Consts& c, bool fp64);
 bool update2d_mfma_supported(const Consts& c);
+// update mode 2 with nw wavefronts per cell: a compiled instance, nv <= 48,
+// Nm <= 14 and its LDS within lds_limit bytes
+bool update2d_mx_supported(const Consts& c, bool fp64, size_t lds_limit, int nw);
+int update2d_mx_default_waves(const Consts& c);
 size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64);
 
 }  // namespace hugks

@@ -273,8 +273,8 @@ __global__ __launch_bounds__(64) void k_wall1d(Consts c, DevState* __restrict__ 
     const int cell = side ? c.nx - 1 : 0, face = side ? c.nx : 0;
     const int lane = threadIdx.x;
     const double dt = st->dt;
-    const bool diffuse = c.bc[side] == HUGKS_BC_DIFFUSE;
-    const double lamw = 1.0 / c.tw[side];
+    const bool diffuse = bc_of(c, side) == HUGKS_BC_DIFFUSE;
+    const double lamw = 1.0 / tw_of(c, side);
     const A hr = (A)rot * (A)0.5 * (A)c.dx;
     A fh_[EPT], fb_[EPT];
     int dl[EPT];

@@ -133,7 +133,7 @@ __device__ __forceinline__ void k_flux_wall_body(
     if (f >= nwall) return;
     int side = -1, idx = 0;
     for (int s = 0; s < 4; ++s) {
-        if (c.bc[s] == HUGKS_BC_PERIODIC) continue;
+        if (bc_of(c, s) == HUGKS_BC_PERIODIC) continue;
         const int n = s < 2 ? c.ny : c.nx;
         if (f < n) { side = s; idx = f; break; }
         f -= n;
@@ -155,10 +155,10 @@ __device__ __forceinline__ void k_flux_wall_body(
     const double len = dir == 0 ? c.dx : c.dy;
     const double flen = dir == 0 ? c.dy : c.dx;
     const A hrl = (A)rot * (A)0.5 * (A)len;
-    const bool diffuse = c.bc[side] == HUGKS_BC_DIFFUSE;
+    const bool diffuse = bc_of(c, side) == HUGKS_BC_DIFFUSE;
     const int nv = c.nv, nn = c.nn;
     const size_t base = (size_t)cell * c.ldv;
-    const double lamw = 1.0 / c.tw[side];
+    const double lamw = 1.0 / tw_of(c, side);
 
     // incoming value at node k (cell value extrapolated to the face,
     // flux.f90:509-510), and for a specular wall its mirror node

@@ -10,7 +10,7 @@
 #include "k_flux2d.h"
 #include "k_flux_split.h"
 #include "k_misc.h"
-#include "k_update2d_mfma.h"
+#include "k_update2d_mx.h"
 
 namespace hugks {
 
@@ -116,17 +116,16 @@ __global__ __launch_bounds__(256) void kg_cell_pre(const int* __restrict__ tab,
                     g.nblk);
 }
 
-template <typename T, int EPT, int NT, int NVC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
-void kg_update(const int* __restrict__ tab,
-                                                const Launch2D* __restrict__ Ls) {
-    using A = typename Arith<T>::A;
+template <typename T, int NW, int MAXP, int NVC>
+__global__ __launch_bounds__(NW * WAVE)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void kg_update(
+    const int* __restrict__ tab, const Launch2D* __restrict__ Ls) {
     const GroupIdx g = group_index(tab);
     const Launch2D& L = Ls[g.cs];
-    k_update2d_mfma_body<T, EPT, NT, NVC>(L.c, L.st, L.cellpre, (T*)L.h, (T*)L.b,
-                                          (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,
-                                          (const T*)L.hfb, (const A*)L.wnode, L.wnode_d, L.u1,
-                                          L.v1, g.bid, g.nblk);
+    k_update2d_mx_body<T, NW, MAXP, NVC>(L.c, L.st, L.cellpre, (T*)L.h, (T*)L.b,
+                                         (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,
+                                         (const T*)L.hfb, (const typename Arith<T>::A*)L.wnode,
+                                         L.u1, L.v1, g.bid, g.nblk);
 }
 
 __global__ __launch_bounds__(RED_NT) void kg_finalize(const int* __restrict__ tab,

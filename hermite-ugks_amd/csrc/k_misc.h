@@ -26,11 +26,13 @@ __device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) lds[wid * NV + i] = v[i];
     __syncthreads();
+    // one wave's partials in registers at a time (as block_sum)
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double s = lds[i];
-        for (int w = 1; w < NT / WAVE; ++w) s = fmax(s, lds[w * NV + i]);
-        v[i] = s;
+    for (int i = 0; i < NV; ++i) v[i] = lds[i];
+#pragma unroll 1
+    for (int w = 1; w < NT / WAVE; ++w) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = fmax(v[i], lds[w * NV + i]);
     }
     __syncthreads();
 }

@@ -1,0 +1,481 @@
+// 2D cell update with the whole node update on MFMA (gfx950).
+//
+// Same step as k_update2d.h - update(), 2d/src/solver.f90:245-395, with
+// symmetric_hermite_force_term_2d, 2d/src/hermite_symmetric2d.f90:23-104,
+// and Shakhov-BGK with the trapezoidal collision (solver.f90:337-388) - with
+// everything node-wise except the face-flux divergence moved onto 16x16x4
+// matrix tiles (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64):
+//
+//   C  = Hx G Hy^T          Hermite coefficients, G = weight * f
+//   D  = C / r!s! / rt      zero where r + s > Nm
+//   R  = (ax D1 Hy + ay D Hy1) o OMy
+//   F^T(jv, iu) = sum_r R(r, jv) Ho(r, iu) + sum_t Rc(t, jv) L(t, iu)
+//
+// with Ho(r, iu) = He_r(xi_iu) OMx(iu) dt/(2 pi sqrt(rt)) / (1 + dt/2tau):
+// the force weight exp(-(xi^2 + eta^2)/2) factors over the two velocity
+// axes, and so do the collision terms: with the per-axis Maxwellian factors
+// ex(u) ey(v), the Shakhov projections cq = qx(u) + qy(v) and the
+// polynomial 2 lam |c|^2 + K - 5 = x(u) + y(v),
+//   H' = H (1 + cq (x + y))
+//      = ex(1 + qx x) (x) ey + ex qx (x) ey y + ex x (x) ey qy + ex (x) ey qy y,
+// four outer products per time level (and B' = kf (H' + 2 cq H) the same
+// with x + 2 in place of x).  So the eight rows t of L (per distribution)
+// and Rc (shared) carry a_n H'^{n+1} + a_o H'^n, scaled by 1/(1 + dt/2tau)
+// like Ho, and the accumulators of the F^T tiles hold
+//   (a_n H'^{n+1} + a_o H'^n + dt F) / (1 + dt/2tau)
+// for the lane's nodes; the update is then
+//   h^{n+1} = (1 - a_o)/(1 + a_n) h^n + div/(area (1 + a_n)) + acc.
+// The heat flux at t^n (solver.f90:337) is a combination of the Hermite
+// coefficients of order <= 3 (xi^3 = He3 + 3 He1, xi eta^2 = He1 He2 + He1):
+//   qx = (rt^{3/2} (C30 + C12 + 4 C10) + rt^{1/2} C10[b]) / 2.
+//
+// Node (iu, jv) of an F^T tile sits in the accumulator element of the lane
+// that loaded it: f^n and b^n are read once, at the start, in that layout
+// and kept in registers; the eight face fluxes of a tile position are read
+// before its tiles (the first position's during the R contraction), and
+// h^{n+1}, b^{n+1} are written from the accumulators.  One workgroup of NW wavefronts per cell; wavefront w owns
+// the tile positions w, w + NW, ... (MAXP at most).
+#pragma once
+#include "k_update2d_mfma.h"
+
+namespace hugks {
+
+// 1/r! for r = 0..15
+static __constant__ const double kInvFact[MF_NB] = {
+    1.0, 1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0, 1.0 / 5040.0,
+    1.0 / 40320.0, 1.0 / 362880.0, 1.0 / 3628800.0, 1.0 / 39916800.0, 1.0 / 479001600.0,
+    1.0 / 6227020800.0, 1.0 / 87178291200.0, 1.0 / 1307674368000.0};
+
+// LDS layout of one cell (elements of T).  Row strides as in k_update2d_mfma.h:
+// a tile operand read as 16 rows x 4 columns needs 4 x odd (Hx, Hy, Tm, D,
+// G), one read as 4 rows x 16 columns an odd multiple of 16 (R, Rc).
+//   region 0: G [2][nv][ldG] (weight * f), later D [2][16][sD], L [2][8][sH]
+//             and Rc [8][sR]
+//   Hx, Ho, Hy [16][sH]
+//   region T: Tm [2][16][sH], later R [2][16][sR]
+//   OMy [nv]
+struct MxLayout {
+    int np, sH, sR, sD, ldG;
+    int oD, oL, oRc, oHx, oHo, oHy, oT, oOM, end;
+};
+__host__ __device__ inline MxLayout mx_layout(int nv) {
+    MxLayout L;
+    L.np = mf_np(nv);
+    L.sH = L.np + 4;
+    L.sR = ((L.np / 16) & 1) ? L.np : L.np + 16;
+    L.sD = MF_NB + 4;
+    L.ldG = (nv + 3) / 4 * 4;
+    if (((L.ldG / 4) & 1) == 0) L.ldG += 4;
+    L.oD = 0;
+    L.oL = L.oD + 2 * MF_NB * L.sD;
+    L.oRc = L.oL + 2 * 8 * L.sH;
+    const int r0 = L.oRc + 8 * L.sR;
+    const int g = 2 * nv * L.ldG;
+    L.oHx = g > r0 ? g : r0;
+    L.oHo = L.oHx + MF_NB * L.sH;
+    L.oHy = L.oHo + MF_NB * L.sH;
+    L.oT = L.oHy + MF_NB * L.sH;
+    L.oOM = L.oT + 2 * MF_NB * (L.sH > L.sR ? L.sH : L.sR);
+    L.end = L.oOM + nv;
+    return L;
+}
+__host__ __device__ inline size_t mx_lds_bytes(int nv, size_t tsize) {
+    return (size_t)mx_layout(nv).end * tsize;
+}
+
+// Compiled instances (wavefronts per cell, tile positions per wavefront,
+// compiled-in nv or 0 for any nv <= 48).
+#define HUGKS_UPDX(X) X(1, 1, 16) X(4, 1, 20) X(2, 2, 20) X(4, 1, 24) X(2, 2, 24) \
+    X(4, 1, 28) X(2, 2, 28) X(4, 1, 32) X(2, 2, 32) X(3, 3, 40) X(5, 2, 40) X(9, 1, 40) \
+    X(1, 1, 0) X(4, 1, 0) X(2, 2, 0) X(3, 3, 0) X(5, 2, 0)
+
+// The wavefronts per cell a case runs with by default: 3 for nv in
+// 33..48 (nine tile positions, three per wavefront), 4 for 17..32, 1 up to
+// 16.
+__host__ __device__ inline int mx_default_nw(int nv) {
+    const int nt = mf_np(nv) / 16;
+    return nt >= 3 ? 3 : (nt == 2 ? 4 : 1);
+}
+__host__ __device__ inline int mx_maxp(int nv, int nw) {
+    const int nt = mf_np(nv) / 16;
+    return (nt * nt + nw - 1) / nw;
+}
+// The instance (nw, maxp, nvc) for nv and nw; nvc = 0 when nv has no
+// compiled-in instance.  Returns false when no instance runs nw at nv.
+__host__ inline bool mx_key(int nv, int nw, int* maxp, int* nvc) {
+    const int mp = mx_maxp(nv, nw);
+    bool in = false, any = false;
+#define HUGKS_UPDX_HAS(W, P, V)                  \
+    if (W == nw && P == mp && V == nv) in = true; \
+    if (W == nw && P == mp && V == 0) any = true;
+    HUGKS_UPDX(HUGKS_UPDX_HAS)
+#undef HUGKS_UPDX_HAS
+    if (!in && !any) return false;
+    *maxp = mp;
+    *nvc = in ? nv : 0;
+    return true;
+}
+
+// The value, opaque to the optimizer: keeps a conversion of it where it is
+// written instead of next to the load that produced the value (where its
+// wider result would stay live across the kernel's phases).
+template <typename X>
+__device__ __forceinline__ X mx_opaque(X x) {
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
+
+template <typename T, int NW, int MAXP, int NVC>
+__device__ __forceinline__ void k_update2d_mx_body(
+    const Consts& c, const DevState* __restrict__ st, const double* __restrict__ cp,
+    T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
+    const T* __restrict__ hfh, const T* __restrict__ hfb,
+    const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ u1,
+    const double* __restrict__ v1, int bid, int nblk) {
+    using A = typename Arith<T>::A;
+    using M = Mfma<T>;
+    using V = typename M::V;
+    constexpr int NT = NW * WAVE;
+    HIP_DYNAMIC_SHARED(double, smem_raw)
+    // unscaled coefficients C10, C30, C12, C01, C03, C21 of h, C10, C01 of b
+    __shared__ double qm[8];
+    if (st->stopped) return;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    const int cell = tiled_cell((int)xcd_block(bid, nblk), c.nx, c.ny);
+    const int nx = c.nx, nm = c.nm;
+    const int nv = NVC ? NVC : c.nv;
+    const int np = mf_np(nv), nt = np / 16, npos = nt * nt;
+    const int i = cell % nx, j = cell / nx;
+    const double dt = st->dt;
+    const double* P = cp + (size_t)CP_LD * cell;
+    double po[4], pn[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        po[q] = P[CP_PO + q];
+        pn[q] = P[CP_PN + q];
+    }
+    const double iden = P[CP_IDEN];
+    const double rt = P[CP_RT];
+    const double sqrt_rt = sqrt(rt);
+
+    const MxLayout ly = mx_layout(nv);
+    const int sH = ly.sH, sR = ly.sR, sD = ly.sD, ldG = ly.ldG, GS = nv * ly.ldG;
+    T* const sm = reinterpret_cast<T*>(smem_raw);
+    T* const G = sm;
+    T* const D = sm + ly.oD;       // G is dead once Tm is formed
+    T* const Lr = sm + ly.oL;
+    T* const Rc = sm + ly.oRc;
+    T* const Hx = sm + ly.oHx;
+    T* const Ho = sm + ly.oHo;
+    T* const Hy = sm + ly.oHy;
+    T* const Tm = sm + ly.oT;
+    T* const R = Tm;               // Tm is dead once D is formed
+    T* const OMy = sm + ly.oOM;
+
+    // ---- Hermite bases He_0..He_15 at xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt)
+    // (build_probabilists_hermite, hermite_symmetric2d.f90:220-233); Ho
+    // carries the force weight of the u axis, dt/(2 pi sqrt(rt)) and
+    // 1/(1 + dt/2tau); zero columns beyond nv
+    {
+        const A oscale = (A)(dt / (2.0 * c.pi * sqrt_rt) * iden);
+        for (int t = tid; t < 2 * np; t += NT) {
+            const bool isx = t < np;
+            const int idx = isx ? t : t - np;
+            if (idx < nv) {
+                const double xv = isx ? u1[idx] : v1[idx];
+                const A x = (A)((xv - (isx ? po[1] : po[2])) / sqrt_rt);
+                const A om = fexp((A)-0.5 * x * x);
+                const A ho = isx ? oscale * om : (A)0;
+                T* Hb = isx ? Hx : Hy;
+                A p0 = (A)1, p1 = x;
+                Hb[idx] = (T)p0;
+                Hb[sH + idx] = (T)p1;
+                if (isx) {
+                    Ho[idx] = (T)(ho * p0);
+                    Ho[sH + idx] = (T)(ho * p1);
+                } else {
+                    OMy[idx] = (T)om;
+                }
+                for (int n = 2; n < MF_NB; ++n) {
+                    const A p2 = x * p1 - (A)(n - 1) * p0;
+                    Hb[n * sH + idx] = (T)p2;
+                    if (isx) Ho[n * sH + idx] = (T)(ho * p2);
+                    p0 = p1;
+                    p1 = p2;
+                }
+            } else {
+                T* Hb = isx ? Hx : Hy;
+                for (int n = 0; n < MF_NB; ++n) {
+                    Hb[n * sH + idx] = (T)0;
+                    if (isx) Ho[n * sH + idx] = (T)0;
+                }
+            }
+        }
+    }
+
+    // ---- f^n and b^n of the lane's nodes, in the layout of the F^T
+    // accumulators (iu = 16 it + lane % 16, jv = 16 jt + row(lane, v)), kept
+    // in registers for the update; weight * f to LDS for the coefficients.
+    // Loads and stores are raw buffer accesses on one resource per array
+    // row with one 32-bit byte offset per node, shared by all the arrays: a
+    // node outside the grid gets an offset past the row, which reads zero
+    // and drops a store, so no access sits under a branch (a load under a
+    // branch is waited for inside it, which serialises the loads).
+    const int vl = i + (nx + 1) * j, vr = vl + 1;
+    const int hd = i + nx * j, hu = i + nx * (j + 1);
+    const size_t ldv = (size_t)c.ldv;
+    const uint32_t rowb = (uint32_t)(nv * nv * (int)sizeof(T));
+    constexpr uint32_t OUT = 0x80000000u;   // past any row
+    const __amdgpu_buffer_rsrc_t rsh = buf_rsrc(h + (size_t)cell * ldv, rowb);
+    const __amdgpu_buffer_rsrc_t rsb = buf_rsrc(b + (size_t)cell * ldv, rowb);
+    const __amdgpu_buffer_rsrc_t rsw = buf_rsrc(wnode, rowb);
+    const int lc = lane & 15;
+    auto node_off = [&](int pi, int v) -> uint32_t {
+        const int p = wv + NW * pi;
+        const int iu = 16 * (p % nt) + lc, jv = 16 * (p / nt) + M::row(lane, v);
+        return (p < npos && iu < nv && jv < nv) ? (uint32_t)((iu + nv * jv) * (int)sizeof(T))
+                                                : OUT;
+    };
+    A rh[MAXP][4], rb[MAXP][4];
+#pragma unroll
+    for (int pi = 0; pi < MAXP; ++pi) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t o = node_off(pi, v);
+            rh[pi][v] = (A)buf_ld<T>(rsh, o);
+            rb[pi][v] = (A)buf_ld<T>(rsb, o);
+        }
+    }
+#pragma unroll
+    for (int pi = 0; pi < MAXP; ++pi) {
+        const int p = wv + NW * pi;
+        const int iu = 16 * (p % nt) + lc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t o = node_off(pi, v);
+            const int jv = 16 * (p / nt) + M::row(lane, v);
+            if (o != OUT) {
+                const A wa = buf_ld<A>(rsw, o / (uint32_t)sizeof(T) * (uint32_t)sizeof(A));
+                G[jv * ldG + iu] = (T)(wa * rh[pi][v]);
+                G[GS + jv * ldG + iu] = (T)(wa * rb[pi][v]);
+            }
+        }
+    }
+    // the eight face fluxes of one tile position's nodes (left/right,
+    // down/up faces of h, then of b)
+    const __amdgpu_buffer_rsrc_t rsf[8] = {
+        buf_rsrc(vfh + (size_t)vl * ldv, rowb), buf_rsrc(vfh + (size_t)vr * ldv, rowb),
+        buf_rsrc(hfh + (size_t)hd * ldv, rowb), buf_rsrc(hfh + (size_t)hu * ldv, rowb),
+        buf_rsrc(vfb + (size_t)vl * ldv, rowb), buf_rsrc(vfb + (size_t)vr * ldv, rowb),
+        buf_rsrc(hfb + (size_t)hd * ldv, rowb), buf_rsrc(hfb + (size_t)hu * ldv, rowb)};
+    auto load_faces = [&](int pi, A (&fvo)[8][4]) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t o = node_off(pi, v);
+#pragma unroll
+            for (int a = 0; a < 8; ++a) fvo[a][v] = (A)buf_ld<T>(rsf[a], o);
+        }
+    };
+    A fv[8][4];
+    if constexpr (MAXP == 1) load_faces(0, fv);
+    __syncthreads();
+
+    const int ksn = (nv + 3) / 4;       // K steps over a velocity index
+    const int lr = lane & 15, lk = lane >> 4;
+
+    // ---- Tm = Hx G: tile (f, jt), rows r, columns jv
+    for (int job = wv; job < 2 * nt; job += NW) {
+        const int f = job / nt, jt = job % nt;
+        const int jv = 16 * jt + lr;
+        V acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < ksn; ++ks) {
+            const int iu = 4 * ks + lk;
+            const bool in = iu < nv;
+            const T a = in ? Hx[lr * sH + iu] : (T)0;
+            const T bb = (in && jv < nv) ? G[f * GS + jv * ldG + iu] : (T)0;
+            acc = M::mma(a, bb, acc);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Tm[(f * MF_NB + M::row(lane, v)) * sH + jv] = acc[v];
+    }
+    __syncthreads();
+
+    // ---- D = Tm Hy^T / (r! s! rt), r + s <= Nm; the coefficients of order
+    // <= 3 that give the heat flux kept unscaled
+    const double jac = 1.0 / rt;
+    for (int f = wv; f < 2; f += NW) {
+        V acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < ksn; ++ks) {
+            const int jv = 4 * ks + lk;
+            const T a = Tm[(f * MF_NB + lr) * sH + jv];     // zero beyond nv
+            const T bb = Hy[lr * sH + jv];
+            acc = M::mma(a, bb, acc);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = M::row(lane, v), s = lr;
+            T val = (T)0;
+            if (r + s <= nm) val = (T)((A)(jac * kInvFact[r] * kInvFact[s]) * (A)acc[v]);
+            D[(f * MF_NB + r) * sD + s] = val;
+            const int rs = 16 * r + s;
+            int slot = -1;
+            if (f == 0)
+                slot = rs == 16 ? 0 : rs == 48 ? 1 : rs == 18 ? 2 : rs == 1 ? 3 : rs == 3 ? 4
+                     : rs == 33 ? 5 : -1;
+            else
+                slot = rs == 16 ? 6 : rs == 1 ? 7 : -1;
+            if (slot >= 0) qm[slot] = (double)acc[v];
+        }
+    }
+    __syncthreads();
+
+    // ---- heat flux at t^n from the coefficients, then the collision rows:
+    // L (u axis, per distribution) and Rc (v axis, shared)
+    {
+        const double s1 = sqrt_rt, s3 = rt * sqrt_rt;
+        const double qx = 0.5 * (s3 * (qm[1] + qm[2] + 4.0 * qm[0]) + s1 * qm[6]);
+        const double qy = 0.5 * (s3 * (qm[4] + qm[5] + 4.0 * qm[3]) + s1 * qm[7]);
+        const double an = 0.5 * dt * P[CP_ITAU] * iden, ao = 0.5 * dt * P[CP_ITAUO] * iden;
+        const double kfo = P[CP_KFO], kfn = P[CP_KFN];
+        const double cso = P[CP_CSO], csn = P[CP_CSN];
+        for (int t = tid; t < 2 * np; t += NT) {
+            const bool isx = t < np;
+            const int idx = isx ? t : t - np;
+            double r8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, b8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (idx < nv) {
+                const double xv = isx ? u1[idx] : v1[idx];
+                const double co = xv - (isx ? po[1] : po[2]), cn = xv - (isx ? pn[1] : pn[2]);
+                const double e0 = (isx ? P[CP_HCO] : 1.0) * exp(-po[3] * co * co);
+                const double e1 = (isx ? P[CP_HCN] : 1.0) * exp(-pn[3] * cn * cn);
+                const double k5 = isx ? c.K - 5.0 : 0.0;
+                const double x0 = 2.0 * po[3] * co * co + k5, x1 = 2.0 * pn[3] * cn * cn + k5;
+                const double q0 = cso * (isx ? qx : qy) * co, q1 = csn * (isx ? qx : qy) * cn;
+                if (isx) {
+                    r8[0] = an * e1 * (1.0 + q1 * x1);
+                    r8[1] = an * e1 * q1;
+                    r8[2] = an * e1 * x1;
+                    r8[3] = an * e1;
+                    r8[4] = ao * e0 * (1.0 + q0 * x0);
+                    r8[5] = ao * e0 * q0;
+                    r8[6] = ao * e0 * x0;
+                    r8[7] = ao * e0;
+                    b8[0] = kfn * an * e1 * (1.0 + q1 * (x1 + 2.0));
+                    b8[1] = kfn * an * e1 * q1;
+                    b8[2] = kfn * an * e1 * (x1 + 2.0);
+                    b8[3] = kfn * an * e1;
+                    b8[4] = kfo * ao * e0 * (1.0 + q0 * (x0 + 2.0));
+                    b8[5] = kfo * ao * e0 * q0;
+                    b8[6] = kfo * ao * e0 * (x0 + 2.0);
+                    b8[7] = kfo * ao * e0;
+                } else {
+                    r8[0] = e1;
+                    r8[1] = e1 * x1;
+                    r8[2] = e1 * q1;
+                    r8[3] = e1 * q1 * x1;
+                    r8[4] = e0;
+                    r8[5] = e0 * x0;
+                    r8[6] = e0 * q0;
+                    r8[7] = e0 * q0 * x0;
+                }
+            }
+            if (isx) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    Lr[q * sH + idx] = (T)r8[q];
+                    Lr[8 * sH + q * sH + idx] = (T)b8[q];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) Rc[q * sR + idx] = (T)r8[q];
+            }
+        }
+    }
+    // the first tile position's face fluxes in flight during the R
+    // contraction; with one position per wavefront, from the start
+    if constexpr (MAXP > 1) load_faces(0, fv);
+    // ---- R(r, jv) = (ax sum_s D(r-1, s) Hy(s, jv) + ay sum_s D(r, s) Hy(s+1, jv)) OMy(jv)
+    {
+        const A axA = (A)P[CP_AX], ayA = (A)P[CP_AY];
+        for (int job = wv; job < 2 * nt; job += NW) {
+            const int f = job / nt, jt = job % nt;
+            const int jv = 16 * jt + lr;
+            V acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < MF_NB / 4; ++ks) {
+                const int s = 4 * ks + lk;
+                const T a1 = lr >= 1 ? (T)(axA * (A)D[(f * MF_NB + lr - 1) * sD + s]) : (T)0;
+                acc = M::mma(a1, Hy[s * sH + jv], acc);
+                const T a2 = (T)(ayA * (A)D[(f * MF_NB + lr) * sD + s]);
+                const T b2 = s + 1 < MF_NB ? Hy[(s + 1) * sH + jv] : (T)0;
+                acc = M::mma(a2, b2, acc);
+            }
+            const A om = jv < nv ? (A)OMy[jv] : (A)0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                R[(f * MF_NB + M::row(lane, v)) * sR + jv] = (T)((A)acc[v] * om);
+        }
+    }
+    __syncthreads();
+
+    // ---- F^T tiles: force and collision terms in the accumulators, then
+    // h^{n+1} = c1 h^n + cd div + acc (FP64)
+    const double c1 = iden * (1.0 - 0.5 * dt * P[CP_ITAUO]);
+    const double cd = iden / (c.dx * c.dy);
+#pragma unroll
+    for (int pi = 0; pi < MAXP; ++pi) {
+        const int p = wv + NW * pi;
+        if (p >= npos) continue;
+        if (pi > 0) load_faces(pi, fv);
+        const int it = p % nt, jt = p / nt;
+        const int iu = 16 * it + lc, jvb = 16 * jt;
+        V acc[2];
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            V a = {0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < MF_NB / 4; ++ks) {
+                const int r = 4 * ks + lk;
+                a = M::mma(R[(f * MF_NB + r) * sR + jvb + lr], Ho[r * sH + iu], a);
+            }
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const int t = 4 * ks + lk;
+                a = M::mma(Rc[t * sR + jvb + lr], Lr[(8 * f + t) * sH + iu], a);
+            }
+            acc[f] = a;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t o = node_off(pi, v);
+            const A dh = fv[0][v] - fv[1][v] + fv[2][v] - fv[3][v];
+            const A db = fv[4][v] - fv[5][v] + fv[6][v] - fv[7][v];
+            const double hn = fma(c1, (double)mx_opaque(rh[pi][v]),
+                                  fma(cd, (double)dh, (double)acc[0][v]));
+            const double bn = fma(c1, (double)mx_opaque(rb[pi][v]),
+                                  fma(cd, (double)db, (double)acc[1][v]));
+            // streamed out (non-temporal): the next step reads them long after
+            buf_st_nt<T>((T)hn, rsh, o);
+            buf_st_nt<T>((T)bn, rsb, o);
+        }
+    }
+}
+
+#ifndef HUGKS_BODIES_ONLY
+// FP32 storage: at least four wavefronts per SIMD (the LDS of a cell, 31 KB
+// at nv = 40, allows five three-wavefront cells per CU).
+template <typename T, int NW, int MAXP, int NVC>
+__global__ __launch_bounds__(NW * WAVE)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_update2d_mx(
+    Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
+    T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
+    const T* __restrict__ hfh, const T* __restrict__ hfb,
+    const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ u1,
+    const double* __restrict__ v1) {
+    k_update2d_mx_body<T, NW, MAXP, NVC>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode, u1, v1,
+                                         (int)blockIdx.x, (int)gridDim.x);
+}
+#endif
+
+}  // namespace hugks

@@ -19,32 +19,29 @@ void group_blocks(const Launch2D& L, int* slope, int* fsum, int* fface, int* fno
     *cpre = (ncell + 255) / 256;
 }
 
-void group_update_key(const Launch2D& L, int* ept, int* nt, int* nvc) {
-    const UpdKey k = update_key(L.c, L.fp64);
-    *ept = k.ept;
-    *nt = k.nt;
-    *nvc = k.nvc;
+void group_update_key(const Launch2D& L, int* nw, int* maxp, int* nvc) {
+    *nw = L.update_nw;
+    if (!mx_key(L.c.nv, L.update_nw, maxp, nvc)) *maxp = *nvc = -1;
+}
+
+size_t group_update_lds(const Launch2D& L) {
+    return mx_lds_bytes(L.c.nv, L.fp64 ? sizeof(double) : sizeof(float));
 }
 
 template <typename T>
 static hipError_t group_update_t(const GroupUpd& u, const Launch2D* Ls, hipStream_t s,
                                  bool prepare) {
-#define HUGKS_GUPD_CASE(E, N, V)                                                            \
-    if (u.ept == E && u.nt == N && u.nvc == V) {                                            \
+#define HUGKS_GUPD_CASE(W, P, V)                                                            \
+    if (u.nw == W && u.maxp == P && u.nvc == V) {                                           \
         if (prepare)                                                                        \
-            return hipFuncSetAttribute((const void*)kg_update<T, E, N, V>,                  \
+            return hipFuncSetAttribute((const void*)kg_update<T, W, P, V>,                  \
                                        hipFuncAttributeMaxDynamicSharedMemorySize,          \
                                        (int)u.lds);                                         \
-        hipLaunchKernelGGL((kg_update<T, E, N, V>), dim3(u.blocks), dim3(N), u.lds, s,      \
-                           u.tab, Ls);                                                      \
+        hipLaunchKernelGGL((kg_update<T, W, P, V>), dim3(u.blocks), dim3(W * WAVE), u.lds, \
+                           s, u.tab, Ls);                                                   \
         return hipGetLastError();                                                           \
     }
-    if (sizeof(T) == 4) {
-        HUGKS_UPDM_F32(HUGKS_GUPD_CASE)
-    } else {
-        HUGKS_UPDM_F64(HUGKS_GUPD_CASE)
-    }
-    HUGKS_UPDM_ANY(HUGKS_GUPD_CASE)
+    HUGKS_UPDX(HUGKS_GUPD_CASE)
 #undef HUGKS_GUPD_CASE
     return hipErrorInvalidValue;
 }

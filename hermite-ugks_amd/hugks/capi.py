@@ -21,7 +21,7 @@ TAU_MU_OVER_RHO, TAU_VHS = 0, 1
 DT_2D_GENERAL, DT_2D_WALL_NORMAL, DT_1D = 0, 1, 2
 FORCE_HERMITE_SYMMETRIC, FORCE_HERMITE_SYMMETRIC_B, FORCE_HERMITE_LEGACY = 0, 1, 2
 STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL, STOP_TIME, STOP_STEADY, STOP_ERROR = 0, 1, 2, 3, 4, 5
-FLUX_FUSED, FLUX_SPLIT, FLUX_STENCIL, UPDATE_LDS, UPDATE_MFMA = 0, 1, 2, 0, 1
+FLUX_FUSED, FLUX_SPLIT, FLUX_STENCIL, UPDATE_LDS, UPDATE_MFMA, UPDATE_MFMA_NODE = 0, 1, 2, 0, 1, 2
 BATCH_AUTOTUNE = 1
 
 _dp = C.POINTER(C.c_double)
@@ -66,7 +66,8 @@ EXPORTS = [
     "hugks_get_macro", "hugks_get_macro_noalloc", "hugks_get_f", "hugks_set_f", "hugks_get_time",
     "hugks_checkpoint_write", "hugks_checkpoint_read", "hugks_kernel_times",
     "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
-    "hugks_set_schedule", "hugks_get_schedule", "hugks_tune", "hugks_batch_create",
+    "hugks_set_schedule", "hugks_get_schedule", "hugks_set_update_waves",
+    "hugks_get_update_waves", "hugks_tune", "hugks_batch_create",
     "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_grouped",
     "hugks_batch_handle",
     "hugks_batch_advance", "hugks_batch_set_timing", "hugks_batch_kernel_times",
@@ -112,6 +113,8 @@ def load(path=LIBPATH):
     I = C.POINTER(C.c_int32)
     lib.hugks_set_schedule.argtypes = [H, C.c_int32, C.c_int32]
     lib.hugks_get_schedule.argtypes = [H, I, I]
+    lib.hugks_set_update_waves.argtypes = [H, C.c_int32]
+    lib.hugks_get_update_waves.argtypes = [H, I]
     lib.hugks_tune.argtypes = [C.POINTER(Desc), I, I, _dp]
     lib.hugks_batch_create.argtypes = [C.c_int32, C.POINTER(Desc), C.c_int32, C.POINTER(H)]
     lib.hugks_batch_destroy.argtypes = [H]
@@ -311,6 +314,14 @@ class Solver:
         f, u = C.c_int32(), C.c_int32()
         self._check(self.lib.hugks_get_schedule(self.h, C.byref(f), C.byref(u)))
         return f.value, u.value
+
+    def set_update_waves(self, waves):
+        self._check(self.lib.hugks_set_update_waves(self.h, int(waves)))
+
+    def update_waves(self):
+        w = C.c_int32()
+        self._check(self.lib.hugks_get_update_waves(self.h, C.byref(w)))
+        return w.value
 
 
 def tune(case):

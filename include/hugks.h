@@ -236,15 +236,26 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
  *                               slopes from each upwind cell's stencil
  *                               (needs each distribution array under 4 GiB)
  *   update: HUGKS_UPDATE_LDS  Hermite contractions as LDS dot products
- *           HUGKS_UPDATE_MFMA Hermite contractions as matrix tiles (default
- *                             when nv <= 48 and Nm <= 14) */
+ *           HUGKS_UPDATE_MFMA Hermite contractions as matrix tiles
+ *                             (nv <= 48, Nm <= 14)
+ *           HUGKS_UPDATE_MFMA_NODE  the force and the collision terms as
+ *                             matrix tiles, each node updated from its
+ *                             tile accumulator (default when nv <= 48,
+ *                             Nm <= 14 and its LDS fits the device) */
 #define HUGKS_FLUX_FUSED   0
 #define HUGKS_FLUX_SPLIT   1
 #define HUGKS_FLUX_STENCIL 2
 #define HUGKS_UPDATE_LDS   0
 #define HUGKS_UPDATE_MFMA  1
+#define HUGKS_UPDATE_MFMA_NODE 2
 int hugks_set_schedule(hugks_handle *h, int32_t flux_kernel, int32_t update_kernel);
 int hugks_get_schedule(hugks_handle *h, int32_t *flux_kernel, int32_t *update_kernel);
+/* Tile parameter of HUGKS_UPDATE_MFMA_NODE: wavefronts per cell (1 to 4,
+ * where a kernel instance exists for the velocity grid; 0 = the default:
+ * 3 for nv 33..48, 4 for 17..32, 1 up to 16).  HUGKS_E_UNSUPPORTED when no
+ * instance exists. */
+int hugks_set_update_waves(hugks_handle *h, int32_t waves);
+int hugks_get_update_waves(hugks_handle *h, int32_t *waves);
 
 /* Autotuner: times every schedule on a scratch handle of the descriptor's
  * shape and returns the fastest (cached per shape for the process);
@@ -271,8 +282,8 @@ int32_t hugks_batch_size(const hugks_batch *b);
 /* Number of cases stepped by grouped launches: one launch per kernel type
  * and storage type covers all small 2D cases with factoring weights and an
  * MFMA-capable grid (at least two of them; they then run the split flux
- * and the MFMA update); the other cases' kernels are launched on their own
- * beside them.  0 when every case runs on its own. */
+ * and the MFMA node update); the other cases' kernels are launched on their
+ * own beside them.  0 when every case runs on its own. */
 int32_t hugks_batch_grouped(const hugks_batch *b);
 int hugks_batch_handle(hugks_batch *b, int32_t i, hugks_handle **h);
 /* max_steps replays (<= 0: until every case stopped); infos[n] receives

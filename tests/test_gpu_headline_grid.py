@@ -6,7 +6,7 @@ kernels compiled for nv = 40 (FP32 storage: 7 nodes per thread, 256
 threads; FP64: 5 nodes per thread, 320 threads), the MFMA Hermite tiles
 with r + s up to 12 in the 16-row tiles, and the interior-face pipeline at
 1600 nodes per face.  Each flux schedule (split, stencil, fused) is run
-with each update kernel (MFMA, LDS):
+with each update kernel (MFMA Hermite tiles, MFMA node update, LDS):
 
 - after 6 steps every conserved field agrees with the oracle to 1e-10
   (FP64) and 1e-5 (mixed) of the field's scale, the distributions to
@@ -34,7 +34,7 @@ TOL_FP64 = 1e-10
 TOL_MIXED = 1e-5
 STEPS = 6
 FLUX = {"split": capi.FLUX_SPLIT, "stencil": capi.FLUX_STENCIL, "fused": capi.FLUX_FUSED}
-UPDATE = {"mfma": capi.UPDATE_MFMA, "lds": capi.UPDATE_LDS}
+UPDATE = {"mfma": capi.UPDATE_MFMA, "lds": capi.UPDATE_LDS, "mfma-node": capi.UPDATE_MFMA_NODE}
 
 
 def rt_params():

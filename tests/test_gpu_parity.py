@@ -117,7 +117,7 @@ def test_flux_kernels_agree(have_gpu, monkeypatch, mode):
         assert np.all(err <= tol), err
 
 
-@pytest.mark.parametrize("mode", ["lds", "mfma"])
+@pytest.mark.parametrize("mode", ["lds", "mfma", "mfma-node"])
 def test_update_kernels_agree(have_gpu, monkeypatch, mode):
     """Both cell-update implementations (LDS dot products; MFMA Hermite
     tiles) against the oracle, FP64 and mixed, at a velocity grid that
