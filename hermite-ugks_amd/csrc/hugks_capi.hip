@@ -120,6 +120,7 @@ struct hugks_handle {
     hugks_desc d{};
     int dim = 2;
     bool fp64 = true;
+    bool b3 = false;    // bf16x3 Hermite contractions (FP32 storage otherwise)
     Launch2D L{};
     hipStream_t stream = nullptr;
     int device = 0;
@@ -287,9 +288,9 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     if (d->nm < 0) return fail(nullptr, HUGKS_E_ARG, "Hermite order Nm must be non-negative");
     if (d->precision < HUGKS_FP64 || d->precision > HUGKS_MIXED_BF16X3)
         return fail(nullptr, HUGKS_E_ARG, "unknown precision");
-    if (d->precision == HUGKS_MIXED_BF16X3)
+    if (d->precision == HUGKS_MIXED_BF16X3 && d->dim != 2)
         return fail(nullptr, HUGKS_E_UNSUPPORTED,
-                    "HUGKS_MIXED_BF16X3 has no kernel in this build");
+                    "HUGKS_MIXED_BF16X3 applies to the 2D Hermite contraction only");
     if (!d->u || !d->weight || !d->prim0 || (dim == 2 && (!d->v || !d->accel_cell ||
                                                           !d->accel_vface || !d->accel_hface)))
         return fail(nullptr, HUGKS_E_ARG, "missing node, weight, prim0 or acceleration array");
@@ -348,6 +349,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     h->d.accel_cell = h->d.accel_vface = h->d.accel_hface = nullptr;
     h->dim = dim;
     h->fp64 = d->precision == HUGKS_FP64;
+    h->b3 = d->precision == HUGKS_MIXED_BF16X3;
     h->ncell = ncell;
     (void)hipGetDevice(&h->device);
     Consts& c = h->L.c;
@@ -364,6 +366,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     c.steady_tol = d->steady_tol; c.mass_tol = d->mass_tol;
     Launch2D& L = h->L;
     L.fp64 = h->fp64;
+    L.b3 = h->b3;
     const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { release(h); delete h; return hip_fail(nullptr, e, "stream"); }
@@ -553,15 +556,18 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         L.update_nw = update2d_mx_default_waves(c);
         // default: the MFMA node update, else the MFMA Hermite tiles, else
         // the LDS kernel
-        L.update_mode = update2d_mx_supported(c, h->fp64, L.lds_limit, L.update_nw) ? 2
-                        : update2d_mfma_supported(c)                               ? 1
-                                                                                   : 0;
-        if (const char* um = getenv("HUGKS_UPDATE_KERNEL")) {
+        const bool mx = update2d_mx_supported(c, h->fp64, h->b3, L.lds_limit, L.update_nw);
+        if (h->b3 && !mx) {
+            release(h);
+            delete h;
+            return fail(nullptr, HUGKS_E_UNSUPPORTED,
+                        "HUGKS_MIXED_BF16X3 needs the MFMA node update (nv <= 48, Nm <= 14)");
+        }
+        L.update_mode = mx ? 2 : update2d_mfma_supported(c) ? 1 : 0;
+        if (const char* um = getenv("HUGKS_UPDATE_KERNEL"); um && !h->b3) {
             if (strcmp(um, "lds") == 0) L.update_mode = 0;
             if (strcmp(um, "mfma") == 0 && update2d_mfma_supported(c)) L.update_mode = 1;
-            if (strcmp(um, "mfma-node") == 0 &&
-                update2d_mx_supported(c, h->fp64, L.lds_limit, L.update_nw))
-                L.update_mode = 2;
+            if (strcmp(um, "mfma-node") == 0 && mx) L.update_mode = 2;
         }
         if (L.update_lds > (size_t)prop.sharedMemPerBlockOptin && prop.sharedMemPerBlockOptin > 0) {
             release(h);
@@ -977,8 +983,11 @@ int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kern
     if (h->dim != 2) return HUGKS_OK;   // 1D has one kernel of each kind
     if (update_kernel == HUGKS_UPDATE_MFMA && !update2d_mfma_supported(h->L.c))
         return fail(h, HUGKS_E_UNSUPPORTED, "MFMA update needs nv <= 48 and Nm <= 14");
+    if (h->b3 && update_kernel != HUGKS_UPDATE_MFMA_NODE)
+        return fail(h, HUGKS_E_UNSUPPORTED,
+                    "HUGKS_MIXED_BF16X3 runs the MFMA node update only");
     if (update_kernel == HUGKS_UPDATE_MFMA_NODE &&
-        !update2d_mx_supported(h->L.c, h->fp64, h->L.lds_limit, h->L.update_nw))
+        !update2d_mx_supported(h->L.c, h->fp64, h->b3, h->L.lds_limit, h->L.update_nw))
         return fail(h, HUGKS_E_UNSUPPORTED,
                     "MFMA node update needs nv <= 48, Nm <= 14 and its LDS on the device");
     if (flux_kernel == HUGKS_FLUX_STENCIL && !h->L.stencil_ok)
@@ -992,7 +1001,7 @@ int hugks_set_update_waves(hugks_handle* h, int32_t waves) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
     if (h->dim != 2) return HUGKS_OK;
     const int nw = waves > 0 ? waves : update2d_mx_default_waves(h->L.c);
-    if (nw > 16 || !update2d_mx_supported(h->L.c, h->fp64, h->L.lds_limit, nw))
+    if (nw > 16 || !update2d_mx_supported(h->L.c, h->fp64, h->b3, h->L.lds_limit, nw))
         return fail(h, HUGKS_E_UNSUPPORTED,
                     "no MFMA node-update kernel with " + std::to_string(nw) +
                         " wavefronts per cell for this velocity grid");
@@ -1150,11 +1159,12 @@ extern "C" {
 // The small 2D cases whose step can be the split flux with factoring
 // weights and the MFMA update run in grouped launches (one launch per
 // kernel type and storage type for all of them) when there are at least two;
-// the other cases keep their own launches on the lane streams.
+// the other cases (and those of the bf16x3 contraction) keep their own
+// launches on the lane streams.
 static bool case_groupable(const hugks_handle* h) {
     const Launch2D& L = h->L;
-    return h->dim == 2 && L.wx &&
-           update2d_mx_supported(L.c, h->fp64, L.lds_limit, L.update_nw) &&
+    return h->dim == 2 && L.wx && !h->b3 &&
+           update2d_mx_supported(L.c, h->fp64, false, L.lds_limit, L.update_nw) &&
            L.c.nx * L.c.ny < PRED_MIN_CELLS;
 }
 static std::vector<char> batch_groupable(const hugks_batch* b) {

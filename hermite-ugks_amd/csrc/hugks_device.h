@@ -54,10 +54,10 @@ struct DevState {
 // a run-time index into the arrays of c: a Consts copied from memory (the
 // grouped kernels read each case's from a table) would otherwise be kept
 // in scratch memory to be indexed.
-__device__ __forceinline__ int bc_of(const Consts& c, int s) {
+__host__ __device__ __forceinline__ int bc_of(const Consts& c, int s) {
     return s == 0 ? c.bc[0] : s == 1 ? c.bc[1] : s == 2 ? c.bc[2] : c.bc[3];
 }
-__device__ __forceinline__ double tw_of(const Consts& c, int s) {
+__host__ __device__ __forceinline__ double tw_of(const Consts& c, int s) {
     return s == 0 ? c.tw[0] : s == 1 ? c.tw[1] : s == 2 ? c.tw[2] : c.tw[3];
 }
 

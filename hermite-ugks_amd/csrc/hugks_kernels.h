@@ -15,6 +15,7 @@ typedef void (*LaunchHook)(void* ctx, int phase, const char* name);
 struct Launch2D {
     Consts c;
     bool fp64;
+    bool b3;             // bf16x3 split-product Hermite contractions (FP32 storage)
     DevState* st;
     double* W;
     double* part;        // per-cell residual partials [5*ncell]
@@ -133,8 +134,9 @@ hipError_t launch_step1d(const Launch2D& L, hipStream_t s);
 size_t update2d_lds_bytes(const Consts& c, bool fp64);
 bool update2d_mfma_supported(const Consts& c);
 // update mode 2 with nw wavefronts per cell: a compiled instance, nv <= 48,
-// Nm <= 14 and its LDS within lds_limit bytes
-bool update2d_mx_supported(const Consts& c, bool fp64, size_t lds_limit, int nw);
+// Nm <= 14 and its LDS within lds_limit bytes (b3: the bf16x3 contraction,
+// FP32 storage only)
+bool update2d_mx_supported(const Consts& c, bool fp64, bool b3, size_t lds_limit, int nw);
 int update2d_mx_default_waves(const Consts& c);
 size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64);
 

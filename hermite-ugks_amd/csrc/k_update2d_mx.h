@@ -84,9 +84,12 @@ __host__ __device__ inline size_t mx_lds_bytes(int nv, size_t tsize) {
 }
 
 // Compiled instances (wavefronts per cell, tile positions per wavefront,
-// compiled-in nv or 0 for any nv <= 48).
+// compiled-in nv or 0 for any nv <= 48); the bf16x3 contraction has the
+// run-time-nv ones and those of the 28- and 40-node grids.
 #define HUGKS_UPDX(X) X(1, 1, 16) X(4, 1, 20) X(2, 2, 20) X(4, 1, 24) X(2, 2, 24) \
     X(4, 1, 28) X(2, 2, 28) X(4, 1, 32) X(2, 2, 32) X(3, 3, 40) X(5, 2, 40) X(9, 1, 40) \
+    X(1, 1, 0) X(4, 1, 0) X(2, 2, 0) X(3, 3, 0) X(5, 2, 0)
+#define HUGKS_UPDX_B3(X) X(4, 1, 28) X(2, 2, 28) X(3, 3, 40) X(5, 2, 40) \
     X(1, 1, 0) X(4, 1, 0) X(2, 2, 0) X(3, 3, 0) X(5, 2, 0)
 
 // The wavefronts per cell a case runs with by default: 3 for nv in
@@ -102,19 +105,74 @@ __host__ __device__ inline int mx_maxp(int nv, int nw) {
 }
 // The instance (nw, maxp, nvc) for nv and nw; nvc = 0 when nv has no
 // compiled-in instance.  Returns false when no instance runs nw at nv.
-__host__ inline bool mx_key(int nv, int nw, int* maxp, int* nvc) {
+__host__ inline bool mx_key(int nv, int nw, int* maxp, int* nvc, bool b3 = false) {
     const int mp = mx_maxp(nv, nw);
     bool in = false, any = false;
 #define HUGKS_UPDX_HAS(W, P, V)                  \
     if (W == nw && P == mp && V == nv) in = true; \
     if (W == nw && P == mp && V == 0) any = true;
-    HUGKS_UPDX(HUGKS_UPDX_HAS)
+    if (b3) {
+        HUGKS_UPDX_B3(HUGKS_UPDX_HAS)
+    } else {
+        HUGKS_UPDX(HUGKS_UPDX_HAS)
+    }
 #undef HUGKS_UPDX_HAS
     if (!in && !any) return false;
     *maxp = mp;
     *nvc = in ? nv : 0;
     return true;
 }
+
+// bf16x3 split-product operands.  An FP32 value is split exactly into three
+// bf16 pieces, a = hi + mid + lo (hi = bf16(a), mid = bf16(a - hi), lo =
+// a - hi - mid, which has at most 8 significant bits; all rounded to
+// nearest), kept as two dwords [hi | mid << 16, lo].  One
+// v_mfma_f32_16x16x32_bf16 then does one 16x16x4 step: the eight K slots
+// of a lane hold the six cross products of its K index, A slots
+// [h m | h m | l 0 | h m] against B slots [h h | m m | h m | l 0], i.e.
+// hi hi, mid hi, hi mid, mid mid, lo hi, hi lo; the three products dropped
+// (mid lo, lo mid, lo lo) are below 2^-23 |a b|, and the products are exact
+// in the FP32 accumulation.  The A operand is the stored pair as it is, the
+// B operand two byte permutes of it.
+struct B3 {
+    uint32_t x, y;
+};
+__device__ __forceinline__ B3 b3_split(float a) {
+    const __bf16 hi = (__bf16)a;
+    const float r1 = a - (float)hi;
+    const __bf16 mid = (__bf16)r1;
+    const __bf16 lo = (__bf16)(r1 - (float)mid);
+    return B3{(uint32_t)__builtin_bit_cast(uint16_t, hi) |
+                  ((uint32_t)__builtin_bit_cast(uint16_t, mid) << 16),
+              (uint32_t)__builtin_bit_cast(uint16_t, lo)};
+}
+struct MfmaB3 {
+    using V = mf_f4;
+    static __device__ __forceinline__ int row(int l, int v) { return Mfma<float>::row(l, v); }
+    static __device__ __forceinline__ V mma(B3 a, B3 b, V c) {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+        const u4 av = {a.x, a.x, a.y, a.x};
+        const u4 bv = {__builtin_amdgcn_perm(b.x, b.x, 0x05040100u),
+                       __builtin_amdgcn_perm(b.x, b.x, 0x07060302u), b.x, b.y};
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, av),
+                                                       __builtin_bit_cast(bf8, bv), c, 0, 0, 0);
+    }
+};
+// The LDS element and the tile product of a storage type: T itself on
+// 16x16x4 tiles, or (FP32 storage, SB3) the bf16x3 pieces.
+template <typename T, bool SB3> struct MxOps {
+    using E = T;
+    using M = Mfma<T>;
+    static __device__ __forceinline__ E put(typename Arith<T>::A x) { return (T)x; }
+    static __device__ __forceinline__ E zero() { return (T)0; }
+};
+template <> struct MxOps<float, true> {
+    using E = B3;
+    using M = MfmaB3;
+    static __device__ __forceinline__ E put(float x) { return b3_split(x); }
+    static __device__ __forceinline__ E zero() { return B3{0u, 0u}; }
+};
 
 // The value, opaque to the optimizer: keeps a conversion of it where it is
 // written instead of next to the load that produced the value (where its
@@ -125,7 +183,7 @@ __device__ __forceinline__ X mx_opaque(X x) {
     return x;
 }
 
-template <typename T, int NW, int MAXP, int NVC>
+template <typename T, int NW, int MAXP, int NVC, bool SB3 = false>
 __device__ __forceinline__ void k_update2d_mx_body(
     const Consts& c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
@@ -133,7 +191,9 @@ __device__ __forceinline__ void k_update2d_mx_body(
     const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ u1,
     const double* __restrict__ v1, int bid, int nblk) {
     using A = typename Arith<T>::A;
-    using M = Mfma<T>;
+    using O = MxOps<T, SB3>;
+    using E = typename O::E;
+    using M = typename O::M;
     using V = typename M::V;
     constexpr int NT = NW * WAVE;
     HIP_DYNAMIC_SHARED(double, smem_raw)
@@ -160,17 +220,17 @@ __device__ __forceinline__ void k_update2d_mx_body(
 
     const MxLayout ly = mx_layout(nv);
     const int sH = ly.sH, sR = ly.sR, sD = ly.sD, ldG = ly.ldG, GS = nv * ly.ldG;
-    T* const sm = reinterpret_cast<T*>(smem_raw);
-    T* const G = sm;
-    T* const D = sm + ly.oD;       // G is dead once Tm is formed
-    T* const Lr = sm + ly.oL;
-    T* const Rc = sm + ly.oRc;
-    T* const Hx = sm + ly.oHx;
-    T* const Ho = sm + ly.oHo;
-    T* const Hy = sm + ly.oHy;
-    T* const Tm = sm + ly.oT;
-    T* const R = Tm;               // Tm is dead once D is formed
-    T* const OMy = sm + ly.oOM;
+    E* const sm = reinterpret_cast<E*>(smem_raw);
+    E* const G = sm;
+    E* const D = sm + ly.oD;       // G is dead once Tm is formed
+    E* const Lr = sm + ly.oL;
+    E* const Rc = sm + ly.oRc;
+    E* const Hx = sm + ly.oHx;
+    E* const Ho = sm + ly.oHo;
+    E* const Hy = sm + ly.oHy;
+    E* const Tm = sm + ly.oT;
+    E* const R = Tm;               // Tm is dead once D is formed
+    A* const OMy = reinterpret_cast<A*>(sm + ly.oOM);
 
     // ---- Hermite bases He_0..He_15 at xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt)
     // (build_probabilists_hermite, hermite_symmetric2d.f90:220-233); Ho
@@ -186,28 +246,28 @@ __device__ __forceinline__ void k_update2d_mx_body(
                 const A x = (A)((xv - (isx ? po[1] : po[2])) / sqrt_rt);
                 const A om = fexp((A)-0.5 * x * x);
                 const A ho = isx ? oscale * om : (A)0;
-                T* Hb = isx ? Hx : Hy;
+                E* Hb = isx ? Hx : Hy;
                 A p0 = (A)1, p1 = x;
-                Hb[idx] = (T)p0;
-                Hb[sH + idx] = (T)p1;
+                Hb[idx] = O::put(p0);
+                Hb[sH + idx] = O::put(p1);
                 if (isx) {
-                    Ho[idx] = (T)(ho * p0);
-                    Ho[sH + idx] = (T)(ho * p1);
+                    Ho[idx] = O::put(ho * p0);
+                    Ho[sH + idx] = O::put(ho * p1);
                 } else {
-                    OMy[idx] = (T)om;
+                    OMy[idx] = om;
                 }
                 for (int n = 2; n < MF_NB; ++n) {
                     const A p2 = x * p1 - (A)(n - 1) * p0;
-                    Hb[n * sH + idx] = (T)p2;
-                    if (isx) Ho[n * sH + idx] = (T)(ho * p2);
+                    Hb[n * sH + idx] = O::put(p2);
+                    if (isx) Ho[n * sH + idx] = O::put(ho * p2);
                     p0 = p1;
                     p1 = p2;
                 }
             } else {
-                T* Hb = isx ? Hx : Hy;
+                E* Hb = isx ? Hx : Hy;
                 for (int n = 0; n < MF_NB; ++n) {
-                    Hb[n * sH + idx] = (T)0;
-                    if (isx) Ho[n * sH + idx] = (T)0;
+                    Hb[n * sH + idx] = O::zero();
+                    if (isx) Ho[n * sH + idx] = O::zero();
                 }
             }
         }
@@ -256,8 +316,8 @@ __device__ __forceinline__ void k_update2d_mx_body(
             const int jv = 16 * (p / nt) + M::row(lane, v);
             if (o != OUT) {
                 const A wa = buf_ld<A>(rsw, o / (uint32_t)sizeof(T) * (uint32_t)sizeof(A));
-                G[jv * ldG + iu] = (T)(wa * rh[pi][v]);
-                G[GS + jv * ldG + iu] = (T)(wa * rb[pi][v]);
+                G[jv * ldG + iu] = O::put(wa * rh[pi][v]);
+                G[GS + jv * ldG + iu] = O::put(wa * rb[pi][v]);
             }
         }
     }
@@ -292,12 +352,12 @@ __device__ __forceinline__ void k_update2d_mx_body(
         for (int ks = 0; ks < ksn; ++ks) {
             const int iu = 4 * ks + lk;
             const bool in = iu < nv;
-            const T a = in ? Hx[lr * sH + iu] : (T)0;
-            const T bb = (in && jv < nv) ? G[f * GS + jv * ldG + iu] : (T)0;
+            const E a = in ? Hx[lr * sH + iu] : O::zero();
+            const E bb = (in && jv < nv) ? G[f * GS + jv * ldG + iu] : O::zero();
             acc = M::mma(a, bb, acc);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Tm[(f * MF_NB + M::row(lane, v)) * sH + jv] = acc[v];
+        for (int v = 0; v < 4; ++v) Tm[(f * MF_NB + M::row(lane, v)) * sH + jv] = O::put(acc[v]);
     }
     __syncthreads();
 
@@ -309,15 +369,15 @@ __device__ __forceinline__ void k_update2d_mx_body(
 #pragma unroll
         for (int ks = 0; ks < ksn; ++ks) {
             const int jv = 4 * ks + lk;
-            const T a = Tm[(f * MF_NB + lr) * sH + jv];     // zero beyond nv
-            const T bb = Hy[lr * sH + jv];
+            const E a = Tm[(f * MF_NB + lr) * sH + jv];     // zero beyond nv
+            const E bb = Hy[lr * sH + jv];
             acc = M::mma(a, bb, acc);
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = M::row(lane, v), s = lr;
-            T val = (T)0;
-            if (r + s <= nm) val = (T)((A)(jac * kInvFact[r] * kInvFact[s]) * (A)acc[v]);
+            E val = O::zero();
+            if (r + s <= nm) val = O::put((A)(jac * kInvFact[r] * kInvFact[s]) * (A)acc[v]);
             D[(f * MF_NB + r) * sD + s] = val;
             const int rs = 16 * r + s;
             int slot = -1;
@@ -383,12 +443,12 @@ __device__ __forceinline__ void k_update2d_mx_body(
             if (isx) {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    Lr[q * sH + idx] = (T)r8[q];
-                    Lr[8 * sH + q * sH + idx] = (T)b8[q];
+                    Lr[q * sH + idx] = O::put((A)r8[q]);
+                    Lr[8 * sH + q * sH + idx] = O::put((A)b8[q]);
                 }
             } else {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) Rc[q * sR + idx] = (T)r8[q];
+                for (int q = 0; q < 8; ++q) Rc[q * sR + idx] = O::put((A)r8[q]);
             }
         }
     }
@@ -401,20 +461,37 @@ __device__ __forceinline__ void k_update2d_mx_body(
         for (int job = wv; job < 2 * nt; job += NW) {
             const int f = job / nt, jt = job % nt;
             const int jv = 16 * jt + lr;
-            V acc = {0, 0, 0, 0};
+            const A om = jv < nv ? OMy[jv] : (A)0;
+            if constexpr (SB3) {
+                // the operands as stored; ax and ay applied to the two sums
+                V ac1 = {0, 0, 0, 0}, ac2 = {0, 0, 0, 0};
 #pragma unroll
-            for (int ks = 0; ks < MF_NB / 4; ++ks) {
-                const int s = 4 * ks + lk;
-                const T a1 = lr >= 1 ? (T)(axA * (A)D[(f * MF_NB + lr - 1) * sD + s]) : (T)0;
-                acc = M::mma(a1, Hy[s * sH + jv], acc);
-                const T a2 = (T)(ayA * (A)D[(f * MF_NB + lr) * sD + s]);
-                const T b2 = s + 1 < MF_NB ? Hy[(s + 1) * sH + jv] : (T)0;
-                acc = M::mma(a2, b2, acc);
+                for (int ks = 0; ks < MF_NB / 4; ++ks) {
+                    const int s = 4 * ks + lk;
+                    const E a1 = lr >= 1 ? D[(f * MF_NB + lr - 1) * sD + s] : O::zero();
+                    ac1 = M::mma(a1, Hy[s * sH + jv], ac1);
+                    const E b2 = s + 1 < MF_NB ? Hy[(s + 1) * sH + jv] : O::zero();
+                    ac2 = M::mma(D[(f * MF_NB + lr) * sD + s], b2, ac2);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    R[(f * MF_NB + M::row(lane, v)) * sR + jv] =
+                        O::put((axA * ac1[v] + ayA * ac2[v]) * om);
+            } else {
+                V acc = {0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < MF_NB / 4; ++ks) {
+                    const int s = 4 * ks + lk;
+                    const T a1 = lr >= 1 ? (T)(axA * (A)D[(f * MF_NB + lr - 1) * sD + s]) : (T)0;
+                    acc = M::mma(a1, Hy[s * sH + jv], acc);
+                    const T a2 = (T)(ayA * (A)D[(f * MF_NB + lr) * sD + s]);
+                    const T b2 = s + 1 < MF_NB ? Hy[(s + 1) * sH + jv] : (T)0;
+                    acc = M::mma(a2, b2, acc);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    R[(f * MF_NB + M::row(lane, v)) * sR + jv] = (T)((A)acc[v] * om);
             }
-            const A om = jv < nv ? (A)OMy[jv] : (A)0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                R[(f * MF_NB + M::row(lane, v)) * sR + jv] = (T)((A)acc[v] * om);
         }
     }
     __syncthreads();
@@ -465,7 +542,7 @@ __device__ __forceinline__ void k_update2d_mx_body(
 #ifndef HUGKS_BODIES_ONLY
 // FP32 storage: at least four wavefronts per SIMD (the LDS of a cell, 31 KB
 // at nv = 40, allows five three-wavefront cells per CU).
-template <typename T, int NW, int MAXP, int NVC>
+template <typename T, int NW, int MAXP, int NVC, bool SB3 = false>
 __global__ __launch_bounds__(NW * WAVE)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_update2d_mx(
     Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
@@ -473,8 +550,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_update2d_mx(
     const T* __restrict__ hfh, const T* __restrict__ hfb,
     const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ u1,
     const double* __restrict__ v1) {
-    k_update2d_mx_body<T, NW, MAXP, NVC>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode, u1, v1,
-                                         (int)blockIdx.x, (int)gridDim.x);
+    k_update2d_mx_body<T, NW, MAXP, NVC, SB3>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode, u1,
+                                              v1, (int)blockIdx.x, (int)gridDim.x);
 }
 #endif
 

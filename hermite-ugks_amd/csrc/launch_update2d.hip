@@ -86,46 +86,60 @@ static hipError_t launch_update_mfma_t(const Launch2D& L, hipStream_t s, bool pr
     return hipErrorInvalidValue;
 }
 
-bool update2d_mx_supported(const Consts& c, bool fp64, size_t lds_limit, int nw) {
+// LDS element size: the storage type, or the two dwords of a bf16x3 operand
+static size_t mx_elem(bool fp64, bool b3) {
+    return fp64 ? sizeof(double) : b3 ? sizeof(B3) : sizeof(float);
+}
+
+bool update2d_mx_supported(const Consts& c, bool fp64, bool b3, size_t lds_limit, int nw) {
     int mp, nvc;
-    return update2d_mfma_supported(c) && mx_key(c.nv, nw, &mp, &nvc) &&
-           mx_lds_bytes(c.nv, fp64 ? sizeof(double) : sizeof(float)) <= lds_limit;
+    return update2d_mfma_supported(c) && !(fp64 && b3) && mx_key(c.nv, nw, &mp, &nvc, b3) &&
+           mx_lds_bytes(c.nv, mx_elem(fp64, b3)) <= lds_limit;
 }
 
 int update2d_mx_default_waves(const Consts& c) { return mx_default_nw(c.nv); }
 
-template <typename T>
+template <typename T, bool SB3>
 static hipError_t launch_update_mx_t(const Launch2D& L, hipStream_t s, int nw, bool prepare) {
     const Consts& c = L.c;
     int maxp = 0, nvc = 0;
-    if (!mx_key(c.nv, nw, &maxp, &nvc)) return hipErrorInvalidValue;
+    if (!mx_key(c.nv, nw, &maxp, &nvc, SB3)) return hipErrorInvalidValue;
     const unsigned ncell = (unsigned)(c.nx * c.ny);
-    const size_t lds = mx_lds_bytes(c.nv, sizeof(T));
+    const size_t lds = mx_lds_bytes(c.nv, mx_elem(sizeof(T) == 8, SB3));
     if (!prepare) {
         HookScope hs(L, "cell_pre");
         hipLaunchKernelGGL((k_cell_pre), dim3((ncell + 255) / 256), dim3(256), 0, s, c, L.st, L.W,
                            L.part, L.vflux, L.hflux, L.acc_cell, L.cellpre);
     }
-#define HUGKS_UPDX_CASE(W, P, V)                                                                \
-    if (nw == W && maxp == P && nvc == V) {                                                     \
-        if (prepare)                                                                            \
-            return hipFuncSetAttribute((const void*)k_update2d_mx<T, W, P, V>,                  \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);   \
-        HookScope hs(L, "update2d");                                                            \
-        hipLaunchKernelGGL((k_update2d_mx<T, W, P, V>), dim3(ncell), dim3(W * WAVE), lds, s, c, \
-            L.st, L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,                \
-            (const T*)L.hfh, (const T*)L.hfb, (const typename Arith<T>::A*)L.wnode, L.u1, L.v1);   \
-        return hipGetLastError();                                                               \
+#define HUGKS_UPDX_CASE(W, P, V)                                                              \
+    if (nw == W && maxp == P && nvc == V) {                                                   \
+        if (prepare)                                                                          \
+            return hipFuncSetAttribute((const void*)k_update2d_mx<T, W, P, V, SB3>,           \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        HookScope hs(L, "update2d");                                                          \
+        hipLaunchKernelGGL((k_update2d_mx<T, W, P, V, SB3>), dim3(ncell), dim3(W * WAVE), lds, \
+            s, c, L.st, L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,        \
+            (const T*)L.hfh, (const T*)L.hfb, (const typename Arith<T>::A*)L.wnode, L.u1,     \
+            L.v1);                                                                            \
+        return hipGetLastError();                                                             \
     }
-    HUGKS_UPDX(HUGKS_UPDX_CASE)
+    if constexpr (SB3) {
+        HUGKS_UPDX_B3(HUGKS_UPDX_CASE)
+    } else {
+        HUGKS_UPDX(HUGKS_UPDX_CASE)
+    }
 #undef HUGKS_UPDX_CASE
     return hipErrorInvalidValue;
 }
 
+static hipError_t launch_update_mx(const Launch2D& L, hipStream_t s, int nw, bool prepare) {
+    if (L.fp64) return launch_update_mx_t<double, false>(L, s, nw, prepare);
+    return L.b3 ? launch_update_mx_t<float, true>(L, s, nw, prepare)
+                : launch_update_mx_t<float, false>(L, s, nw, prepare);
+}
+
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
-    if (L.update_mode == 2)
-        return L.fp64 ? launch_update_mx_t<double>(L, s, L.update_nw, false)
-                      : launch_update_mx_t<float>(L, s, L.update_nw, false);
+    if (L.update_mode == 2) return launch_update_mx(L, s, L.update_nw, false);
     if (L.update_mode == 1)
         return L.fp64 ? launch_update_mfma_t<double>(L, s, false)
                       : launch_update_mfma_t<float>(L, s, false);
@@ -141,9 +155,8 @@ hipError_t prepare_update2d(const Launch2D& L) {
         e = L.fp64 ? launch_update_mfma_t<double>(L, nullptr, true)
                    : launch_update_mfma_t<float>(L, nullptr, true);
     for (int nw = 1; nw <= 16 && e == hipSuccess; ++nw) {
-        if (!update2d_mx_supported(L.c, L.fp64, L.lds_limit, nw)) continue;
-        e = L.fp64 ? launch_update_mx_t<double>(L, nullptr, nw, true)
-                   : launch_update_mx_t<float>(L, nullptr, nw, true);
+        if (!update2d_mx_supported(L.c, L.fp64, L.b3, L.lds_limit, nw)) continue;
+        e = launch_update_mx(L, nullptr, nw, true);
     }
     return e;
 }

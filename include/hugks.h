@@ -56,9 +56,13 @@ extern "C" {
 #define HUGKS_FP64           0  /* everything FP64 (parity path)            */
 #define HUGKS_MIXED_FP32     1  /* FP32 f/flux, FP64 moments and W,
                                    FP32 Hermite contractions                 */
-#define HUGKS_MIXED_BF16X3   2  /* as HUGKS_MIXED_FP32 with bf16x3 split-
-                                   product Hermite contractions; refused with
-                                   HUGKS_E_UNSUPPORTED where not built       */
+#define HUGKS_MIXED_BF16X3   2  /* as HUGKS_MIXED_FP32 with the Hermite
+                                   contractions as bf16x3 split products
+                                   (operands split exactly into bf16
+                                   hi+mid+lo, six products per K index on
+                                   bf16 MFMA, FP32 accumulation): 2D only,
+                                   MFMA node update only (nv <= 48,
+                                   Nm <= 14), else HUGKS_E_UNSUPPORTED       */
 
 /* boundary kind per side (x-lo, x-hi, y-lo, y-hi) */
 #define HUGKS_BC_PERIODIC    0  /* 2d/src/boundary.f90:16-53                 */
@@ -241,7 +245,9 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
  *           HUGKS_UPDATE_MFMA_NODE  the force and the collision terms as
  *                             matrix tiles, each node updated from its
  *                             tile accumulator (default when nv <= 48,
- *                             Nm <= 14 and its LDS fits the device) */
+ *                             Nm <= 14 and its LDS fits the device; the
+ *                             only update of HUGKS_MIXED_BF16X3, which
+ *                             refuses the others with HUGKS_E_UNSUPPORTED) */
 #define HUGKS_FLUX_FUSED   0
 #define HUGKS_FLUX_SPLIT   1
 #define HUGKS_FLUX_STENCIL 2
@@ -250,10 +256,11 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
 #define HUGKS_UPDATE_MFMA_NODE 2
 int hugks_set_schedule(hugks_handle *h, int32_t flux_kernel, int32_t update_kernel);
 int hugks_get_schedule(hugks_handle *h, int32_t *flux_kernel, int32_t *update_kernel);
-/* Tile parameter of HUGKS_UPDATE_MFMA_NODE: wavefronts per cell (1 to 4,
- * where a kernel instance exists for the velocity grid; 0 = the default:
- * 3 for nv 33..48, 4 for 17..32, 1 up to 16).  HUGKS_E_UNSUPPORTED when no
- * instance exists. */
+/* Tile parameter of HUGKS_UPDATE_MFMA_NODE: wavefronts per cell, 1 to 16
+ * where a compiled kernel instance exists for the velocity grid and the
+ * precision (e.g. 3, 5 and, FP32/FP64 only, 9 at nv = 33..48; 2 and 4 at
+ * 17..32; 1 up to 16); 0 = the default: 3 for nv 33..48, 4 for 17..32, 1 up
+ * to 16.  HUGKS_E_UNSUPPORTED when no instance exists. */
 int hugks_set_update_waves(hugks_handle *h, int32_t waves);
 int hugks_get_update_waves(hugks_handle *h, int32_t *waves);
 

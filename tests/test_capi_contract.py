@@ -160,12 +160,15 @@ def have_no_gpu():
     return info
 
 
-def test_bf16x3_without_kernel_is_refused():
-    """HUGKS_MIXED_BF16X3 names split-product Hermite contractions; without
-    them the library refuses the mode instead of running another one."""
-    case = small_case()
+def test_bf16x3_refused_in_1d():
+    """HUGKS_MIXED_BF16X3 names the split-product 2D Hermite contraction; a
+    1D case has none, and the library refuses the mode instead of running
+    another one (validation before any device call, so also without a GPU)."""
+    p = dict(namelist.DEFAULTS_1D)
+    p.update(nx=16, nv=12)
+    case = cases.build_1d(p)
     d = case.bind()
     d.precision = capi.MIXED_BF16X3
     rc, msg = create(d)
     assert rc == capi.E_UNSUPPORTED, (rc, msg)
-    assert "BF16X3" in msg
+    assert "BF16X3" in msg and "2D" in msg

@@ -8,9 +8,11 @@ loop state only):
 - a 512 x 16-cell band of the 512 x 512 Rayleigh-Taylor case (the same
   cell size, GH40, Nm = 12): 4 steps.
 
-FP64 path: every conserved field within 1e-10 of the reference, the same iteration index and simulated time.  Mixed path (FP32 f and
-flux, FP64 moments): within 1e-5 on the same measure; its per-field
-errors in rho, U, V, T are printed.  Momentum is measured against the
+FP64 path: every conserved field within 1e-10 of the reference, the same
+iteration index and simulated time.  Mixed paths (FP32 f and flux, FP64
+moments; Hermite contractions on FP32 tiles or as bf16x3 split products):
+within 1e-5 on the same measure; their per-field errors in rho, U, V, T
+are printed.  Momentum is measured against the
 largest momentum of the developed channel flow, and against the thermal
 momentum sqrt(max rho * max rho E) in the Rayleigh-Taylor cases, which
 are a few steps from rest (velocities of 1e-4).  The reference
@@ -31,10 +33,11 @@ pytestmark = pytest.mark.gpu
 
 LARGE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "large")
 FIXTURES = sorted(glob.glob(os.path.join(LARGE, "*.npz")))
-TOL = {capi.FP64: 1e-10, capi.MIXED_FP32: 1e-5}
+TOL = {capi.FP64: 1e-10, capi.MIXED_FP32: 1e-5, capi.MIXED_BF16X3: 1e-5}
 
 
-@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED_FP32], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED_FP32, capi.MIXED_BF16X3],
+                         ids=["fp64", "mixed", "bf16x3"])
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
 def test_full_size_against_reference(have_gpu, path, precision):
     fx, dim, params, gpu = load_fixture(path)
