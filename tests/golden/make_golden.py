@@ -39,6 +39,10 @@ FIXTURES = {
     "p2d_rt_cartesian_6x10": ("p2d_rt_cartesian_6x10.namelist", 2, 15, 1, 5),
     # the headline velocity grid and Hermite order: GH40, Nm = 12
     "p2d_rt_gh40_nm12_6x8": ("p2d_rt_gh40_nm12_6x8.namelist", 2, 6, 1, 3),
+    # Newton-Cotes velocity grids (2d/src/velocity.f90:29-61, 1d:34-63)
+    "p2d_poiseuille_nc21_4x8": ("p2d_poiseuille_nc21_4x8.namelist", 2, 20, 0, 0),
+    "p2d_rt_cartesian_nc17_6x10": ("p2d_rt_cartesian_nc17_6x10.namelist", 2, 12, 1, 9),
+    "p1d_sod_nc100_40": ("p1d_sod_nc100_40.namelist", 1, 50, 0, 0),
     "ref1d_fourier_smoke": ("ref1d_fourier_smoke.namelist", 1, 40, 0, 0),
     "p1d_sod_40": ("p1d_sod_40.namelist", 1, 60, 0, 0),
     "p1d_hydrostatic_30": ("p1d_hydrostatic_30.namelist", 1, 40, 0, 0),
