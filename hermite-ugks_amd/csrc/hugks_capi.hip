@@ -302,8 +302,15 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
                  : d->dt_rule != HUGKS_DT_1D)
         return fail(nullptr, HUGKS_E_ARG, "time-step rule does not match the dimension");
     if (d->force_variant < HUGKS_FORCE_HERMITE_SYMMETRIC ||
-        d->force_variant > HUGKS_FORCE_HERMITE_LEGACY)
-        return fail(nullptr, HUGKS_E_ARG, "unknown Hermite force variant");
+        d->force_variant > HUGKS_FORCE_FINITE_DIFFERENCE)
+        return fail(nullptr, HUGKS_E_ARG, "unknown force variant");
+    if (d->force_variant == HUGKS_FORCE_FINITE_DIFFERENCE && (!(d->fd_span > 0) || d->nv < 3))
+        return fail(nullptr, HUGKS_E_ARG,
+                    "the finite-difference force needs fd_span > 0 and nv >= 3");
+    if (d->force_variant == HUGKS_FORCE_FINITE_DIFFERENCE && d->precision == HUGKS_MIXED_BF16X3)
+        return fail(nullptr, HUGKS_E_UNSUPPORTED,
+                    "HUGKS_MIXED_BF16X3 names the Hermite contraction; the finite-difference "
+                    "force has none");
     if (d->steady_steps < 0 || (d->steady_steps > 0 && (dim != 1 || !(d->steady_tol >= 0) ||
                                                          !(d->mass_tol >= 0))))
         return fail(nullptr, HUGKS_E_ARG, "steady test needs dim 1 and tolerances >= 0");
@@ -362,6 +369,8 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
     c.omega = d->omega; c.cfl = d->cfl; c.maxtime = d->maxtime; c.eps_res = d->eps_res;
     c.umax_dt = d->umax_dt; c.vmax_dt = d->vmax_dt; c.force_rate = d->force_rate;
     c.tref = d->tref; c.phi = d->phi;
+    c.force_fd = d->force_variant == HUGKS_FORCE_FINITE_DIFFERENCE;
+    c.fd_du = d->fd_span / d->nv;
     c.steady_steps = dim == 1 ? d->steady_steps : 0;
     c.steady_tol = d->steady_tol; c.mass_tol = d->mass_tol;
     Launch2D& L = h->L;
@@ -983,6 +992,9 @@ int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kern
     if (h->dim != 2) return HUGKS_OK;   // 1D has one kernel of each kind
     if (update_kernel == HUGKS_UPDATE_MFMA && !update2d_mfma_supported(h->L.c))
         return fail(h, HUGKS_E_UNSUPPORTED, "MFMA update needs nv <= 48 and Nm <= 14");
+    if (h->L.c.force_fd && update_kernel != HUGKS_UPDATE_LDS)
+        return fail(h, HUGKS_E_UNSUPPORTED,
+                    "the finite-difference force runs in HUGKS_UPDATE_LDS only");
     if (h->b3 && update_kernel != HUGKS_UPDATE_MFMA_NODE)
         return fail(h, HUGKS_E_UNSUPPORTED,
                     "HUGKS_MIXED_BF16X3 runs the MFMA node update only");

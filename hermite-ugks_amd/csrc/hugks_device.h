@@ -25,6 +25,8 @@ struct Consts {
     int nx, ny, nv, nn, ldv, nm;
     int bc[4];
     int tau_model, dt_rule, clamp_dt, stop_on_residual, steady_steps;
+    int force_fd;     // the finite-difference force instead of the Hermite one
+    double fd_du;     // its velocity step, lengthU / nv
     double dx, dy, pi, smv, K, gamma, prandtl, shk, muref, omega;
     double steady_tol, mass_tol;
     double cfl, maxtime, eps_res, umax_dt, vmax_dt, force_rate, tref, phi;

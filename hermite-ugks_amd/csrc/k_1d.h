@@ -341,7 +341,21 @@ __global__ __launch_bounds__(64) void k_wall1d(Consts c, DevState* __restrict__ 
     }
 }
 
+// cal_fv1, 1d/src/gravity.f90:127-154: the velocity difference of f (nodes
+// f[0..n-1], step du) at node i, one-sided second order at both ends,
+// central next to them, fourth-order central inside, in the reference's
+// expression order
+template <typename A>
+__device__ __forceinline__ double fd1(const A* f, int n, int i, double du) {
+    auto F = [&](int q) { return (double)f[q]; };
+    if (i == 0) return (-3.0 * F(0) + 4.0 * F(1) - F(2)) / 2.0 / du;
+    if (i == n - 1) return (3.0 * F(n - 1) - 4.0 * F(n - 2) + F(n - 3)) / 2.0 / du;
+    if (i == 1 || i == n - 2) return (F(i + 1) - F(i - 1)) / 2.0 / du;
+    return (-F(i + 2) + 8.0 * F(i + 1) - 8.0 * F(i - 1) + F(i - 2)) / 12.0 / du;
+}
+
 // update, 1d/src/solver.f90:141-264, with cal_fv (1d/src/gravity.f90:18-123)
+// or, for the finite-difference force, cal_fv1
 template <typename T, int EPT>
 __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict__ st,
                                                  double* __restrict__ W, double* __restrict__ part,
@@ -392,6 +406,18 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
     }
     wave_sum<2>(q2);
     const double qf = 0.5 * (q2[0] + q2[1]);
+    // the finite-difference force reads the neighbouring nodes of h and b
+    // (other lanes) from LDS
+    const bool fd = c.force_fd != 0;
+    __shared__ A fdf[2 * 64 * EPT];
+    if (fd) {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            fdf[lane + 64 * e] = rh[e];
+            fdf[64 * EPT + lane + 64 * e] = rb[e];
+        }
+        __syncthreads();
+    }
     // d_j = sqrt(2/t) sum_i h_i w_i He_j(v_i); the force accumulates
     // d_j He_{j+1}(v)/j! in the same sweep over j
     const A rt2 = (A)sqrt(2.0 / t);
@@ -403,7 +429,8 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
         p1[e] = vv[e];
     }
     double fact = 1.0;
-    for (int jj = 0; jj <= N; ++jj) {
+    const int nsweep = fd ? 0 : N + 1;   // no Hermite sweep for the difference force
+    for (int jj = 0; jj < nsweep; ++jj) {
         if (jj > 0) fact *= (double)jj;
         double d[2] = {0, 0};
 #pragma unroll
@@ -440,8 +467,15 @@ __global__ __launch_bounds__(64) void k_update1d(Consts c, DevState* __restrict_
             const double l2 = 2.0 * pn[2] * cun * cun;
             const double Hp = H + csn * cun * (l2 + c.K - 5.0) * H;
             const double Bp = B + csn * cun * (l2 + c.K - 3.0) * B;
-            const A wv = fexp(-vv[e] * vv[e] / (A)2) * inv_sqrt_2pi;
-            const double dvh = (double)(-rt2 * wv * Fh[e]), dvb = (double)(-rt2 * wv * Fb[e]);
+            double dvh, dvb;
+            if (fd) {
+                dvh = fd1(fdf, c.nn, k, c.fd_du);
+                dvb = fd1(fdf + 64 * EPT, c.nn, k, c.fd_du);
+            } else {
+                const A wv = fexp(-vv[e] * vv[e] / (A)2) * inv_sqrt_2pi;
+                dvh = (double)(-rt2 * wv * Fh[e]);
+                dvb = (double)(-rt2 * wv * Fb[e]);
+            }
             const double dfh = ((double)fh[(size_t)cell * c.ldv + k] - (double)fh[(size_t)(cell + 1) * c.ldv + k]) * idx;
             const double dfb = ((double)fb[(size_t)cell * c.ldv + k] - (double)fb[(size_t)(cell + 1) * c.ldv + k]) * idx;
             const size_t o = (size_t)cell * c.ldv + k;

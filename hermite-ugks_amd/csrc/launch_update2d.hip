@@ -47,7 +47,10 @@ static hipError_t launch_update_t(const Launch2D& L, hipStream_t s, bool prepare
     return hipGetLastError();
 }
 
-bool update2d_mfma_supported(const Consts& c) { return c.nv <= 48 && c.nm <= MF_NB - 2; }
+// (the finite-difference force has no Hermite contraction: LDS kernel only)
+bool update2d_mfma_supported(const Consts& c) {
+    return c.nv <= 48 && c.nm <= MF_NB - 2 && !c.force_fd;
+}
 
 size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64) {
     return mf_lds_bytes(c.nv, fp64 ? sizeof(double) : sizeof(float));

@@ -24,7 +24,8 @@ module hugks_capi
   integer(c_int32_t), parameter :: HUGKS_DT_2D_GENERAL = 0, HUGKS_DT_2D_WALL_NORMAL = 1, &
        HUGKS_DT_1D = 2
   integer(c_int32_t), parameter :: HUGKS_FORCE_HERMITE_SYMMETRIC = 0, &
-       HUGKS_FORCE_HERMITE_SYMMETRIC_B = 1, HUGKS_FORCE_HERMITE_LEGACY = 2
+       HUGKS_FORCE_HERMITE_SYMMETRIC_B = 1, HUGKS_FORCE_HERMITE_LEGACY = 2, &
+       HUGKS_FORCE_FINITE_DIFFERENCE = 3
   integer(c_int32_t), parameter :: HUGKS_STOP_NONE = 0, HUGKS_STOP_MAXTIME = 1, &
        HUGKS_STOP_RESIDUAL = 2, HUGKS_STOP_TIME = 3, HUGKS_STOP_STEADY = 4, HUGKS_STOP_ERROR = 5
   integer(c_int32_t), parameter :: HUGKS_FLUX_FUSED = 0, HUGKS_FLUX_SPLIT = 1, HUGKS_FLUX_STENCIL = 2, &
@@ -53,7 +54,8 @@ module hugks_capi
     real(c_double) :: tw(4) = 1
     real(c_double) :: tref = 1, phi = 0
     real(c_double) :: steady_tol = 0, mass_tol = 0
-    real(c_double) :: reserved_d(2) = 0
+    real(c_double) :: fd_span = 0
+    real(c_double) :: reserved_d(1) = 0
     type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, weight = c_null_ptr, prim0 = c_null_ptr
     type(c_ptr) :: accel_cell = c_null_ptr, accel_vface = c_null_ptr, accel_hface = c_null_ptr
   end type hugks_desc

@@ -92,6 +92,7 @@ contains
       call gauss_hermite(nv, Tref, PI, uspace, weight, roots, wgh)
       umax_dt = uspace(nv)
       du = (uspace(nv) - uspace(1)) / (nv - 1)    ! 1d/src/velocity.f90:146
+      d%fd_span = uspace(nv) - uspace(1)          ! lengthU, 1d/src/velocity.f90:147
     else if (trim(quad_type) == 'newton_cotes') then
       du = (2.d0 * vmax * sqrt(Tref)) / (nv - 1)
       do i = 1, nv
@@ -99,12 +100,9 @@ contains
         weight(i) = newton_cotes_coeff(i, nv) * du
       end do
       umax_dt = vmax * sqrt(Tref)
+      d%fd_span = vmax * sqrt(Tref) - (-vmax * sqrt(Tref))   ! lengthU, 1d/src/config.f90:75-78
     else
       write(*, *) "unknown quad_type '", trim(quad_type), "'"
-      error stop 1
-    end if
-    if (trim(force_type) /= 'hermite') then
-      write(*, *) "1D force_type '", trim(force_type), "' has no GPU operator"
       error stop 1
     end if
 
@@ -127,7 +125,15 @@ contains
       write(*, *) "unknown precision '", trim(precision), "'"
       error stop 1
     end select
-    d%force_variant = HUGKS_FORCE_HERMITE_LEGACY
+    select case (trim(force_type))              ! 1d/src/solver.f90:233-242
+    case ('hermite')
+      d%force_variant = HUGKS_FORCE_HERMITE_LEGACY
+    case ('finite_difference')
+      d%force_variant = HUGKS_FORCE_FINITE_DIFFERENCE
+    case default
+      write(*, *) "unknown force_type '", trim(force_type), "'"
+      error stop 1
+    end select
     d%clamp_dt = 0
     d%dx = dx
     d%dy = 1.d0

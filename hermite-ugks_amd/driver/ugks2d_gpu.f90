@@ -121,6 +121,7 @@ contains
       end do
       umax_dt = uspace(nv)
       vmax_dt = vspace(nv)
+      d%fd_span = uspace(nv) - uspace(1)      ! lengthU, 2d/src/velocity.f90:131-134
     else if (trim(quad_type) == 'newton_cotes') then
       du = (2.d0 * vmax * sqrt(Tref)) / (nv - 1)
       do i = 1, nv
@@ -134,6 +135,7 @@ contains
       end do
       umax_dt = vmax * sqrt(Tref)
       vmax_dt = umax_dt
+      d%fd_span = vmax * sqrt(Tref) - (-vmax * sqrt(Tref))   ! lengthU, 2d/src/config.f90:75-78
     else
       write(*, *) "unknown quad_type '", trim(quad_type), "'"
       error stop 1
@@ -168,8 +170,10 @@ contains
       d%force_variant = HUGKS_FORCE_HERMITE_SYMMETRIC_B
     case ('hermite')
       d%force_variant = HUGKS_FORCE_HERMITE_LEGACY
+    case ('finite_difference')
+      d%force_variant = HUGKS_FORCE_FINITE_DIFFERENCE
     case default
-      write(*, *) "force_type '", trim(force_type), "' has no GPU operator"
+      write(*, *) "unknown force_type '", trim(force_type), "'"
       error stop 1
     end select
     d%clamp_dt = 1

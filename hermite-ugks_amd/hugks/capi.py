@@ -20,6 +20,7 @@ BC_PERIODIC, BC_SPECULAR, BC_DIFFUSE = 0, 1, 2
 TAU_MU_OVER_RHO, TAU_VHS = 0, 1
 DT_2D_GENERAL, DT_2D_WALL_NORMAL, DT_1D = 0, 1, 2
 FORCE_HERMITE_SYMMETRIC, FORCE_HERMITE_SYMMETRIC_B, FORCE_HERMITE_LEGACY = 0, 1, 2
+FORCE_FINITE_DIFFERENCE = 3
 STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL, STOP_TIME, STOP_STEADY, STOP_ERROR = 0, 1, 2, 3, 4, 5
 FLUX_FUSED, FLUX_SPLIT, FLUX_STENCIL, UPDATE_LDS, UPDATE_MFMA, UPDATE_MFMA_NODE = 0, 1, 2, 0, 1, 2
 BATCH_AUTOTUNE = 1
@@ -40,7 +41,8 @@ class Desc(C.Structure):
         ("cfl", C.c_double), ("maxtime", C.c_double), ("eps_res", C.c_double),
         ("umax_dt", C.c_double), ("vmax_dt", C.c_double), ("force_rate", C.c_double),
         ("tw", C.c_double * 4), ("tref", C.c_double), ("phi", C.c_double),
-        ("steady_tol", C.c_double), ("mass_tol", C.c_double), ("reserved_d", C.c_double * 2),
+        ("steady_tol", C.c_double), ("mass_tol", C.c_double), ("fd_span", C.c_double),
+        ("reserved_d", C.c_double * 1),
         ("u", _dp), ("v", _dp), ("weight", _dp), ("prim0", _dp), ("accel_cell", _dp),
         ("accel_vface", _dp), ("accel_hface", _dp),
     ]

@@ -50,11 +50,13 @@ def _velocity_2d(p):
     if str(p["quad_type"]).strip() == "gauss_hermite":
         u, v, w = gauss_hermite_2d(nv, tref)
         umax, vmax = u[-1], v[-1]
+        span = u[-1] - u[0]                      # lengthU, 2d/src/velocity.f90:131-134
     else:
         u, v, w = newton_cotes_2d(nv, float(p["vmax"]), tref)
         umax = vmax = float(p["vmax"]) * math.sqrt(tref)
+        span = umax - (-umax)                    # lengthU, 2d/src/config.f90:73-80
     gap = min(np.min(np.diff(u)), np.min(np.diff(v)))   # 2d/src/solver.f90:53-54
-    return u, v, w, umax, vmax, gap
+    return u, v, w, umax, vmax, gap, span
 
 
 def _force_variant(force_type):
@@ -65,7 +67,9 @@ def _force_variant(force_type):
         return capi.FORCE_HERMITE_SYMMETRIC_B
     if ft == "hermite":
         return capi.FORCE_HERMITE_LEGACY
-    raise ValueError(f"force_type '{ft}' has no GPU operator (finite_difference is not supported)")
+    if ft == "finite_difference":
+        return capi.FORCE_FINITE_DIFFERENCE
+    raise ValueError(f"unknown force_type '{ft}'")
 
 
 def build_2d(p, precision=capi.FP64, rt_variant="reference", seed=0):
@@ -75,7 +79,7 @@ def build_2d(p, precision=capi.FP64, rt_variant="reference", seed=0):
     phi = float(p["phi"])
     xlength, ylength = float(p["xlength"]), float(p["ylength"])
     dx, dy = xlength / nx, ylength / ny          # 2d/src/initialization.f90:69-70
-    u, v, w, umax, vmax, gap = _velocity_2d(p)
+    u, v, w, umax, vmax, gap, span = _velocity_2d(p)
     kn = float(p["knudsen"])
     innerk = 1.0
     gamma = (innerk + 4.0) / (innerk + 2.0)      # 2d/src/gastheory2d.f90:70-76
@@ -91,6 +95,7 @@ def build_2d(p, precision=capi.FP64, rt_variant="reference", seed=0):
     d.nx, d.ny, d.nv, d.nm = nx, ny, nv, int(p["Nm"])
     d.precision = precision
     d.force_variant = _force_variant(p["force_type"])
+    d.fd_span = span
     d.clamp_dt = 1
     d.dx, d.dy = dx, dy
     d.pi, d.smv, d.innerk, d.gamma = PI_REF, SMV_2D, innerk, gamma
@@ -202,10 +207,12 @@ def build_1d(p, precision=capi.FP64):
         u, w, _, _ = gauss_hermite_1d(nv, tref)
         umin, umax = u[0], u[-1]
         du = (umax - umin) / (nv - 1)              # 1d/src/velocity.f90:146
+        span = umax - umin                         # lengthU, 1d/src/velocity.f90:147
     else:
         u, w = newton_cotes_1d(nv, float(p["vmax"]), tref)
         umax = float(p["vmax"]) * math.sqrt(tref)
         du = (2 * umax) / (nv - 1)
+        span = umax - (-umax)                      # lengthU, 1d/src/config.f90:75-78
     innerk = 2.0
     gamma = (innerk + 3.0) / (innerk + 1.0)        # 1d/src/gastheory.f90:60-66
     x = (np.arange(1, nx + 1) - 0.5) * dx
@@ -216,9 +223,11 @@ def build_1d(p, precision=capi.FP64):
     d.nx, d.ny, d.nv, d.nm = nx, 1, nv, int(p["Nm"])
     d.precision = precision
     ft = str(p["force_type"]).strip()
-    if ft != "hermite":
-        raise ValueError(f"1D force_type '{ft}' has no GPU operator")
-    d.force_variant = capi.FORCE_HERMITE_LEGACY
+    if ft not in ("hermite", "finite_difference"):         # 1d/src/solver.f90:233-242
+        raise ValueError(f"unknown 1D force_type '{ft}'")
+    d.force_variant = (capi.FORCE_HERMITE_LEGACY if ft == "hermite"
+                       else capi.FORCE_FINITE_DIFFERENCE)
+    d.fd_span = span
     d.clamp_dt = 0
     d.dx, d.dy = dx, 1.0
     d.pi, d.smv, d.innerk, d.gamma = PI_REF, SMV_1D, innerk, gamma

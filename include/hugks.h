@@ -84,6 +84,12 @@ extern "C" {
 #define HUGKS_FORCE_HERMITE_SYMMETRIC   0 /* 2d/src/hermite_symmetric2d.f90:23  */
 #define HUGKS_FORCE_HERMITE_SYMMETRIC_B 1 /* 2d/src/hermite_symmetric2d.f90:120 */
 #define HUGKS_FORCE_HERMITE_LEGACY      2 /* 2d/src/gravity.f90:189; 1D cal_fv */
+#define HUGKS_FORCE_FINITE_DIFFERENCE   3 /* 'finite_difference': velocity-space
+                                             differences of f on the step
+                                             fd_span/nv (2d/src/gravity.f90:
+                                             108-187, 1d/src/gravity.f90:127-
+                                             154); nv >= 3; 2D runs it in
+                                             HUGKS_UPDATE_LDS only        */
 
 /* reason an advance stopped */
 #define HUGKS_STOP_NONE      0  /* step cap reached                         */
@@ -108,8 +114,8 @@ typedef struct hugks_desc {
     int32_t stop_on_residual; /* 1: advance stops when all(res < eps_res)     */
     int32_t bc[4];            /* x-lo, x-hi, y-lo, y-hi (1D uses bc[0..1])    */
     int32_t force_variant;    /* HUGKS_FORCE_*: the reference operator named
-                                 by force_type; the library evaluates all of
-                                 them with one separable Hermite operator    */
+                                 by force_type; the library evaluates the
+                                 Hermite ones with one separable operator    */
     int32_t steady_steps;     /* 1D: stop after this many consecutive steady
                                  steps (case 4, 1d/src/main.f90:109-126);
                                  0 disables the test                         */
@@ -132,7 +138,12 @@ typedef struct hugks_desc {
     double phi;               /* 1D acceleration (2D uses the arrays below)   */
     double steady_tol;        /* 1D steady test: max steady_res <= steady_tol */
     double mass_tol;          /*   and |relative mass change| <= mass_tol     */
-    double reserved_d[2];
+    double fd_span;           /* velocity span lengthU of the finite-difference
+                                 force (its step is fd_span/nv): umax - umin
+                                 of the reference's CONFIG for Newton-Cotes
+                                 grids, the node span for Gauss-Hermite ones;
+                                 unused by the Hermite forces                */
+    double reserved_d[1];
     const double *u;          /* [nv] u nodes                                 */
     const double *v;          /* [nv] v nodes (2D)                            */
     const double *weight;     /* [nv*nv] (2D, u fastest) or [nv] (1D)         */

@@ -303,6 +303,29 @@ static void wall1(oracle1d *o, int kind, int rot, double tw, int ic, double *flu
     flux[2] = o->dt * (0.5 * s2);
 }
 
+/* cal_fv1, 1d/src/gravity.f90:127-154: velocity differences on the step
+ * du = lengthU/unum, one-sided second order at both ends, central next to
+ * them, fourth-order central inside, in the reference's expression order
+ * (for nv = 3 the second node's form is written twice, the same value) */
+static void calfv1(const hugks_desc *d, int nv, const double *h, const double *b, double *dvh,
+                   double *dvb)
+{
+    const double du = d->fd_span / nv;
+    const double *f[2] = {h, b};
+    double *o[2] = {dvh, dvb};
+    for (int a = 0; a < 2; ++a) {
+        const double *x = f[a];
+        double *y = o[a];
+        int n = nv;
+        y[0] = (-3.0 * x[0] + 4.0 * x[1] - x[2]) / 2.0 / du;
+        y[1] = (x[2] - x[0]) / 2.0 / du;
+        y[n - 2] = (x[n - 1] - x[n - 3]) / 2.0 / du;
+        y[n - 1] = (3.0 * x[n - 1] - 4.0 * x[n - 2] + x[n - 3]) / 2.0 / du;
+        for (int i = 2; i <= n - 3; ++i)
+            y[i] = (-x[i + 2] + 8.0 * x[i + 1] - 8.0 * x[i - 1] + x[i - 2]) / 12.0 / du;
+    }
+}
+
 /* cal_fv, 1d/src/gravity.f90:18-123 */
 static void calfv(const hugks_desc *d, int nv, const double *u, const double *weight, int N,
                   const double *h, const double *b, const double *prim, double *dvh, double *dvb)
@@ -459,7 +482,10 @@ int oracle1d_step(oracle1d *o, hugks_step_info *info)
         shakhov1(d, nv, o->u, Ho, Bo, qf, prim_old, Hp, Bp);
         shakhov1(d, nv, o->u, H, B, qf, prim, Hp, Bp);
         for (int k = 0; k < nv; ++k) { H[k] += Hp[k]; B[k] += Bp[k]; }
-        calfv(d, nv, o->u, o->weight, d->nm, h, b, prim_old, dvh, dvb);
+        if (d->force_variant == HUGKS_FORCE_FINITE_DIFFERENCE)
+            calfv1(d, nv, h, b, dvh, dvb);   /* 1d/src/solver.f90:234-235 */
+        else
+            calfv(d, nv, o->u, o->weight, d->nm, h, b, prim_old, dvh, dvb);
         const double *fhl = o->fh + (size_t)nv * i, *fhr = o->fh + (size_t)nv * (i + 1);
         const double *fbl = o->fb + (size_t)nv * i, *fbr = o->fb + (size_t)nv * (i + 1);
         for (int k = 0; k < nv; ++k) {

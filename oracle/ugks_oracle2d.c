@@ -827,6 +827,46 @@ static void evolution(oracle2d *o)
 #undef HF
 }
 
+/* One velocity-axis difference of cal_fsource1, 2d/src/gravity.f90:108-187,
+ * at node i (0-based) of a line of n nodes (stride st) of f, step du: the
+ * one-sided second-order forms at both ends, the central one next to them
+ * and the fourth-order central one inside, in the reference's expression
+ * order.  As the reference writes them: the interior form takes its three
+ * inner terms from g (the u-axis difference of b reads them from h), and
+ * the last node's form has the sign of a forward difference when last_fwd
+ * (the v axis). */
+static double fd_axis(const double *f, const double *g, size_t st, int n, int i, double du,
+                      int last_fwd)
+{
+#define F(q) f[(size_t)(q) * st]
+#define G(q) g[(size_t)(q) * st]
+    if (i == 0) return (-3.0 * F(0) + 4.0 * F(1) - F(2)) / du / 2.0;
+    if (i == n - 1)
+        return last_fwd ? (-3.0 * F(n - 1) + 4.0 * F(n - 2) - F(n - 3)) / du / 2.0
+                        : (3.0 * F(n - 1) - 4.0 * F(n - 2) + F(n - 3)) / du / 2.0;
+    if (i == 1 || i == n - 2) return (F(i + 1) - F(i - 1)) / du / 2.0;
+    return (-F(i + 2) + 8.0 * G(i + 1) - 8.0 * G(i - 1) + G(i - 2)) / du / 12.0;
+#undef F
+#undef G
+}
+
+/* force_type 'finite_difference' (2d/src/solver.f90:353-356): hsource =
+ * duh dt fx + dvh dt fy, bsource likewise, with du = dv = lengthU/unum */
+static void force_fd(const hugks_desc *d, int nv, const double *h, const double *b, double dt,
+                     double fx, double fy, double *hs, double *bs)
+{
+    const double du = d->fd_span / nv, dv = d->fd_span / nv;
+    for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < nv; ++i) {
+            const double duh = fd_axis(h + (size_t)nv * j, h + (size_t)nv * j, 1, nv, i, du, 0);
+            const double dub = fd_axis(b + (size_t)nv * j, h + (size_t)nv * j, 1, nv, i, du, 0);
+            const double dvh = fd_axis(h + i, h + i, (size_t)nv, nv, j, dv, 1);
+            const double dvb = fd_axis(b + i, b + i, (size_t)nv, nv, j, dv, 1);
+            hs[i + (size_t)nv * j] = duh * dt * fx + dvh * dt * fy;
+            bs[i + (size_t)nv * j] = dub * dt * fx + dvb * dt * fy;
+        }
+}
+
 /* update, 2d/src/solver.f90:245-395 */
 static void update(oracle2d *o)
 {
@@ -862,7 +902,12 @@ static void update(oracle2d *o)
             for (size_t k = 0; k < nn; ++k) { Ho[k] += Hp[k]; Bo[k] += Bp[k]; }
             shakhov_part(d, (int)nn, H, B, o->uspace, o->vspace, qf, prim, Hp, Bp);
             for (size_t k = 0; k < nn; ++k) { H[k] += Hp[k]; B[k] += Bp[k]; }
-            if (d->force_variant == HUGKS_FORCE_HERMITE_LEGACY)
+            /* fth, ftb: the Hermite force terms ft1, ft2 (hsource = ft1 dt),
+               or the finite-difference hsource, bsource themselves */
+            const int use_fd = d->force_variant == HUGKS_FORCE_FINITE_DIFFERENCE;
+            if (use_fd)
+                force_fd(d, o->nv, h, b, o->dt, fx, fy, fth, ftb);
+            else if (d->force_variant == HUGKS_FORCE_HERMITE_LEGACY)
                 force_legacy(d, (int)nn, o->weight, o->uspace, o->vspace, d->nm, h, b, prim_old,
                              fx, fy, fth, ftb);
             else
@@ -875,7 +920,8 @@ static void update(oracle2d *o)
             const double *bd = o->hfb + nn * HI(o, i, j), *bu = o->hfb + nn * HI(o, i, j + 1);
             double den = 1.0 + 0.5 * o->dt / tau;
             for (size_t k = 0; k < nn; ++k) {
-                double hs = fth[k] * o->dt, bs = ftb[k] * o->dt;
+                double hs = use_fd ? fth[k] : fth[k] * o->dt;
+                double bs = use_fd ? ftb[k] : ftb[k] * o->dt;
                 h[k] = (h[k] + (hl[k] - hr[k] + hd[k] - hu[k]) / area +
                         0.5 * o->dt * (H[k] / tau + (Ho[k] - h[k]) / tau_old) - hs) / den;
                 b[k] = (b[k] + (bl[k] - br[k] + bd[k] - bu[k]) / area +

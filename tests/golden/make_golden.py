@@ -43,6 +43,10 @@ FIXTURES = {
     "p2d_poiseuille_nc21_4x8": ("p2d_poiseuille_nc21_4x8.namelist", 2, 20, 0, 0),
     "p2d_rt_cartesian_nc17_6x10": ("p2d_rt_cartesian_nc17_6x10.namelist", 2, 12, 1, 9),
     "p1d_sod_nc100_40": ("p1d_sod_nc100_40.namelist", 1, 50, 0, 0),
+    # the finite-difference force (2d/src/gravity.f90:108-187, 1d:127-154)
+    "p2d_poiseuille_fd_nc21_4x8": ("p2d_poiseuille_fd_nc21_4x8.namelist", 2, 20, 0, 0),
+    "p2d_rt_cartesian_fd_gh12_6x10": ("p2d_rt_cartesian_fd_gh12_6x10.namelist", 2, 12, 1, 4),
+    "p1d_sod_fd_nc100_40": ("p1d_sod_fd_nc100_40.namelist", 1, 50, 0, 0),
     "ref1d_fourier_smoke": ("ref1d_fourier_smoke.namelist", 1, 40, 0, 0),
     "p1d_sod_40": ("p1d_sod_40.namelist", 1, 60, 0, 0),
     "p1d_hydrostatic_30": ("p1d_hydrostatic_30.namelist", 1, 40, 0, 0),
