@@ -13,8 +13,10 @@
 #include <map>
 
 #include "hugks.h"
+#include "block_writer.h"
 #include "hugks_device.h"
 #include "hugks_kernels.h"
+#include <memory>
 
 using namespace hugks;
 
@@ -128,6 +130,7 @@ struct hugks_handle {
     std::string err;
     bool timing = false;
     KTimes kt;
+    std::unique_ptr<BlockWriter> writer;   // macro-file output thread, made on first use
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t ncell = 0;
     uint64_t run_hash = 0;   // of the inputs that define the run (checkpoints)
@@ -162,7 +165,8 @@ static uint64_t run_hash_of(const hugks_desc* d, size_t ncell) {
                           d->bc[3], d->force_variant};
     const double dv[] = {d->dx, d->dy, d->pi, d->smv, d->innerk, d->gamma, d->prandtl,
                          d->shakhov_coef, d->muref, d->omega, d->tw[0], d->tw[1], d->tw[2],
-                         d->tw[3], d->phi, d->umax_dt, d->vmax_dt, d->force_rate};
+                         d->tw[3], d->phi, d->umax_dt, d->vmax_dt, d->force_rate,
+                         d->fd_span};
     r = fnv1a(r, iv, sizeof iv);
     return fnv1a(r, dv, sizeof dv);
 }
@@ -190,6 +194,7 @@ static hipError_t dalloc(hugks_handle* h, P** p, size_t bytes) {
 
 static void release(hugks_handle* h) {
     if (!h) return;
+    h->writer.reset();   // writes the files still queued
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
@@ -630,6 +635,41 @@ int hugks_destroy(hugks_handle* h) {
     release(h);
     delete h;
     return HUGKS_OK;
+}
+
+// ----------------------------------------------------------- output
+
+int hugks_write_blocks_async(hugks_handle* h, const char* path, const char* header,
+                             int32_t per_line, int32_t nblock, const int64_t* block_len,
+                             const double* values) {
+    if (!path || !header || per_line < 1 || nblock < 0 || (nblock > 0 && !block_len))
+        return fail(h, HUGKS_E_ARG, "write_blocks: null path/header/lengths or bad counts");
+    BlockFile f;
+    f.path = path;
+    f.header = header;
+    f.per_line = per_line;
+    int64_t total = 0;
+    for (int32_t k = 0; k < nblock; ++k) {
+        if (block_len[k] < 0) return fail(h, HUGKS_E_ARG, "write_blocks: negative block length");
+        total += block_len[k];
+    }
+    if (total > 0 && !values) return fail(h, HUGKS_E_ARG, "write_blocks: null values");
+    f.block_len.assign(block_len, block_len + nblock);
+    f.values.assign(values, values + total);
+    if (!h) {
+        const std::string e = write_block_file(f);
+        return e.empty() ? HUGKS_OK : fail(nullptr, HUGKS_E_IO, e);
+    }
+    if (!h->writer) h->writer = std::make_unique<BlockWriter>();
+    h->writer->push(std::move(f));
+    return HUGKS_OK;
+}
+
+int hugks_write_wait(hugks_handle* h) {
+    if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    if (!h->writer) return HUGKS_OK;
+    const std::string e = h->writer->wait();
+    return e.empty() ? HUGKS_OK : fail(h, HUGKS_E_IO, e);
 }
 
 int hugks_step(hugks_handle* h, hugks_step_info* info) {

@@ -113,6 +113,22 @@ module hugks_capi
       type(c_ptr), value :: w, q
     end function hugks_get_macro
 
+    ! macro*.dat files written by a writer thread of the library, off the
+    ! step loop; path and header are NUL-terminated
+    integer(c_int) function hugks_write_blocks_async(handle, path, header, per_line, nblock, &
+         block_len, values) bind(c, name='hugks_write_blocks_async')
+      import :: c_int, c_int32_t, c_ptr, c_char
+      type(c_ptr), value :: handle
+      character(kind=c_char), dimension(*), intent(in) :: path, header
+      integer(c_int32_t), value :: per_line, nblock
+      type(c_ptr), value :: block_len, values
+    end function hugks_write_blocks_async
+
+    integer(c_int) function hugks_write_wait(handle) bind(c, name='hugks_write_wait')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+    end function hugks_write_wait
+
     integer(c_int) function hugks_get_f(handle, hf, bf) bind(c, name='hugks_get_f')
       import :: c_int, c_ptr
       type(c_ptr), value :: handle

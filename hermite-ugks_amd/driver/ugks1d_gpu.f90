@@ -238,10 +238,13 @@ contains
     logical, intent(in) :: stepped
     real(8), allocatable, target :: w(:, :), q(:)
     real(8), allocatable :: sol(:, :), pref(:)
+    real(c_double), allocatable, target :: vals(:)
+    integer(c_int64_t), target :: lens(10)
     real(8) :: rho, uu, lam
     character(len=8) :: ctemp
+    character(len=80) :: zone
     logical :: have
-    integer :: i, u
+    integer :: i
 
     allocate(w(3, nx), q(nx), sol(9, nx), pref(nx))
     call hugks_check(hugks_get_macro(h, c_loc(w), c_loc(q)), h, 'get_macro')
@@ -267,16 +270,22 @@ contains
         sol(7:9, i) = 0.d0
       end if
     end do
+    ! the file of output.f90:89-99 -- the list-directed VARIABLES line (a
+    ! leading blank), the ZONE line, then the blocks in the
+    ! "(9(ES23.16,2X))" form -- is written by the library's writer thread
     write(ctemp, '(i8)') loop
-    open(newunit=u, file='macro'//trim(adjustl(ctemp))//'.dat', status='replace', &
-         action='write')
-    write(u, *) "VARIABLES = X, RHO, U, T, P, PERTURBATION, Q, error1, error2, N"
-    write(u, '(A,I5,A,I5,A)') 'ZONE I =', nx, ', DATAPACKING=BLOCK'
-    write(u, "(9(ES23.16,2X))") xc
+    write(zone, '(A,I5,A,I5,A)') 'ZONE I =', nx, ', DATAPACKING=BLOCK'
+    allocate(vals(10 * nx))
+    vals(1:nx) = xc
     do i = 1, 9
-      write(u, "(9(ES23.16,2X))") sol(i, :)
+      vals(i * nx + 1:(i + 1) * nx) = sol(i, :)
     end do
-    close(u)
+    lens = nx
+    call hugks_check(hugks_write_blocks_async(h, &
+        'macro'//trim(adjustl(ctemp))//'.dat'//c_null_char, &
+        ' VARIABLES = X, RHO, U, T, P, PERTURBATION, Q, error1, error2, N' // &
+        new_line('a')//trim(zone)//new_line('a')//c_null_char, &
+        9_c_int32_t, 10_c_int32_t, c_loc(lens), c_loc(vals)), h, 'write_macro')
   end subroutine write_macro
 
 end module ugks1d_setup
@@ -319,6 +328,7 @@ program ugks1d_gpu
     write(*, *) 'WARNING: Fourier case reached maxtime before steady tolerance'
   end if
   call write_macro(h, iter, info%steps > 0)
+  call hugks_check(hugks_write_wait(h), h, 'write_macro')
   call system_clock(c1)
   write(*, *) "Elapsed time: ", real(c1 - c0) / real(rate), " seconds"
   call hugks_check(hugks_destroy(h), h, 'destroy')

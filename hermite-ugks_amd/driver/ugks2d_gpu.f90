@@ -323,9 +323,12 @@ contains
     integer, intent(in) :: loop
     real(8), allocatable, target :: w(:, :, :), q(:, :, :)
     real(8), allocatable :: sol(:, :, :), coord(:, :, :)
+    real(c_double), allocatable, target :: vals(:)
+    integer(c_int64_t), target :: lens(13)
     real(8) :: prim(4)
     character(len=11) :: ctemp
-    integer :: i, j, u
+    character(len=160) :: zone
+    integer :: i, j, np1, nc
 
     allocate(w(4, nx, ny), q(2, nx, ny), sol(11, nx, ny), coord(2, nx + 1, ny + 1))
     call hugks_check(hugks_get_macro(h, c_loc(w), c_loc(q)), h, 'get_macro')
@@ -353,18 +356,28 @@ contains
         coord(2, i, j) = (j - 1) * dy
       end do
     end do
+    ! the file of output2d.f90:38-55 -- the list-directed VARIABLES line
+    ! (a leading blank), the ZONE line, then the blocks in the
+    ! "(11(ES23.16,2X))" form -- is written by the library's writer thread
+    ! while the run goes on
     write(ctemp, '(i11)') loop
-    open(newunit=u, file='macro'//trim(adjustl(ctemp))//'.dat', status='replace', &
-         action='write')
-    write(u, *) "VARIABLES = X, Y, RHO, U, V, T, P, QX, QY, radius, radius-0, RHO-0,error"
-    write(u, '(A,I5,A,I5,A)') 'ZONE I =', nx + 1, ', J =', ny + 1, &
+    write(zone, '(A,I5,A,I5,A)') 'ZONE I =', nx + 1, ', J =', ny + 1, &
         ', DATAPACKING=BLOCK,VARLOCATION=([3-13]=CELLCENTERED)'
-    write(u, "(11(ES23.16,2X))") coord(1, :, :)
-    write(u, "(11(ES23.16,2X))") coord(2, :, :)
+    np1 = (nx + 1) * (ny + 1)
+    nc = nx * ny
+    allocate(vals(2 * np1 + 11 * nc))
+    vals(1:np1) = reshape(coord(1, :, :), [np1])
+    vals(np1 + 1:2 * np1) = reshape(coord(2, :, :), [np1])
     do i = 1, 11
-      write(u, "(11(ES23.16,2X))") sol(i, :, :)
+      vals(2 * np1 + (i - 1) * nc + 1:2 * np1 + i * nc) = reshape(sol(i, :, :), [nc])
     end do
-    close(u)
+    lens(1:2) = np1
+    lens(3:13) = nc
+    call hugks_check(hugks_write_blocks_async(h, &
+        'macro'//trim(adjustl(ctemp))//'.dat'//c_null_char, &
+        ' VARIABLES = X, Y, RHO, U, V, T, P, QX, QY, radius, radius-0, RHO-0,error' // &
+        new_line('a')//trim(zone)//new_line('a')//c_null_char, &
+        11_c_int32_t, 13_c_int32_t, c_loc(lens), c_loc(vals)), h, 'write_macro')
   end subroutine write_macro
 
   ! get_primary, 2d/src/gastheory2d.f90:28-36 (gamma of K = 1)
@@ -517,5 +530,6 @@ program ugks2d_gpu
     call hugks_check(hugks_checkpoint_write(h, checkpoint_file), h, 'checkpoint')
     write(*, *) "state saved to ", trim(checkpoint_file)
   end if
+  call hugks_check(hugks_write_wait(h), h, 'write_macro')
   call hugks_check(hugks_destroy(h), h, 'destroy')
 end program ugks2d_gpu

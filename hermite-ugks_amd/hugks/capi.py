@@ -15,6 +15,7 @@ LIBPATH = os.environ.get("HUGKS_LIB") or os.path.join(os.path.dirname(HERE), "li
 ABI_VERSION = 1
 OK = 0
 E_ARG, E_NOMEM, E_DEVICE, E_UNSUPPORTED, E_NUMERIC, E_WALL, E_NOHIP = -1, -2, -3, -4, -5, -6, -7
+E_IO = -8
 FP64, MIXED_FP32, MIXED_BF16X3 = 0, 1, 2
 BC_PERIODIC, BC_SPECULAR, BC_DIFFUSE = 0, 1, 2
 TAU_MU_OVER_RHO, TAU_VHS = 0, 1
@@ -73,7 +74,7 @@ EXPORTS = [
     "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_grouped",
     "hugks_batch_handle",
     "hugks_batch_advance", "hugks_batch_set_timing", "hugks_batch_kernel_times",
-    "hugks_batch_reset_kernel_times",
+    "hugks_batch_reset_kernel_times", "hugks_write_blocks_async", "hugks_write_wait",
 ]
 ARR_SHX, ARR_SHY, ARR_SBX, ARR_SBY, ARR_VFH, ARR_VFB, ARR_HFH, ARR_HFB, ARR_VFLUX, ARR_HFLUX = range(10)
 
@@ -132,8 +133,22 @@ def load(path=LIBPATH):
     lib.hugks_batch_kernel_times.argtypes = [H, C.c_int32, C.POINTER(C.c_char_p), _dp,
                                              C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.hugks_batch_reset_kernel_times.argtypes = [H]
+    lib.hugks_write_blocks_async.argtypes = [H, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32,
+                                             C.POINTER(C.c_int64), _dp]
+    lib.hugks_write_wait.argtypes = [H]
     _lib = lib
     return lib
+
+
+def write_blocks(path, header, per_line, blocks, lib=None, h=None):
+    """hugks_write_blocks_async; with h None the file is written before
+    the call returns.  Returns the status code."""
+    lib = lib or load()
+    vals = np.ascontiguousarray(np.concatenate([np.ravel(b) for b in blocks])
+                                if blocks else np.zeros(0), dtype=np.float64)
+    lens = (C.c_int64 * max(len(blocks), 1))(*[int(np.size(b)) for b in blocks])
+    return lib.hugks_write_blocks_async(h, str(path).encode(), header.encode(), int(per_line),
+                                        len(blocks), lens, _ptr(vals))
 
 
 class HugksError(RuntimeError):
@@ -250,6 +265,15 @@ class Solver:
         q = np.empty((self.case.ncell, d.dim)) if with_q else None
         self._check(self.lib.hugks_get_macro_noalloc(self.h, _ptr(w), _ptr(q)))
         return (w, q) if with_q else w
+
+    def write_blocks_async(self, path, header, per_line, blocks):
+        """Queues a macro*.dat-style file on the handle's writer thread
+        (hugks_write_blocks_async): header text, then each block (a 1D array)
+        as records of per_line ES23.16 fields."""
+        self._check(write_blocks(path, header, per_line, blocks, self.lib, self.h))
+
+    def write_wait(self):
+        self._check(self.lib.hugks_write_wait(self.h))
 
     def get_f(self):
         n = self.case.ncell * self.case.nn

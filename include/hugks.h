@@ -51,6 +51,7 @@ extern "C" {
 #define HUGKS_E_NUMERIC      -5  /* NaN/Inf or T <= 0 met on the device      */
 #define HUGKS_E_WALL         -6  /* zero outgoing wall mass coefficient      */
 #define HUGKS_E_NOHIP        -7  /* no HIP device visible                    */
+#define HUGKS_E_IO           -8  /* an output file could not be written      */
 
 /* precision of f storage and per-node arithmetic */
 #define HUGKS_FP64           0  /* everything FP64 (parity path)            */
@@ -214,6 +215,27 @@ int hugks_get_macro(hugks_handle *h, double *w, double *q);
  * other handles' streams) race. */
 #define HUGKS_MACRO_NOALLOC_MAX (1 << 20)
 int hugks_get_macro_noalloc(hugks_handle *h, double *w, double *q);
+
+/* macro*.dat output off the step's critical path (write_macro,
+ * 2d/src/output2d.f90:8-56 and 1d/src/output.f90:56-103, replaced: the
+ * caller builds the columns from hugks_get_macro and hands them over).
+ * Copies path, header (written as it is: the VARIABLES and ZONE lines with
+ * their newlines) and nblock blocks of values (block k holds block_len[k]
+ * doubles, consecutive in values) and returns; a writer thread of the
+ * handle then writes the file: the header, then every block as records of
+ * per_line fields in Fortran ES23.16 form with two blanks between fields,
+ * each block starting a new record -- byte for byte what the reference's
+ * "(11(ES23.16,2X))" (2D) and "(9(ES23.16,2X))" (1D) writes make.  Files
+ * are written in call order, while the caller goes on stepping.  With h
+ * NULL the file is written before the call returns.  HUGKS_E_ARG for a bad
+ * argument; HUGKS_E_IO (h NULL) when the file cannot be written. */
+int hugks_write_blocks_async(hugks_handle *h, const char *path, const char *header,
+                             int32_t per_line, int32_t nblock, const int64_t *block_len,
+                             const double *values);
+/* Waits until every file queued on the handle is written; HUGKS_E_IO, with
+ * the file named by hugks_last_error, when one of them could not be written
+ * since the last wait.  hugks_destroy also writes what is still queued. */
+int hugks_write_wait(hugks_handle *h);
 
 /* Distribution functions as FP64, [ncell * nv^dim] each. */
 int hugks_get_f(hugks_handle *h, double *hf, double *bf);
