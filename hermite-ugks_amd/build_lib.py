@@ -5,7 +5,8 @@ Usage: python hermite-ugks_amd/build_lib.py [--jobs N]
 Every translation unit under csrc/ is compiled separately (in parallel)
 and linked into hermite-ugks_amd/libhugks.so.  Packed FP32 instructions
 are disabled for device code.  The build fails when the compiler's
-resource-usage remarks report scratch (private) memory for any kernel.  The Fortran drivers under driver/
+resource-usage remarks report scratch (private) memory, a spilled VGPR or
+all 256 VGPRs in use for any kernel.  The Fortran drivers under driver/
 (ugks2d_gpu, ugks1d_gpu: the reference's main programs over the C-ABI)
 are then built with amdflang and linked against the library.
 """
@@ -45,16 +46,26 @@ def compile_one(src):
 
 
 def scratch_kernels(stderr):
-    """(kernel, bytes per lane) of every kernel whose resource-usage remark
-    reports a private segment (register spills or stack)."""
+    """(kernel, what) of every kernel whose resource-usage remarks report a
+    private segment (register spills or stack), spilled VGPRs, or the
+    whole 256-VGPR file in use (where the compiler starts keeping spilled
+    values in AGPRs: the register picture of a flux kernel that once gave
+    wrong values and a memory fault).  SGPRs spilled to VGPR lanes are
+    allowed: they stay in registers."""
     out, name = [], None
     for ln in stderr.splitlines():
         if "Function Name:" in ln:
             name = ln.split("Function Name:")[1].split()[0]
-        elif "ScratchSize [bytes/lane]:" in ln and name:
-            n = int(ln.split("ScratchSize [bytes/lane]:")[1].split()[0])
-            if n > 0:
-                out.append((name, n))
+            continue
+        if not name:
+            continue
+        for key, limit, what in (("ScratchSize [bytes/lane]:", 0, "bytes/lane of scratch"),
+                                 ("VGPRs Spill:", 0, "VGPRs spilled"),
+                                 (" VGPRs:", 255, "VGPRs (the whole file)")):
+            if key in ln:
+                n = int(ln.split(key)[1].split()[0])
+                if n > limit:
+                    out.append((name, f"{n} {what}"))
     return out
 
 
@@ -81,11 +92,12 @@ def build(jobs=None, verbose=False):
             objs.append(obj)
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
-    # no shipped kernel may use scratch (private) memory: a spill there is
-    # a silent slowdown and a class of fault this library rules out
+    # no shipped kernel may use scratch (private) memory or spill vector
+    # registers: a spill is a silent slowdown and a class of fault this
+    # library rules out
     if spills:
-        raise RuntimeError("kernels with scratch memory:\n" +
-                           "\n".join(f"  {f}: {k} {n} bytes/lane" for f, k, n in spills))
+        raise RuntimeError("kernels that spill or use scratch memory:\n" +
+                           "\n".join(f"  {f}: {k} {n}" for f, k, n in spills))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
