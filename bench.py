@@ -304,9 +304,11 @@ def bench_c4(a):
             b.advance(develop - done)
         out = [s.macro() for s in b.solvers]
         scheds = [s.schedule() for s in b.solvers]
+        waves = [s.update_waves() for s in b.solvers]
         grouped = b.grouped_cases
         b.close()
-        return dict(seconds=t1 - t0, wq=out, w0=w0, kt=kt, scheds=scheds, grouped=grouped)
+        return dict(seconds=t1 - t0, wq=out, w0=w0, kt=kt, scheds=scheds, waves=waves,
+                    grouped=grouped)
 
     develop = 2000     # the channel flows develop: compare there, not near rest
     rm = run_batch(mixed, develop)
@@ -397,6 +399,8 @@ def bench_c4(a):
         "roofline": roof, "cpu_baseline": cpu,
         "schedules": {f"flux{f}_update{u}": n for (f, u), n in Counter(rm["scheds"]).items()},
         "grouped_cases": rm["grouped"],
+        # the autotuner's wavefronts-per-cell tile of the MFMA node update, per case
+        "update_waves": {f"{w} per cell": n for w, n in sorted(Counter(rm["waves"]).items())},
     }
     line.update(out)
     if cpu and "value" in cpu:

@@ -1076,30 +1076,53 @@ int hugks_get_schedule(hugks_handle* h, int32_t* flux_kernel, int32_t* update_ke
 
 }  // extern "C"
 
-// Autotuner: for one case shape, time every kernel schedule on a scratch
-// handle of that shape (a few steps after a warm-up, HIP events on the
-// handle's stream) and keep the fastest.  Results are cached per shape for
-// the life of the process.
+// Autotuner: for one case shape, time every kernel schedule and, for the
+// MFMA node update, every compiled wavefronts-per-cell tile, on a scratch
+// handle of that shape (2 warm-up and 4 timed steps, HIP events on the
+// handle's stream), and keep the fastest.  For a case stepped in grouped
+// launches only the tile of the grouped schedule (split flux, MFMA node
+// update) is searched.  Results are cached per shape for the life of the
+// process.
 namespace {
 struct TuneKey {
-    int dim, nx, ny, nv, nm, precision, bc0, bc2;
+    int dim, nx, ny, nv, nm, precision, bc0, bc2, force, grouped;
     bool operator<(const TuneKey& o) const {
-        return std::tie(dim, nx, ny, nv, nm, precision, bc0, bc2) <
-               std::tie(o.dim, o.nx, o.ny, o.nv, o.nm, o.precision, o.bc0, o.bc2);
+        return std::tie(dim, nx, ny, nv, nm, precision, bc0, bc2, force, grouped) <
+               std::tie(o.dim, o.nx, o.ny, o.nv, o.nm, o.precision, o.bc0, o.bc2, o.force,
+                        o.grouped);
     }
 };
-std::map<TuneKey, std::pair<int, int>> g_tuned;
+struct TuneResult {
+    int flux, update, waves;
+};
+std::map<TuneKey, TuneResult> g_tuned;
 std::mutex g_tune_mu;
+
+// ms per step of the handle's current schedule over 4 steps after 2
+hipError_t time_schedule(hugks_handle* h, float* ms) {
+    hipError_t e = launch_begin(h->L.st, INFINITY, h->stream);
+    for (int s = 0; s < 2 && e == hipSuccess; ++s) e = enqueue_step_on(h, h->stream, false);
+    if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+    for (int s = 0; s < 4 && e == hipSuccess; ++s) e = enqueue_step_on(h, h->stream, false);
+    if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, h->ev0, h->ev1);
+    if (e == hipSuccess) *ms /= 4.f;
+    return e;
+}
 }  // namespace
 
-static int tune_case(const hugks_desc* d, int* flux_kernel, int* update_kernel, double* best_ms) {
-    const TuneKey key{d->dim, d->nx, d->ny, d->nv, d->nm, d->precision, d->bc[0], d->bc[2]};
+static int tune_case(const hugks_desc* d, bool grouped, int* flux_kernel, int* update_kernel,
+                     int* waves, double* best_ms) {
+    const TuneKey key{d->dim, d->nx, d->ny, d->nv, d->nm, d->precision, d->bc[0], d->bc[2],
+                      d->force_variant, grouped ? 1 : 0};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
-            *flux_kernel = it->second.first;
-            *update_kernel = it->second.second;
+            *flux_kernel = it->second.flux;
+            *update_kernel = it->second.update;
+            *waves = it->second.waves;
             if (best_ms) *best_ms = -1.0;
             return HUGKS_OK;
         }
@@ -1107,40 +1130,39 @@ static int tune_case(const hugks_desc* d, int* flux_kernel, int* update_kernel, 
     hugks_handle* h = nullptr;
     int rc = hugks_create(d, &h);
     if (rc != HUGKS_OK) return rc;
-    int bf = h->L.flux_mode, bu = h->L.update_mode;
+    TuneResult r{h->L.flux_mode, h->L.update_mode, h->dim == 2 ? h->L.update_nw : 0};
     double best = 1e300;
     if (d->dim == 2) {
-        for (int fk = HUGKS_FLUX_FUSED; fk <= HUGKS_FLUX_STENCIL; ++fk)
-            for (int uk = HUGKS_UPDATE_LDS; uk <= HUGKS_UPDATE_MFMA_NODE; ++uk) {
+        const int f0 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_FUSED;
+        const int f1 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_STENCIL;
+        const int u0 = grouped ? HUGKS_UPDATE_MFMA_NODE : HUGKS_UPDATE_LDS;
+        for (int fk = f0; fk <= f1; ++fk)
+            for (int uk = u0; uk <= HUGKS_UPDATE_MFMA_NODE; ++uk) {
                 if (hugks_set_schedule(h, fk, uk) != HUGKS_OK) continue;
-                hipError_t e = launch_begin(h->L.st, INFINITY, h->stream);
-                for (int s = 0; s < 2 && e == hipSuccess; ++s) e = enqueue_step_on(h, h->stream, false);
-                if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
-                const int nrep = 4;
-                for (int s = 0; s < nrep && e == hipSuccess; ++s)
-                    e = enqueue_step_on(h, h->stream, false);
-                if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
-                if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
-                float ms = 0.f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
-                if (e != hipSuccess) {
-                    rc = hip_fail(nullptr, e, "autotune");
-                    hugks_destroy(h);
-                    return rc;
-                }
-                if (ms < best) {
-                    best = ms;
-                    bf = fk;
-                    bu = uk;
+                const bool node = uk == HUGKS_UPDATE_MFMA_NODE;
+                for (int nw = node ? 1 : 0; nw <= (node ? 16 : 0); ++nw) {
+                    if (node && hugks_set_update_waves(h, nw) != HUGKS_OK) continue;
+                    float ms = 0.f;
+                    hipError_t e = time_schedule(h, &ms);
+                    if (e != hipSuccess) {
+                        rc = hip_fail(nullptr, e, "autotune");
+                        hugks_destroy(h);
+                        return rc;
+                    }
+                    if (ms < best) {
+                        best = ms;
+                        r = TuneResult{fk, uk, node ? nw : h->L.update_nw};
+                    }
                 }
             }
     }
     hugks_destroy(h);
-    *flux_kernel = bf;
-    *update_kernel = bu;
-    if (best_ms) *best_ms = best < 1e300 ? best / 4 : -1.0;
+    *flux_kernel = r.flux;
+    *update_kernel = r.update;
+    *waves = r.waves;
+    if (best_ms) *best_ms = best < 1e300 ? best : -1.0;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[key] = {bf, bu};
+    g_tuned[key] = r;
     return HUGKS_OK;
 }
 
@@ -1148,13 +1170,20 @@ extern "C" {
 
 int hugks_tune(const hugks_desc* d, int32_t* flux_kernel, int32_t* update_kernel,
                double* ms_per_step) {
-    if (!d || !flux_kernel || !update_kernel)
+    int32_t nw = 0;
+    return hugks_tune_tiles(d, flux_kernel, update_kernel, &nw, ms_per_step);
+}
+
+int hugks_tune_tiles(const hugks_desc* d, int32_t* flux_kernel, int32_t* update_kernel,
+                     int32_t* update_waves, double* ms_per_step) {
+    if (!d || !flux_kernel || !update_kernel || !update_waves)
         return fail(nullptr, HUGKS_E_ARG, "null descriptor or output");
-    int fk = 0, uk = 0;
-    int rc = tune_case(d, &fk, &uk, ms_per_step);
+    int fk = 0, uk = 0, nw = 0;
+    int rc = tune_case(d, false, &fk, &uk, &nw, ms_per_step);
     if (rc != HUGKS_OK) return rc;
     *flux_kernel = fk;
     *update_kernel = uk;
+    *update_waves = nw;
     return HUGKS_OK;
 }
 
@@ -1359,19 +1388,24 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
         }
         b->cases.push_back(h);
     }
-    // grouped cases run their own schedule (split flux, MFMA update); the
-    // tuner picks the schedules of the cases stepped on their own
+    // grouped cases run the grouped schedule (split flux, MFMA node
+    // update), for which the tuner picks the wavefronts per cell; it picks
+    // schedule and tile of the cases stepped on their own
     b->in_group = batch_groupable(b);
     for (int32_t i = 0; i < n && (flags & HUGKS_BATCH_AUTOTUNE); ++i) {
-        if (b->in_group[i]) continue;
-        int fk = -1, uk = -1;
-        int rc = tune_case(&descs[i], &fk, &uk, nullptr);
+        int fk = -1, uk = -1, nw = 0;
+        int rc = tune_case(&descs[i], b->in_group[i] != 0, &fk, &uk, &nw, nullptr);
+        if (rc == HUGKS_OK && b->cases[i]->dim == 2) {
+            rc = hugks_set_schedule(b->cases[i], fk, uk);
+            if (rc == HUGKS_OK && uk == HUGKS_UPDATE_MFMA_NODE)
+                rc = hugks_set_update_waves(b->cases[i], nw);
+            if (rc != HUGKS_OK) g_create_error = b->cases[i]->err;
+        }
         if (rc != HUGKS_OK) {
             std::string m = "case " + std::to_string(i) + ": " + g_create_error;
             batch_release(b);
             return batch_fail(nullptr, rc, m);
         }
-        hugks_set_schedule(b->cases[i], fk, uk);
     }
     // one step of every case, fanned out over a few streams and captured
     // into one graph: a replay advances the whole batch by one step

@@ -70,7 +70,7 @@ EXPORTS = [
     "hugks_checkpoint_write", "hugks_checkpoint_read", "hugks_kernel_times",
     "hugks_reset_kernel_times", "hugks_set_timing", "hugks_device_info", "hugks_debug_fetch",
     "hugks_set_schedule", "hugks_get_schedule", "hugks_set_update_waves",
-    "hugks_get_update_waves", "hugks_tune", "hugks_batch_create",
+    "hugks_get_update_waves", "hugks_tune", "hugks_tune_tiles", "hugks_batch_create",
     "hugks_batch_destroy", "hugks_batch_last_error", "hugks_batch_size", "hugks_batch_grouped",
     "hugks_batch_handle",
     "hugks_batch_advance", "hugks_batch_set_timing", "hugks_batch_kernel_times",
@@ -119,6 +119,7 @@ def load(path=LIBPATH):
     lib.hugks_set_update_waves.argtypes = [H, C.c_int32]
     lib.hugks_get_update_waves.argtypes = [H, I]
     lib.hugks_tune.argtypes = [C.POINTER(Desc), I, I, _dp]
+    lib.hugks_tune_tiles.argtypes = [C.POINTER(Desc), I, I, I, _dp]
     lib.hugks_batch_create.argtypes = [C.c_int32, C.POINTER(Desc), C.c_int32, C.POINTER(H)]
     lib.hugks_batch_destroy.argtypes = [H]
     lib.hugks_batch_last_error.argtypes = [H]
@@ -360,6 +361,18 @@ def tune(case):
     if rc != OK:
         raise HugksError(rc, lib.hugks_last_error(None).decode())
     return f.value, u.value, ms.value
+
+
+def tune_tiles(case):
+    """(flux_kernel, update_kernel, update_waves, ms_per_step) the
+    autotuner picks for the case's shape, tile included."""
+    lib = load()
+    f, u, w, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+    d = case.bind()
+    rc = lib.hugks_tune_tiles(C.byref(d), C.byref(f), C.byref(u), C.byref(w), C.byref(ms))
+    if rc != OK:
+        raise HugksError(rc, lib.hugks_last_error(None).decode())
+    return f.value, u.value, w.value, ms.value
 
 
 class Batch:

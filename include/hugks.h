@@ -297,9 +297,13 @@ int hugks_get_schedule(hugks_handle *h, int32_t *flux_kernel, int32_t *update_ke
 int hugks_set_update_waves(hugks_handle *h, int32_t waves);
 int hugks_get_update_waves(hugks_handle *h, int32_t *waves);
 
-/* Autotuner: times every schedule on a scratch handle of the descriptor's
- * shape and returns the fastest (cached per shape for the process);
- * *ms_per_step (may be NULL) receives its time, or -1 from the cache. */
+/* Autotuner: times every schedule and, for HUGKS_UPDATE_MFMA_NODE, every
+ * wavefronts-per-cell tile with a compiled instance, on a scratch handle
+ * of the descriptor's shape, and returns the fastest (cached per shape for
+ * the process); *ms_per_step (may be NULL) receives its time, or -1 from
+ * the cache.  hugks_tune returns the schedule only. */
+int hugks_tune_tiles(const hugks_desc *d, int32_t *flux_kernel, int32_t *update_kernel,
+                     int32_t *update_waves, double *ms_per_step);
 int hugks_tune(const hugks_desc *d, int32_t *flux_kernel, int32_t *update_kernel,
                double *ms_per_step);
 
@@ -307,9 +311,10 @@ int hugks_tune(const hugks_desc *d, int32_t *flux_kernel, int32_t *update_kernel
  * velocity sizes).  The reference runs each case as its own process; here
  * one step of every case is captured into one HIP graph, whose replay
  * advances the whole batch by a step, each case stopping on its own exit
- * conditions.  With HUGKS_BATCH_AUTOTUNE each case that is not in the
- * grouped launches (hugks_batch_grouped) gets the schedule hugks_tune picks
- * for its shape.  hugks_batch_handle gives the case's
+ * conditions.  With HUGKS_BATCH_AUTOTUNE each case gets the schedule and
+ * tile hugks_tune_tiles picks for its shape; a case in the grouped
+ * launches (hugks_batch_grouped) keeps the grouped schedule and gets the
+ * fastest wavefronts-per-cell tile of it.  hugks_batch_handle gives the case's
  * handle for hugks_get_macro and the other per-case calls (not for
  * hugks_step/advance while the batch is in use). */
 #define HUGKS_BATCH_AUTOTUNE 1

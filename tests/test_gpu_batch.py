@@ -36,7 +36,8 @@ def batch_cases():
     ]
 
 
-@pytest.mark.parametrize("autotune,grouped", [(False, True), (False, False), (True, False)])
+@pytest.mark.parametrize("autotune,grouped", [(False, True), (False, False), (True, False),
+                                              (True, True)])
 def test_batch_equals_cases_stepped_alone(have_gpu, monkeypatch, autotune, grouped):
     """FP64 and FP32 cases in one batch: grouped (one plan per storage type)
     or stepped case by case over lane streams, with or without the tuner."""
@@ -47,11 +48,13 @@ def test_batch_equals_cases_stepped_alone(have_gpu, monkeypatch, autotune, group
     with capi.Batch(cs, autotune=autotune) as b:
         assert b.grouped == grouped
         scheds = [s.schedule() for s in b.solvers]
-        print("schedules", scheds, flush=True)
+        waves = [s.update_waves() for s in b.solvers]
+        print("schedules", scheds, "waves", waves, flush=True)
         singles = []
-        for c, sc in zip(cs, scheds):
+        for c, sc, nw in zip(cs, scheds, waves):
             s = capi.Solver(c)
             s.set_schedule(*sc)
+            s.set_update_waves(nw)
             singles.append(s)
         for rep, n in enumerate((3, 1, 4)):
             infos = b.advance(n)
@@ -206,3 +209,20 @@ def test_small_cases_grouped_beside_a_large_one(have_gpu):
                 assert np.array_equal(wa, wb), (rep, k, np.max(np.abs(wa - wb)))
         for s in singles:
             s.close()
+
+
+def test_tune_tiles_picks_a_compiled_tile(have_gpu):
+    """hugks_tune_tiles times the wavefronts-per-cell tiles of the MFMA node
+    update too; what it returns is a schedule and tile the case accepts, and
+    the second call comes from the cache."""
+    c = cases.build_2d(poiseuille(nv=20, ny=16), capi.MIXED_FP32)
+    fk, uk, nw, ms = capi.tune_tiles(c)
+    print("tuned", fk, uk, nw, ms, flush=True)
+    assert ms > 0
+    with capi.Solver(c) as s:
+        s.set_schedule(fk, uk)
+        if uk == capi.UPDATE_MFMA_NODE:
+            s.set_update_waves(nw)
+            assert s.update_waves() == nw
+    again = capi.tune_tiles(c)
+    assert again[:3] == (fk, uk, nw) and again[3] == -1.0

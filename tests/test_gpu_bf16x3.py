@@ -76,9 +76,11 @@ def test_bf16x3_batch_equals_cases_alone(have_gpu):
         assert all(i.steps == 5 for i in infos)
         wb = [s.macro(with_q=False) for s in b.solvers]
         scheds = [s.schedule() for s in b.solvers]
-    for k, (c, sc) in enumerate(zip(cs, scheds)):
+        waves = [s.update_waves() for s in b.solvers]
+    for k, (c, sc, nw) in enumerate(zip(cs, scheds, waves)):
         with capi.Solver(c) as s:
             s.set_schedule(*sc)
+            s.set_update_waves(nw)
             s.advance(5)
             w = s.macro(with_q=False)
         assert np.array_equal(w, wb[k]), k

@@ -45,9 +45,12 @@ def test_sweep_batch_equals_cases_alone_and_oracle(have_gpu, prec):
         assert all(i.steps == STEPS for i in infos)
         wb = [s.macro(with_q=False) for s in b.solvers]
         scheds = [s.schedule() for s in b.solvers]
-    for k, (c, sc) in enumerate(zip(cs, scheds)):
+        waves = [s.update_waves() for s in b.solvers]
+    print(f"  tuned waves per cell: {sorted(set(waves))}", flush=True)
+    for k, (c, sc, nw) in enumerate(zip(cs, scheds, waves)):
         with capi.Solver(c) as s:
             s.set_schedule(*sc)
+            s.set_update_waves(nw)
             s.advance(STEPS)
             w = s.macro(with_q=False)
         assert np.array_equal(w, wb[k]), k
