@@ -1204,8 +1204,11 @@ struct hugks_batch {
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> joins;
     hipEvent_t fork = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;     // one step of every case
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_k = nullptr;   // graph_steps steps, one after the other
+    hipGraphExec_t exec_k = nullptr;
+    int graph_steps = 1;
     std::string err;
     bool timing = false;   // eager steps with timed grouped launches
     KTimes kt;
@@ -1216,6 +1219,8 @@ static void batch_release(hugks_batch* b) {
     if (b->main) (void)hipStreamSynchronize(b->main);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->exec_k) (void)hipGraphExecDestroy(b->exec_k);
+    if (b->graph_k) (void)hipGraphDestroy(b->graph_k);
     for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (hipStream_t s : b->lanes) (void)hipStreamDestroy(s);
@@ -1439,32 +1444,45 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
     for (int32_t i = 0; i < n; ++i) nsep += b->in_group[i] ? 0 : 1;
     b->grouped = e == hipSuccess && nsep < n;
     if (b->grouped) e = batch_group_plan(b);
-    if (e == hipSuccess) e = hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal);
-    // cases with their own launches on the lanes, forked from the batch
-    // stream; the grouped launches on the batch stream beside them
+    // k steps of every case captured into one graph: per step, the cases
+    // with their own launches on the lanes, forked from the batch stream,
+    // and the grouped launches on the batch stream beside them
     const int nlu = nsep < nl ? nsep : nl;
-    if (nlu > 0) {
-        if (e == hipSuccess) e = hipEventRecord(b->fork, b->main);
-        for (int l = 0; l < nlu && e == hipSuccess; ++l)
-            e = hipStreamWaitEvent(b->lanes[l], b->fork, 0);
-        int q = 0;
-        for (int32_t i = 0; i < n && e == hipSuccess; ++i)
-            if (!b->in_group[i]) e = enqueue_step_on(b->cases[i], b->lanes[q++ % nlu], false);
-    }
-    for (size_t q = 0; q < b->plans.size() && e == hipSuccess; ++q)
-        e = launch_group_step(b->plans[q], b->main);
-    for (int l = 0; l < nlu && e == hipSuccess; ++l) {
-        e = hipEventRecord(b->joins[l], b->lanes[l]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->joins[l], 0);
-    }
-    if (e == hipSuccess) {
-        e = hipStreamEndCapture(b->main, &b->graph);
-    } else {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(b->main, &g);
-        if (g) (void)hipGraphDestroy(g);
-    }
-    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    auto capture = [&](int k, hipGraph_t* graph, hipGraphExec_t* exec) {
+        hipError_t r = hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal);
+        for (int step = 0; step < k && r == hipSuccess; ++step) {
+            if (nlu > 0) {
+                r = hipEventRecord(b->fork, b->main);
+                for (int l = 0; l < nlu && r == hipSuccess; ++l)
+                    r = hipStreamWaitEvent(b->lanes[l], b->fork, 0);
+                int q = 0;
+                for (int32_t i = 0; i < n && r == hipSuccess; ++i)
+                    if (!b->in_group[i]) r = enqueue_step_on(b->cases[i], b->lanes[q++ % nlu], false);
+            }
+            for (size_t q = 0; q < b->plans.size() && r == hipSuccess; ++q)
+                r = launch_group_step(b->plans[q], b->main);
+            for (int l = 0; l < nlu && r == hipSuccess; ++l) {
+                r = hipEventRecord(b->joins[l], b->lanes[l]);
+                if (r == hipSuccess) r = hipStreamWaitEvent(b->main, b->joins[l], 0);
+            }
+        }
+        if (r == hipSuccess) {
+            r = hipStreamEndCapture(b->main, graph);
+        } else {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(b->main, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (r == hipSuccess) r = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
+        return r;
+    };
+    // the multi-step graph: a case that stops partway through turns the
+    // rest of its launches into no-ops (its device state's stop flag)
+    b->graph_steps = 8;
+    if (const char* gs = getenv("HUGKS_BATCH_GRAPH_STEPS"))
+        b->graph_steps = atoi(gs) > 0 ? atoi(gs) : b->graph_steps;
+    if (e == hipSuccess) e = capture(1, &b->graph, &b->exec);
+    if (e == hipSuccess && b->graph_steps > 1) e = capture(b->graph_steps, &b->graph_k, &b->exec_k);
     if (e != hipSuccess) {
         std::string m = std::string("batch graph: ") + hipGetErrorString(e);
         batch_release(b);
@@ -1516,6 +1534,17 @@ static hipError_t batch_step(hugks_batch* b) {
     return e;
 }
 
+// n steps of every case: replays of the multi-step graph, then of the
+// one-step graph for the remainder
+static hipError_t batch_steps(hugks_batch* b, int n) {
+    hipError_t e = hipSuccess;
+    if (!b->timing && b->exec_k)
+        for (; n >= b->graph_steps && e == hipSuccess; n -= b->graph_steps)
+            e = hipGraphLaunch(b->exec_k, b->main);
+    for (; n > 0 && e == hipSuccess; --n) e = batch_step(b);
+    return e;
+}
+
 int hugks_batch_set_timing(hugks_batch* b, int32_t on) {
     if (!b) return batch_fail(nullptr, HUGKS_E_ARG, "null batch");
     b->timing = on != 0;
@@ -1549,11 +1578,11 @@ int hugks_batch_advance(hugks_batch* b, int32_t max_steps, hugks_step_info* info
     };
     hipError_t e = launch_begin_all(b->d_sts, n, INFINITY, b->main);
     if (max_steps > 0) {
-        for (int32_t s = 0; s < max_steps && e == hipSuccess; ++s) e = batch_step(b);
+        e = batch_steps(b, max_steps);
     } else {
         // no cap: replay until every case met an exit condition
         for (;;) {
-            for (int s = 0; s < 32 && e == hipSuccess; ++s) e = batch_step(b);
+            e = batch_steps(b, 32);
             if (e == hipSuccess) e = fetch();
             if (e != hipSuccess) break;
             bool all = true;

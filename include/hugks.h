@@ -309,9 +309,11 @@ int hugks_tune(const hugks_desc *d, int32_t *flux_kernel, int32_t *update_kernel
 
 /* Batches of independent cases (a Kn / force sweep of uneven grid and
  * velocity sizes).  The reference runs each case as its own process; here
- * one step of every case is captured into one HIP graph, whose replay
- * advances the whole batch by a step, each case stopping on its own exit
- * conditions.  With HUGKS_BATCH_AUTOTUNE each case gets the schedule and
+ * one step of every case, and eight steps one after the other, are
+ * captured into two HIP graphs (HUGKS_BATCH_GRAPH_STEPS sets the eight);
+ * an advance replays the multi-step graph, then the one-step graph for the
+ * remainder, each case stopping on its own exit conditions (the launches
+ * of a stopped case do nothing).  With HUGKS_BATCH_AUTOTUNE each case gets the schedule and
  * tile hugks_tune_tiles picks for its shape; a case in the grouped
  * launches (hugks_batch_grouped) keeps the grouped schedule and gets the
  * fastest wavefronts-per-cell tile of it.  hugks_batch_handle gives the case's

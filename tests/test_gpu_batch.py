@@ -56,7 +56,8 @@ def test_batch_equals_cases_stepped_alone(have_gpu, monkeypatch, autotune, group
             s.set_schedule(*sc)
             s.set_update_waves(nw)
             singles.append(s)
-        for rep, n in enumerate((3, 1, 4)):
+        # 9 and 17 steps: replays of the eight-step graph and a remainder
+        for rep, n in enumerate((3, 1, 9, 17)):
             infos = b.advance(n)
             assert [i.steps for i in infos] == [n] * len(cs)
             for k, s in enumerate(singles):
