@@ -1,9 +1,11 @@
 """Benchmark of the Hermite-UGKS time step on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|c2]
-                [--no-cpu] [--no-fp64]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|c2|c4]
+                [--no-cpu] [--no-fp64] [--no-bf16x3]
 python bench.py --copy-probe                  (calibration copy for counters)
 python bench.py --rocprof-summary DIR [DIR..] (summarise rocprofv3 CSV output)
+python bench.py --traffic-summary WORKLOAD FETCH WRITE COPY_FETCH COPY_WRITE TRACE
+                                              (profiles/traffic_<workload>.json)
 
 A step is one full reference time step (timestep, interpolation,
 evolution, update) over the whole phase space of the workload.  The
@@ -248,6 +250,60 @@ def rocprof_summary(dirs):
     print(json.dumps(out, indent=1, sort_keys=True))
 
 
+# rocprofv3 kernel names of the 2D step -> the library's timer names
+TIMER_OF = [("k_update2d_mx", "update2d"), ("k_update2d_mfma", "update2d"),
+            ("k_update2d<", "update2d"), ("k_fsum", "flux_sum"), ("k_fnode", "flux_node"),
+            ("k_fface", "flux_face"), ("k_slope2d", "slope2d"), ("k_flux_wall", "flux_wall"),
+            ("k_cell_pre", "cell_pre"), ("k_finalize", "finalize"),
+            ("k_part_sum", "finalize_partials"), ("k_rate_part", "timestep_rates"),
+            ("k_timestep", "timestep")]
+
+
+def traffic_summary(workload, dirs):
+    """profiles/traffic_<workload>.json content from rocprofv3 CSV output:
+    dirs = FETCH_SIZE run, WRITE_SIZE run, the copy probe's FETCH_SIZE and
+    WRITE_SIZE runs, and a kernel trace of the same command.  Each counter
+    is scaled by what it reads over the 1 GiB copy (bytes read and written
+    known), then set against the kernel's algorithmic bytes."""
+    import contextlib
+    import io
+
+    def summ(d):
+        buf = io.StringIO()
+        with contextlib.redirect_stdout(buf):
+            rocprof_summary([d])
+        return json.loads(buf.getvalue())
+    fetch, write, cf, cw, trace = (summ(d) for d in dirs)
+    copy = "__amd_rocclr_copyBuffer"
+    big = lambda s, key: max(v[key] for k, v in s.items() if k.startswith(copy) and key in v)
+    gib = float(1 << 30)
+    fk, wk = big(cf, "FETCH_SIZE_avg"), big(cw, "WRITE_SIZE_avg")
+    cal = {"copy_bytes_read": gib, "copy_bytes_written": gib, "FETCH_SIZE_kib": fk,
+           "WRITE_SIZE_kib": wk, "fetch_scale": gib / (fk * 1024), "write_scale": gib / (wk * 1024)}
+    case = cases.build_2d(WORKLOADS[workload], capi.MIXED_FP32, rt_variant=VARIANT[workload],
+                          seed=1)
+    kern = {}
+    for name, v in fetch.items():
+        timer = next((t for pat, t in TIMER_OF if pat in name), None)
+        if timer is None or "FETCH_SIZE_avg" not in v or name not in write:
+            continue
+        fb = v["FETCH_SIZE_avg"] * 1024 * cal["fetch_scale"]
+        wb = write[name]["WRITE_SIZE_avg"] * 1024 * cal["write_scale"]
+        ab = algorithmic_bytes(case, timer)
+        ms = trace.get(name, {}).get("avg_ms")
+        kern[timer] = {"rocprof_name": name, "FETCH_SIZE_kib": v["FETCH_SIZE_avg"],
+                       "WRITE_SIZE_kib": write[name]["WRITE_SIZE_avg"],
+                       "dispatches": v["FETCH_SIZE_dispatches"],
+                       "hbm_bytes_per_launch": fb + wb, "fetch_bytes": fb, "write_bytes": wb,
+                       "algorithmic_bytes_per_launch": ab,
+                       "traffic_over_algorithmic": (fb + wb) / ab if ab else None,
+                       "avg_ms_trace": ms,
+                       "traffic_GBs": (fb + wb) / (ms * 1e-3) / 1e9 if ms else None,
+                       "algorithmic_GBs": ab / (ms * 1e-3) / 1e9 if ms and ab else None}
+    print(json.dumps({"workload": NAMES[workload], "precision": "mixed", "calibration": cal,
+                      "kernels": kern}, indent=1, sort_keys=True))
+
+
 def traffic_from_profiles(workload, kernel):
     """HBM bytes per launch of kernel from the committed counter summary
     profiles/traffic_<workload>.json (FETCH_SIZE + WRITE_SIZE, KiB per
@@ -413,6 +469,9 @@ def main():
         return copy_probe()
     if "--rocprof-summary" in sys.argv:
         return rocprof_summary(sys.argv[sys.argv.index("--rocprof-summary") + 1:])
+    if "--traffic-summary" in sys.argv:
+        i = sys.argv.index("--traffic-summary")
+        return traffic_summary(sys.argv[i + 1], sys.argv[i + 2:i + 7])
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -420,6 +479,7 @@ def main():
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS) + ["c4"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-bf16x3", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -458,6 +518,20 @@ def main():
             "note": "velocities against the largest speed, q against the largest |q|; "
                     "energy is not conserved under gravity (work of the body force)",
         }
+    # the bf16x3 split-product Hermite contraction: the same step with the
+    # update's tiles as bf16 MFMA products of operands split in three
+    if not a.no_bf16x3:
+        rb = run_gpu(mixed.with_precision(capi.MIXED_BF16X3), a.warmup, a.steps)
+        from metrics import prim_errors
+        b3 = {"value": N * a.steps / rb["seconds"], "ms_per_step": 1e3 * rb["seconds"] / a.steps,
+              "update2d_ms": (rb["kernels"]["update2d"][0] / rb["kernels"]["update2d"][1]
+                              if "update2d" in rb["kernels"] else None)}
+        pb = prim_errors(rb["w"], rm["w"], mixed.desc.gamma)
+        b3["rel_err_vs_fp32_tiles"] = {"rho": pb[0], "U": pb[1], "V": pb[2], "T": pb[3]}
+        if not a.no_fp64:
+            pf = prim_errors(rb["w"], rf["w"], mixed.desc.gamma)
+            b3["rel_err_vs_fp64"] = {"rho": pf[0], "U": pf[1], "V": pf[2], "T": pf[3]}
+        out["bf16x3"] = b3
     # dominant kernel of the headline (mixed) path
     kt = rm["kernels"]
     dom = max(kt, key=lambda k: kt[k][0])

@@ -5,7 +5,7 @@
 !
 ! The input is the reference's &params namelist (2d/src/config.f90:47-48,
 ! same names and defaults) optionally followed by a &gpu group
-! (precision = 'fp64' | 'mixed'; max_steps, 0 = no cap; rt_variant =
+! (precision = 'fp64' | 'mixed' | 'bf16x3'; max_steps, 0 = no cap; rt_variant =
 ! 'reference' | 'cartesian'; seed; restart_file, checkpoint_file: continue
 ! from / save the whole solver state, hugks_checkpoint_read/_write).  The driver keeps what the reference
 ! main program does around the step - case selection, velocity
