@@ -26,7 +26,8 @@ from test_oracle_golden import case_of, load_fixture
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NC = sorted(glob.glob(os.path.join(GOLDEN, "*_nc*.npz")))
+# (the finite-difference fixtures on Newton-Cotes grids are test_gpu_finite_difference.py's)
+NC = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*_nc*.npz")) if "_fd_" not in p)
 NC2D = [p for p in NC if os.path.basename(p).startswith("p2d")]
 SCHED = [(f, u) for f in (capi.FLUX_FUSED, capi.FLUX_SPLIT, capi.FLUX_STENCIL)
          for u in (capi.UPDATE_LDS, capi.UPDATE_MFMA, capi.UPDATE_MFMA_NODE)]
