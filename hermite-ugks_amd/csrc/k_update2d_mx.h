@@ -30,11 +30,15 @@
 //   qx = (rt^{3/2} (C30 + C12 + 4 C10) + rt^{1/2} C10[b]) / 2.
 //
 // Node (iu, jv) of an F^T tile sits in the accumulator element of the lane
-// that loaded it: f^n and b^n are read once, at the start, in that layout
-// and kept in registers; the eight face fluxes of a tile position are read
-// before its tiles (the first position's during the R contraction), and
-// h^{n+1}, b^{n+1} are written from the accumulators.  One workgroup of NW wavefronts per cell; wavefront w owns
-// the tile positions w, w + NW, ... (MAXP at most).
+// that holds it.  With 4-byte operands f^n and b^n are read once, at the
+// start, in that layout and kept in registers (G = w f staged in LDS from
+// them); with 8-byte ones G is formed in registers for each K step of Tm
+// and f^n, b^n of a tile position are read again (from L2) with its face
+// fluxes.  The eight face fluxes of a tile position are read before its
+// tiles (the first position's during the R contraction), and h^{n+1},
+// b^{n+1} are written from the accumulators.  One workgroup of NW
+// wavefronts per cell; wavefront w owns the tile positions w, w + NW, ...
+// (MAXP at most).
 #pragma once
 #include "k_update2d_mfma.h"
 
@@ -46,19 +50,27 @@ static __constant__ const double kInvFact[MF_NB] = {
     1.0 / 40320.0, 1.0 / 362880.0, 1.0 / 3628800.0, 1.0 / 39916800.0, 1.0 / 479001600.0,
     1.0 / 6227020800.0, 1.0 / 87178291200.0, 1.0 / 1307674368000.0};
 
-// LDS layout of one cell (elements of T).  Row strides as in k_update2d_mfma.h:
-// a tile operand read as 16 rows x 4 columns needs 4 x odd (Hx, Hy, Tm, D,
-// G), one read as 4 rows x 16 columns an odd multiple of 16 (R, Rc).
-//   region 0: G [2][nv][ldG] (weight * f), later D [2][16][sD], L [2][8][sH]
-//             and Rc [8][sR]
+// LDS layout of one cell (elements of the tile operand type).  Row strides
+// as in k_update2d_mfma.h: a tile operand read as 16 rows x 4 columns needs
+// 4 x odd (Hx, Hy, Tm, D, G), one read as 4 rows x 16 columns an odd
+// multiple of 16 (R, Rc).
+//   region 0: G [2][nv][ldG] (weight * f; staged only for 4-byte operands),
+//             later D [2][16][sD], L [2][8][sH] and Rc [8][sR]
 //   Hx, Ho, Hy [16][sH]
 //   region T: Tm [2][16][sH], later R [2][16][sR]
 //   OMy [nv]
+// G, the other operand of Tm = Hx G, is staged for 4-byte operands (FP32
+// tiles): five cells of 31 KB per CU at nv = 40.  With 8-byte operands
+// (FP64, bf16x3 pieces) LDS bounds the cells per CU, so G is formed in
+// registers from global memory instead: 48 instead of 62 KB per cell at
+// nv = 40, three cells per CU instead of two (FP64 9.40 -> 8.55 ms, bf16x3
+// 8.27 -> 6.93 ms at 512^2 cells, GH40; FP32 that way: 4.41 -> 5.85 ms,
+// its twenty scattered loads per tile job outweigh the occupancy).
 struct MxLayout {
     int np, sH, sR, sD, ldG;
     int oD, oL, oRc, oHx, oHo, oHy, oT, oOM, end;
 };
-__host__ __device__ inline MxLayout mx_layout(int nv) {
+__host__ __device__ inline MxLayout mx_layout(int nv, bool stage_g) {
     MxLayout L;
     L.np = mf_np(nv);
     L.sH = L.np + 4;
@@ -70,7 +82,7 @@ __host__ __device__ inline MxLayout mx_layout(int nv) {
     L.oL = L.oD + 2 * MF_NB * L.sD;
     L.oRc = L.oL + 2 * 8 * L.sH;
     const int r0 = L.oRc + 8 * L.sR;
-    const int g = 2 * nv * L.ldG;
+    const int g = stage_g ? 2 * nv * L.ldG : 0;
     L.oHx = g > r0 ? g : r0;
     L.oHo = L.oHx + MF_NB * L.sH;
     L.oHy = L.oHo + MF_NB * L.sH;
@@ -79,8 +91,9 @@ __host__ __device__ inline MxLayout mx_layout(int nv) {
     L.end = L.oOM + nv;
     return L;
 }
+// G is staged exactly when the operand element is 4 bytes
 __host__ __device__ inline size_t mx_lds_bytes(int nv, size_t tsize) {
-    return (size_t)mx_layout(nv).end * tsize;
+    return (size_t)mx_layout(nv, tsize == 4).end * tsize;
 }
 
 // Compiled instances (wavefronts per cell, tile positions per wavefront,
@@ -218,10 +231,11 @@ __device__ __forceinline__ void k_update2d_mx_body(
     const double rt = P[CP_RT];
     const double sqrt_rt = sqrt(rt);
 
-    const MxLayout ly = mx_layout(nv);
+    constexpr bool STAGE_G = sizeof(E) == 4;
+    const MxLayout ly = mx_layout(nv, STAGE_G);
     const int sH = ly.sH, sR = ly.sR, sD = ly.sD, ldG = ly.ldG, GS = nv * ly.ldG;
     E* const sm = reinterpret_cast<E*>(smem_raw);
-    E* const G = sm;
+    E* const G = sm;               // staged G (4-byte operands)
     E* const D = sm + ly.oD;       // G is dead once Tm is formed
     E* const Lr = sm + ly.oL;
     E* const Rc = sm + ly.oRc;
@@ -231,6 +245,55 @@ __device__ __forceinline__ void k_update2d_mx_body(
     E* const Tm = sm + ly.oT;
     E* const R = Tm;               // Tm is dead once D is formed
     A* const OMy = reinterpret_cast<A*>(sm + ly.oOM);
+
+    // ---- node offsets.  Loads and stores are raw buffer accesses on one
+    // resource per array row with one 32-bit byte offset per node, shared
+    // by all the arrays: a node outside the grid gets an offset past the
+    // row, which reads zero and drops a store, so no access sits under a
+    // branch (a load under a branch is waited for inside it, which
+    // serialises the loads).  The lane's nodes of tile position pi are
+    // those of the F^T accumulators: iu = 16 it + lane % 16, jv = 16 jt +
+    // row(lane, v).
+    const int vl = i + (nx + 1) * j, vr = vl + 1;
+    const int hd = i + nx * j, hu = i + nx * (j + 1);
+    const size_t ldv = (size_t)c.ldv;
+    const uint32_t rowb = (uint32_t)(nv * nv * (int)sizeof(T));
+    constexpr uint32_t OUT = 0x80000000u;   // past any row
+    const __amdgpu_buffer_rsrc_t rsh = buf_rsrc(h + (size_t)cell * ldv, rowb);
+    const __amdgpu_buffer_rsrc_t rsb = buf_rsrc(b + (size_t)cell * ldv, rowb);
+    const __amdgpu_buffer_rsrc_t rsw = buf_rsrc(wnode, rowb);
+    const int lc = lane & 15;
+    auto node_off = [&](int pi, int v) -> uint32_t {
+        const int p = wv + NW * pi;
+        const int iu = 16 * (p % nt) + lc, jv = 16 * (p / nt) + M::row(lane, v);
+        return (p < npos && iu < nv && jv < nv) ? (uint32_t)((iu + nv * jv) * (int)sizeof(T))
+                                                : OUT;
+    };
+
+    // ---- 8-byte operands: the B operand of Tm = Hx G, G = w f, for job
+    // (f, jt): node (iu = 4 ks + lk, jv = 16 jt + lr) of K step ks, loaded
+    // and weighted in registers (zero outside the grid).  The wavefront's
+    // first job is requested before the bases are built, so its loads are
+    // in flight meanwhile.
+    const int ksn = (nv + 3) / 4;       // K steps over a velocity index
+    constexpr int KSN = STAGE_G ? 1 : NVC ? (NVC + 3) / 4 : (48 + 3) / 4;
+    const int lr = lane & 15, lk = lane >> 4;
+    A gf[KSN], gw[KSN];
+    auto load_g = [&](int job) {
+        const int f = job / nt, jt = job % nt;
+        const int jv = 16 * jt + lr;
+        const __amdgpu_buffer_rsrc_t rs = f ? rsb : rsh;
+#pragma unroll
+        for (int ks = 0; ks < KSN; ++ks) {
+            const int iu = 4 * ks + lk;
+            const bool in = ks < ksn && iu < nv && jv < nv;
+            const uint32_t o = in ? (uint32_t)((iu + nv * jv) * (int)sizeof(T)) : OUT;
+            gf[ks] = (A)buf_ld<T>(rs, o);
+            gw[ks] = buf_ld<A>(rsw, in ? o / (uint32_t)sizeof(T) * (uint32_t)sizeof(A) : OUT);
+        }
+    };
+    if constexpr (!STAGE_G)
+        if (wv < 2 * nt) load_g(wv);
 
     // ---- Hermite bases He_0..He_15 at xi = (u-U)/sqrt(rt), eta = (v-V)/sqrt(rt)
     // (build_probabilists_hermite, hermite_symmetric2d.f90:220-233); Ho
@@ -273,54 +336,37 @@ __device__ __forceinline__ void k_update2d_mx_body(
         }
     }
 
-    // ---- f^n and b^n of the lane's nodes, in the layout of the F^T
-    // accumulators (iu = 16 it + lane % 16, jv = 16 jt + row(lane, v)), kept
-    // in registers for the update; weight * f to LDS for the coefficients.
-    // Loads and stores are raw buffer accesses on one resource per array
-    // row with one 32-bit byte offset per node, shared by all the arrays: a
-    // node outside the grid gets an offset past the row, which reads zero
-    // and drops a store, so no access sits under a branch (a load under a
-    // branch is waited for inside it, which serialises the loads).
-    const int vl = i + (nx + 1) * j, vr = vl + 1;
-    const int hd = i + nx * j, hu = i + nx * (j + 1);
-    const size_t ldv = (size_t)c.ldv;
-    const uint32_t rowb = (uint32_t)(nv * nv * (int)sizeof(T));
-    constexpr uint32_t OUT = 0x80000000u;   // past any row
-    const __amdgpu_buffer_rsrc_t rsh = buf_rsrc(h + (size_t)cell * ldv, rowb);
-    const __amdgpu_buffer_rsrc_t rsb = buf_rsrc(b + (size_t)cell * ldv, rowb);
-    const __amdgpu_buffer_rsrc_t rsw = buf_rsrc(wnode, rowb);
-    const int lc = lane & 15;
-    auto node_off = [&](int pi, int v) -> uint32_t {
-        const int p = wv + NW * pi;
-        const int iu = 16 * (p % nt) + lc, jv = 16 * (p / nt) + M::row(lane, v);
-        return (p < npos && iu < nv && jv < nv) ? (uint32_t)((iu + nv * jv) * (int)sizeof(T))
-                                                : OUT;
-    };
-    A rh[MAXP][4], rb[MAXP][4];
+    // ---- 4-byte operands: f^n and b^n of the lane's nodes, in the layout
+    // of the F^T accumulators, kept in registers for the update; weight * f
+    // to LDS for the coefficients
+    A rhs[STAGE_G ? MAXP : 1][4], rbs[STAGE_G ? MAXP : 1][4];
+    if constexpr (STAGE_G) {
 #pragma unroll
-    for (int pi = 0; pi < MAXP; ++pi) {
+        for (int pi = 0; pi < MAXP; ++pi) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const uint32_t o = node_off(pi, v);
-            rh[pi][v] = (A)buf_ld<T>(rsh, o);
-            rb[pi][v] = (A)buf_ld<T>(rsb, o);
+            for (int v = 0; v < 4; ++v) {
+                const uint32_t o = node_off(pi, v);
+                rhs[pi][v] = (A)buf_ld<T>(rsh, o);
+                rbs[pi][v] = (A)buf_ld<T>(rsb, o);
+            }
         }
-    }
 #pragma unroll
-    for (int pi = 0; pi < MAXP; ++pi) {
-        const int p = wv + NW * pi;
-        const int iu = 16 * (p % nt) + lc;
+        for (int pi = 0; pi < MAXP; ++pi) {
+            const int p = wv + NW * pi;
+            const int iu = 16 * (p % nt) + lc;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const uint32_t o = node_off(pi, v);
-            const int jv = 16 * (p / nt) + M::row(lane, v);
-            if (o != OUT) {
-                const A wa = buf_ld<A>(rsw, o / (uint32_t)sizeof(T) * (uint32_t)sizeof(A));
-                G[jv * ldG + iu] = O::put(wa * rh[pi][v]);
-                G[GS + jv * ldG + iu] = O::put(wa * rb[pi][v]);
+            for (int v = 0; v < 4; ++v) {
+                const uint32_t o = node_off(pi, v);
+                const int jv = 16 * (p / nt) + M::row(lane, v);
+                if (o != OUT) {
+                    const A wa = buf_ld<A>(rsw, o / (uint32_t)sizeof(T) * (uint32_t)sizeof(A));
+                    G[jv * ldG + iu] = O::put(wa * rhs[pi][v]);
+                    G[GS + jv * ldG + iu] = O::put(wa * rbs[pi][v]);
+                }
             }
         }
     }
+
     // the eight face fluxes of one tile position's nodes (left/right,
     // down/up faces of h, then of b)
     const __amdgpu_buffer_rsrc_t rsf[8] = {
@@ -328,33 +374,49 @@ __device__ __forceinline__ void k_update2d_mx_body(
         buf_rsrc(hfh + (size_t)hd * ldv, rowb), buf_rsrc(hfh + (size_t)hu * ldv, rowb),
         buf_rsrc(vfb + (size_t)vl * ldv, rowb), buf_rsrc(vfb + (size_t)vr * ldv, rowb),
         buf_rsrc(hfb + (size_t)hd * ldv, rowb), buf_rsrc(hfb + (size_t)hu * ldv, rowb)};
-    auto load_faces = [&](int pi, A (&fvo)[8][4]) {
+    // one tile position's face fluxes and, for 8-byte operands, f^n, b^n
+    // of its nodes (read a second time: the cell's rows are in L2 from the
+    // Tm contraction)
+    auto load_faces = [&](int pi, A (&fvo)[8][4], A (&rho)[4], A (&rbo)[4]) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const uint32_t o = node_off(pi, v);
 #pragma unroll
             for (int a = 0; a < 8; ++a) fvo[a][v] = (A)buf_ld<T>(rsf[a], o);
+            if constexpr (!STAGE_G) {
+                rho[v] = (A)buf_ld<T>(rsh, o);
+                rbo[v] = (A)buf_ld<T>(rsb, o);
+            }
         }
     };
-    A fv[8][4];
-    if constexpr (MAXP == 1) load_faces(0, fv);
+    A fv[8][4], rh[4], rb[4];
+    if constexpr (STAGE_G && MAXP == 1) load_faces(0, fv, rh, rb);
     __syncthreads();
-
-    const int ksn = (nv + 3) / 4;       // K steps over a velocity index
-    const int lr = lane & 15, lk = lane >> 4;
 
     // ---- Tm = Hx G: tile (f, jt), rows r, columns jv
     for (int job = wv; job < 2 * nt; job += NW) {
         const int f = job / nt, jt = job % nt;
         const int jv = 16 * jt + lr;
         V acc = {0, 0, 0, 0};
+        if constexpr (STAGE_G) {
 #pragma unroll
-        for (int ks = 0; ks < ksn; ++ks) {
-            const int iu = 4 * ks + lk;
-            const bool in = iu < nv;
-            const E a = in ? Hx[lr * sH + iu] : O::zero();
-            const E bb = (in && jv < nv) ? G[f * GS + jv * ldG + iu] : O::zero();
-            acc = M::mma(a, bb, acc);
+            for (int ks = 0; ks < ksn; ++ks) {
+                const int iu = 4 * ks + lk;
+                const bool in = iu < nv;
+                const E a = in ? Hx[lr * sH + iu] : O::zero();
+                const E bb = (in && jv < nv) ? G[f * GS + jv * ldG + iu] : O::zero();
+                acc = M::mma(a, bb, acc);
+            }
+        } else {
+            if (job != wv) load_g(job);
+#pragma unroll
+            for (int ks = 0; ks < KSN; ++ks) {
+                if (ks < ksn) {
+                    const int iu = 4 * ks + lk;
+                    const E a = iu < nv ? Hx[lr * sH + iu] : O::zero();
+                    acc = M::mma(a, O::put(gw[ks] * gf[ks]), acc);
+                }
+            }
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) Tm[(f * MF_NB + M::row(lane, v)) * sH + jv] = O::put(acc[v]);
@@ -452,9 +514,10 @@ __device__ __forceinline__ void k_update2d_mx_body(
             }
         }
     }
-    // the first tile position's face fluxes in flight during the R
-    // contraction; with one position per wavefront, from the start
-    if constexpr (MAXP > 1) load_faces(0, fv);
+    // the first tile position's face fluxes (and f^n) in flight during the
+    // R contraction; with one position per wavefront and staged G, from
+    // the start
+    if constexpr (!(STAGE_G && MAXP == 1)) load_faces(0, fv, rh, rb);
     // ---- R(r, jv) = (ax sum_s D(r-1, s) Hy(s, jv) + ay sum_s D(r, s) Hy(s+1, jv)) OMy(jv)
     {
         const A axA = (A)P[CP_AX], ayA = (A)P[CP_AY];
@@ -504,7 +567,7 @@ __device__ __forceinline__ void k_update2d_mx_body(
     for (int pi = 0; pi < MAXP; ++pi) {
         const int p = wv + NW * pi;
         if (p >= npos) continue;
-        if (pi > 0) load_faces(pi, fv);
+        if (pi > 0) load_faces(pi, fv, rh, rb);
         const int it = p % nt, jt = p / nt;
         const int iu = 16 * it + lc, jvb = 16 * jt;
         V acc[2];
@@ -528,9 +591,11 @@ __device__ __forceinline__ void k_update2d_mx_body(
             const uint32_t o = node_off(pi, v);
             const A dh = fv[0][v] - fv[1][v] + fv[2][v] - fv[3][v];
             const A db = fv[4][v] - fv[5][v] + fv[6][v] - fv[7][v];
-            const double hn = fma(c1, (double)mx_opaque(rh[pi][v]),
+            const A ho = STAGE_G ? rhs[STAGE_G ? pi : 0][v] : rh[v];
+            const A bo = STAGE_G ? rbs[STAGE_G ? pi : 0][v] : rb[v];
+            const double hn = fma(c1, (double)mx_opaque(ho),
                                   fma(cd, (double)dh, (double)acc[0][v]));
-            const double bn = fma(c1, (double)mx_opaque(rb[pi][v]),
+            const double bn = fma(c1, (double)mx_opaque(bo),
                                   fma(cd, (double)db, (double)acc[1][v]));
             // streamed out (non-temporal): the next step reads them long after
             buf_st_nt<T>((T)hn, rsh, o);
