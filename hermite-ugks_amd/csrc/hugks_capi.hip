@@ -518,11 +518,15 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         const size_t nint = (size_t)(xper ? d->nx + 1 : d->nx - 1) * d->ny +
                             (size_t)d->nx * (yper ? d->ny + 1 : d->ny - 1);
         L.stencil_ok = fsz < ((size_t)1 << 32);
-        L.flux_mode = HUGKS_FLUX_SPLIT;
+        // default: the cell-sum pipeline where it applies (set below), else
+        // the split one
+        L.flux_mode = HUGKS_FLUX_CELLSUM;
         bool maxw_nodes = false;
         if (const char* fm = getenv("HUGKS_FLUX_KERNEL")) {
             if (strcmp(fm, "fused") == 0) L.flux_mode = HUGKS_FLUX_FUSED;
+            if (strcmp(fm, "split") == 0) L.flux_mode = HUGKS_FLUX_SPLIT;
             if (strcmp(fm, "stencil") == 0 && L.stencil_ok) L.flux_mode = HUGKS_FLUX_STENCIL;
+            if (strcmp(fm, "cellsum") == 0) L.flux_mode = HUGKS_FLUX_CELLSUM;
             if (strcmp(fm, "split-nodes") == 0) {
                 L.flux_mode = HUGKS_FLUX_SPLIT;
                 maxw_nodes = true;
@@ -559,6 +563,22 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
             }
         }
         ALLOC(L.fsum, (nint ? nint : 1) * FACE_SUM_LD * sizeof(double));
+        // per-cell half-space sums (HUGKS_FLUX_CELLSUM): one lane per u
+        // column (nv <= 64), the v nodes split as rows [0, js) negative,
+        // [js, nv) non-negative (upwind_delta's sign test), and weights
+        // that factor (w = wx wy)
+        L.cs_js = -1;
+        if (d->nv <= 64 && L.wx) {
+            int js = 0;
+            while (js < d->nv && std::signbit(d->v[js])) ++js;
+            bool split = true;
+            for (int j = js; j < d->nv; ++j) split = split && !std::signbit(d->v[j]);
+            if (split) {
+                L.cs_js = js;
+                ALLOC(L.csum, ncell * 4 * CELL_SUM_LD * sizeof(double));
+            }
+        }
+        if (L.flux_mode == HUGKS_FLUX_CELLSUM && L.cs_js < 0) L.flux_mode = HUGKS_FLUX_SPLIT;
         ALLOC(L.fcoef, (nint ? nint : 1) * FACE_COEF_LD * sizeof(double));
         ALLOC(L.fmaxw, (nint ? nint : 1) * FACE_MAXW_LD * sizeof(double));
         ALLOC(L.cellpre, ncell * 24 * sizeof(double));
@@ -1026,7 +1046,7 @@ int hugks_set_timing(hugks_handle* h, int32_t on) {
 
 int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kernel) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
-    if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_STENCIL ||
+    if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_CELLSUM ||
         update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA_NODE)
         return fail(h, HUGKS_E_ARG, "unknown kernel choice");
     if (h->dim != 2) return HUGKS_OK;   // 1D has one kernel of each kind
@@ -1044,6 +1064,9 @@ int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kern
                     "MFMA node update needs nv <= 48, Nm <= 14 and its LDS on the device");
     if (flux_kernel == HUGKS_FLUX_STENCIL && !h->L.stencil_ok)
         return fail(h, HUGKS_E_UNSUPPORTED, "stencil flux needs distribution arrays under 4 GiB");
+    if (flux_kernel == HUGKS_FLUX_CELLSUM && h->L.cs_js < 0)
+        return fail(h, HUGKS_E_UNSUPPORTED,
+                    "cell-sum flux needs nv <= 64 and the v nodes negative before non-negative");
     h->L.flux_mode = flux_kernel;
     h->L.update_mode = update_kernel;
     return HUGKS_OK;
@@ -1134,7 +1157,7 @@ static int tune_case(const hugks_desc* d, bool grouped, int* flux_kernel, int* u
     double best = 1e300;
     if (d->dim == 2) {
         const int f0 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_FUSED;
-        const int f1 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_STENCIL;
+        const int f1 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_CELLSUM;
         const int u0 = grouped ? HUGKS_UPDATE_MFMA_NODE : HUGKS_UPDATE_LDS;
         for (int fk = f0; fk <= f1; ++fk)
             for (int uk = u0; uk <= HUGKS_UPDATE_MFMA_NODE; ++uk) {

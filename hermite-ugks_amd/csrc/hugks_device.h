@@ -16,6 +16,12 @@
 #include <stdint.h>
 #include "hugks.h"
 
+// Occupancy hint of a kernel (waves per SIMD); a host build of the sources
+// for testing defines it away beforehand.
+#ifndef HUGKS_WAVES_PER_EU
+#define HUGKS_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
+
 namespace hugks {
 
 constexpr int WAVE = 64;
@@ -344,6 +350,16 @@ __device__ __forceinline__ A vl_slope(A fl, A fn, A fr, bool inner, A inv_den, A
         return FAST ? fast_div(num, den) : num / den;
     }
     return (fr - fl) * inv_den;
+}
+
+// Orders one wavefront's LDS accesses across its lanes: what any lane wrote
+// before is visible to every lane after (the compiler keeps its LDS
+// accesses on their side; the hardware completes one wavefront's LDS
+// operations in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ------------------------------------------------------------ buffer loads

@@ -32,13 +32,17 @@ struct Launch2D {
     const double *acc_cell, *acc_v, *acc_h;
     unsigned nwall;      // number of wall faces (non-periodic sides)
     int flux_mode;       // interior faces: 0 one fused kernel, 1 five-kernel pipeline,
-                         // 2 the pipeline with slopes from the cell stencils
+                         // 2 the pipeline with slopes from the cell stencils, 3 the
+                         // pipeline with the face sums formed per cell in the slope pass
+    int cs_js;           // flux mode 3: first v row with v >= 0 (rows below: v < 0);
+                         // -1 where the mode is unavailable (nv > 64, unsorted v)
     int stencil_ok;      // distribution arrays under 4 GiB (32-bit byte offsets)
     int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles,
                          // 2 MFMA node update (force and collision on tiles)
     int update_nw;       // wavefronts per cell of update mode 2
     size_t lds_limit;    // largest dynamic LDS of one workgroup on the device
     double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
+    double* csum;        // flux mode 3: per-cell half-space sums [ncell][4][CELL_SUM_LD]
     double* cellpre;     // per-cell scalars of the MFMA update [ncell][24]
     double* red;         // partials of the cell reductions [RED_SCRATCH]
     int update_nt;       // threads per update workgroup
@@ -52,6 +56,8 @@ struct Launch2D {
 constexpr int FACE_SUM_LD = 40;
 constexpr int FACE_COEF_LD = 40;
 constexpr int FACE_MAXW_LD = 16;
+// Leading dimension of one half-space bin of the per-cell sums (flux mode 3).
+constexpr int CELL_SUM_LD = 40;
 // Grids below this many cells reduce over their cells in one workgroup (one
 // launch); larger ones in two passes.
 constexpr int PRED_MIN_CELLS = 16384;

@@ -1,0 +1,431 @@
+// Velocity sums of the interior faces formed per cell (gfx950): the K1
+// sums of the interior-face pipeline (k_flux_split.h, calc_flux_mtforcev/h,
+// 2d/src/flux.f90:68-102,116-120,184-202) from one read of each cell's
+// upwind arrays for both face orientations, instead of one read per
+// orientation.
+//
+// A face's sums are sums over its nodes of (weight x polynomial in vn, vt)
+// x an upwind value, taken from the left/lower cell where
+// vn - fn dt/2 >= 0 (upwind_delta) and from the right/upper cell
+// elsewhere.  Split instead by the sign of vn alone, a cell's nodes fall
+// into two half-spaces per axis: u >= 0 feeds its right face (the cell is
+// its left cell), u < 0 its left face, and likewise along v.  The sums are
+// linear in the upwind values, so a face's sums are the sum of its left
+// cell's u >= 0 (or v >= 0) bin and its right cell's u < 0 (v < 0) bin
+// whenever no node changes side between vn and vn - fn dt/2 ("flips").
+//
+//   k_cell_sum    one wavefront per cell, after k_slope2d: the four
+//                 half-space bins of the cell's 37 sums, in FP64, from h,
+//                 b and the four slope arrays
+//   k_fcombine    one wavefront per interior face: the face's sums as the
+//                 sum of two bins, or, where a node flips, K1's own pass
+//                 over the face (fsum_face)
+//
+// Lane mapping of k_cell_sum: lane l takes column iu = l % nv of row
+// jv = j0 + l / nv (64 / nv rows per pass; nv <= 64), so each lane keeps
+// one u for the whole cell and the rows of a pass share their v.  With the
+// weights a tensor product, w = wx(u) wy(v), the vertical-face sums factor
+// as wx u^a x (sums over rows of wy v^b x value) and the horizontal-face
+// ones as wx (-u)^b x (sums of wy v^a x value): 16 and 18 FP64
+// accumulators per lane instead of 2 x 35 sums, expanded into the 37 sums
+// once per bin and reduced over the wavefront.  Rows are taken in two
+// passes, v < 0 then v >= 0 (rows [0, js) and [js, nv), checked on the
+// host to split the v nodes that way), so a pass's rows all feed one
+// horizontal bin.  The arrays are staged through LDS a chunk of rows at a
+// time with 16-byte loads, which keep several KB per wavefront in flight
+// where the column layout's own loads would keep a few hundred bytes
+// (one-node loads: 3.6 ms at 512^2 cells, GH40, FP32; staged: 2.5 ms).
+#pragma once
+#include "hugks_device.h"
+#include "hugks_kernels.h"
+#include "k_flux2d.h"
+#include "k_flux_split.h"
+
+namespace hugks {
+
+// bins of a cell: 0 u < 0, 1 u >= 0, 2 v < 0, 3 v >= 0 (as upwind_delta)
+constexpr int CSUM_BINS = 4;
+constexpr int CSUM_LD = CELL_SUM_LD;
+
+// The 37 sums of a vertical face (vn = u, vt = v) from a lane's
+// accumulators over its column: x[0..3] = sum a v^b, x[4..6] = e v^b,
+// x[7..10] = f v^b, x[11..12] = g v^b, x[13] = p, x[14..15] = r v^b, with
+// a = w hf, e = w sn, f = w st, g = w bf, p = w bn, r = w bt as in
+// fsum_face.  Zero when !on.
+__device__ __forceinline__ void cs_expand_v(const double (&x)[16], double u, bool on,
+                                            double (&s)[FSUM_N]) {
+    const double A0 = x[0], A1 = x[1], A2 = x[2], A3 = x[3];
+    const double E0 = x[4], E1 = x[5], E2 = x[6];
+    const double F0 = x[7], F1 = x[8], F2 = x[9], F3 = x[10];
+    const double G0 = x[11], G1 = x[12], P0 = x[13], R0 = x[14], R1 = x[15];
+    const double u2 = u * u, u3 = u2 * u;
+    const double e2 = u2 * E0 + E2, f1 = u2 * F1 + F3, f0 = u2 * F0 + F2;
+    s[0] = A0;
+    s[1] = u * A0;
+    s[2] = A1;
+    s[4] = G0;
+    s[5] = F0;
+    s[6] = u * F0;
+    s[7] = F1;
+    s[8] = f0;
+    s[9] = R0;
+    s[10] = u2 * E0;
+    s[11] = u * F1;
+    s[12] = u2 * A0;
+    s[13] = u3 * E0;
+    s[14] = u2 * F1;
+    s[15] = u * A1;
+    s[16] = u2 * E1;
+    s[17] = u * F2;
+    s[19] = u * G0;
+    s[20] = u2 * e2;
+    s[21] = u2 * P0;
+    s[22] = u * f1;
+    s[23] = u * R1;
+    s[24] = u * E0;
+    s[25] = u * E1;
+    s[26] = u * e2;
+    s[27] = u * P0;
+    s[28] = u2 * F0;
+    s[29] = u * f0;
+    s[30] = u * R0;
+    s[31] = A2;
+    s[32] = u3 * A0;
+    s[33] = u2 * A1;
+    s[34] = u * A2;
+    s[35] = A3;
+    s[36] = G1;
+    s[3] = s[12] + s[31];
+    s[18] = s[32] + s[34];
+#pragma unroll
+    for (int q = 0; q < FSUM_N; ++q) s[q] = on ? s[q] : 0.0;
+}
+
+// The 37 sums of a horizontal face (vn = v, vt = -u), tangential-slope
+// sums negated as fsum_face does, from x[0..3] = sum a v^b, x[4..7] =
+// e v^(b+1), x[8..11] = f v^b, x[12..13] = g v^b, x[14..15] = p v^(b+1),
+// x[16..17] = r v^b.
+__device__ __forceinline__ void cs_expand_h(const double (&x)[18], double u,
+                                            double (&s)[FSUM_N]) {
+    const double A0 = x[0], A1 = x[1], A2 = x[2], A3 = x[3];
+    const double E1 = x[4], E2 = x[5], E3 = x[6], E4 = x[7];
+    const double F0 = x[8], F1 = x[9], F2 = x[10], F3 = x[11];
+    const double G0 = x[12], G1 = x[13], P1 = x[14], P2 = x[15], R0 = x[16], R1 = x[17];
+    const double u2 = u * u, nu = -u;
+    const double f1 = F3 + u2 * F1;
+    s[0] = A0;
+    s[1] = A1;
+    s[2] = nu * A0;
+    s[4] = G0;
+    s[5] = F0;
+    s[6] = F1;
+    s[7] = nu * F0;
+    s[8] = F2 + u2 * F0;
+    s[9] = R0;
+    s[10] = E2;
+    s[11] = nu * F1;
+    s[12] = A2;
+    s[13] = E3;
+    s[14] = nu * F2;
+    s[15] = nu * A1;
+    s[16] = nu * E2;
+    s[17] = u2 * F1;
+    s[19] = G1;
+    s[20] = E4 + u2 * E2;
+    s[21] = P2;
+    s[22] = nu * f1;
+    s[23] = nu * R1;
+    s[24] = E1;
+    s[25] = nu * E1;
+    s[26] = E3 + u2 * E1;
+    s[27] = P1;
+    s[28] = F2;
+    s[29] = f1;
+    s[30] = R1;
+    s[31] = u2 * A0;
+    s[32] = A3;
+    s[33] = nu * A2;
+    s[34] = u2 * A1;
+    s[35] = nu * u2 * A0;
+    s[36] = nu * G0;
+    s[3] = s[12] + s[31];
+    s[18] = s[32] + s[34];
+    // horizontal faces take the tangential slopes with the opposite sign
+    constexpr int tq[13] = {5, 6, 7, 8, 9, 11, 14, 17, 22, 23, 28, 29, 30};
+#pragma unroll
+    for (int t = 0; t < 13; ++t) s[tq[t]] = -s[tq[t]];
+}
+
+// One bin: the 37 sums expanded from a lane's accumulators (vertical V,
+// else horizontal) and reduced over the wavefront in two halves (fewer
+// values live at once), written to out[0..36].
+template <bool V, int N>
+__device__ __forceinline__ void cs_reduce_bin(const double (&x0)[N], double u, double wu, bool on,
+                                              double* __restrict__ out) {
+    // the column's weight factor, left out of the per-node products
+    double x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = wu * x0[i];
+    constexpr int H1 = 19, H2 = FSUM_N - H1;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        double s[FSUM_N];
+        if constexpr (V) {
+            static_assert(N == 16, "vertical accumulators");
+            cs_expand_v(x, u, on, s);
+        } else {
+            static_assert(N == 18, "horizontal accumulators");
+            cs_expand_h(x, u, s);
+        }
+        int q;
+        if (half == 0) {
+            double t[H1];
+#pragma unroll
+            for (int i = 0; i < H1; ++i) t[i] = s[i];
+            const double y = wave_reduce_scatter<H1>(t, q);
+            if (q >= 0) out[q] = y;
+        } else {
+            double t[H2];
+#pragma unroll
+            for (int i = 0; i < H2; ++i) t[i] = s[H1 + i];
+            const double y = wave_reduce_scatter<H2>(t, q);
+            if (q >= 0) out[H1 + q] = y;
+        }
+    }
+}
+
+// Bytes of one LDS chunk per array and wavefront: whole rows, at most
+// what one 16-byte load per lane covers.
+constexpr int CS_CHUNK_BYTES = WAVE * 16;
+constexpr int CS_MAX_ROWS = 16;
+__host__ __device__ constexpr int cs_chunk_rows(int nv, int es) {
+    return CS_CHUNK_BYTES / (nv * es) < 1 ? 1
+         : CS_CHUNK_BYTES / (nv * es) > CS_MAX_ROWS ? CS_MAX_ROWS
+         : CS_CHUNK_BYTES / (nv * es);
+}
+// LDS of one wavefront: the chunk of the six arrays, then each chunk row's
+// v and weight factor (FP64)
+template <typename T>
+constexpr int cs_lds_per_wave() {
+    return 6 * CS_CHUNK_BYTES + 2 * CS_MAX_ROWS * 8;
+}
+
+// The four half-space bins of one cell, from h, b and the four slope
+// arrays.  The rows are staged through LDS a chunk at a time: every lane
+// first loads its 16-byte pieces of the chunk of all six arrays (VW
+// elements; one element per load where a row is not a multiple of 16
+// bytes), then the lanes read the chunk back one node per lane in the
+// column layout above.
+template <typename T, int VW>
+__device__ __forceinline__ void k_cell_sum_body(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ wx, const double* __restrict__ wy, double* __restrict__ csum,
+    int js, unsigned char* __restrict__ lds, int bid, int nblk) {
+    using A = typename Arith<T>::A;
+    if (st->stopped) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int L = __builtin_amdgcn_readfirstlane(
+        (int)xcd_block(bid, nblk) * SLOPE_CPW + (int)(threadIdx.x / WAVE));
+    if (L >= c.nx * c.ny) return;
+    const int cell = tiled_cell(L, c.nx, c.ny);
+    const size_t o = (size_t)cell * c.ldv;
+    const int nv = c.nv;
+    const int CR = cs_chunk_rows(nv, (int)sizeof(T));
+    const int cw = CR * nv;                  // LDS elements per array
+    unsigned char* myb = lds + (size_t)(threadIdx.x / WAVE) * cs_lds_per_wave<T>();
+    T* my = reinterpret_cast<T*>(myb);
+    double* rowc = reinterpret_cast<double*>(myb + 6 * CS_CHUNK_BYTES);
+    const int R = WAVE / nv;                 // rows per pass of the sums
+    const int r = lane / nv, iu = lane - r * nv;
+    const bool act = r < R;
+    const double u = act ? u1[iu] : 0.0;
+    const double wu = act ? wx[iu] : 0.0;    // the column's weight factor
+    const int su = upwind_delta((A)u);
+    const double hlx = 0.5 * c.dx, hly = 0.5 * c.dy;
+    const double sgnV = su ? hlx : -hlx;
+    double av[16], ah[18];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) av[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 18; ++q) ah[q] = 0.0;
+    double* out = csum + (size_t)cell * CSUM_BINS * CSUM_LD;
+    typedef T Vec __attribute__((ext_vector_type(VW)));
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+        const int lo = p ? js : 0, hi = p ? nv : js;
+        const double sgnH = p ? hly : -hly;
+#pragma unroll 1
+        for (int j0 = lo; j0 < hi; j0 += CR) {
+            const int cr = hi - j0 < CR ? hi - j0 : CR;
+            const int ne = cr * nv;          // elements of this chunk per array
+            const int base = j0 * nv;
+            // every load of the chunk first, then the LDS stores
+            constexpr int NL = VW > 1 ? 1 : (CS_CHUNK_BYTES / (int)sizeof(T) + WAVE - 1) / WAVE;
+#pragma unroll
+            for (int t = 0; t < NL; ++t) {
+                const int e0 = (lane + WAVE * t) * VW;
+                const bool in = e0 < ne;
+                const size_t g = o + (size_t)base + (size_t)(in ? e0 : 0);
+                // lanes past the chunk load its first piece again (in
+                // bounds) and do not store it
+                const Vec x0 = *reinterpret_cast<const Vec*>(h + g);
+                const Vec x1 = *reinterpret_cast<const Vec*>(b + g);
+                const Vec x2 = *reinterpret_cast<const Vec*>(shx + g);
+                const Vec x3 = *reinterpret_cast<const Vec*>(shy + g);
+                const Vec x4 = *reinterpret_cast<const Vec*>(sbx + g);
+                const Vec x5 = *reinterpret_cast<const Vec*>(sby + g);
+                if (t == 0) {
+                    // the chunk rows' v and weight factor, with the loads
+                    const int jr = lane < cr ? j0 + lane : j0;
+                    const double vr = v1[jr], wr = wy[jr];
+                    wave_lds_sync();     // the previous chunk's reads are done
+                    if (lane < cr) {
+                        rowc[2 * lane] = vr;
+                        rowc[2 * lane + 1] = wr;
+                    }
+                }
+                if (in) {
+                    *reinterpret_cast<Vec*>(my + e0) = x0;
+                    *reinterpret_cast<Vec*>(my + cw + e0) = x1;
+                    *reinterpret_cast<Vec*>(my + 2 * cw + e0) = x2;
+                    *reinterpret_cast<Vec*>(my + 3 * cw + e0) = x3;
+                    *reinterpret_cast<Vec*>(my + 4 * cw + e0) = x4;
+                    *reinterpret_cast<Vec*>(my + 5 * cw + e0) = x5;
+                }
+            }
+            wave_lds_sync();
+#pragma unroll 2
+            for (int rr = 0; rr < cr; rr += R) {
+                const int jj = rr + r;
+                if (!act || jj >= cr) continue;
+                const int e = jj * nv + iu;
+                // the row's weight factor times powers of its v
+                const double v = rowc[2 * jj], c0 = rowc[2 * jj + 1];
+                const double c1 = c0 * v, c2 = c1 * v, c3 = c2 * v, c4 = c3 * v;
+                const double hu = (double)my[e], bu = (double)my[cw + e];
+                const double sx = (double)my[2 * cw + e], sy = (double)my[3 * cw + e];
+                const double tx = (double)my[4 * cw + e], ty = (double)my[5 * cw + e];
+                {
+                    // vertical faces: sn = sx, st = sy (and the b slopes)
+                    const double a = hu + sgnV * sx, g = bu + sgnV * tx;
+                    av[0] += a * c0;
+                    av[1] += a * c1;
+                    av[2] += a * c2;
+                    av[3] += a * c3;
+                    av[4] += sx * c0;
+                    av[5] += sx * c1;
+                    av[6] += sx * c2;
+                    av[7] += sy * c0;
+                    av[8] += sy * c1;
+                    av[9] += sy * c2;
+                    av[10] += sy * c3;
+                    av[11] += g * c0;
+                    av[12] += g * c1;
+                    av[13] += tx * c0;
+                    av[14] += ty * c0;
+                    av[15] += ty * c1;
+                }
+                {
+                    // horizontal faces: sn = sy, st = sx
+                    const double a = hu + sgnH * sy, g = bu + sgnH * ty;
+                    ah[0] += a * c0;
+                    ah[1] += a * c1;
+                    ah[2] += a * c2;
+                    ah[3] += a * c3;
+                    ah[4] += sy * c1;
+                    ah[5] += sy * c2;
+                    ah[6] += sy * c3;
+                    ah[7] += sy * c4;
+                    ah[8] += sx * c0;
+                    ah[9] += sx * c1;
+                    ah[10] += sx * c2;
+                    ah[11] += sx * c3;
+                    ah[12] += g * c0;
+                    ah[13] += g * c1;
+                    ah[14] += ty * c1;
+                    ah[15] += ty * c2;
+                    ah[16] += tx * c0;
+                    ah[17] += tx * c1;
+                }
+            }
+        }
+        // the horizontal bin of this pass's rows
+        cs_reduce_bin<false>(ah, u, wu, true, out + (2 + p) * CSUM_LD);
+#pragma unroll
+        for (int t = 0; t < 18; ++t) ah[t] = 0.0;
+    }
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side)
+        cs_reduce_bin<true>(av, u, wu, act && su == side, out + side * CSUM_LD);
+}
+#ifndef HUGKS_BODIES_ONLY
+template <typename T, int VW>
+__global__ __launch_bounds__(WAVE * SLOPE_CPW) void k_cell_sum(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ wx, const double* __restrict__ wy, double* __restrict__ csum,
+    int js) {
+    __shared__ __align__(16) unsigned char lds[SLOPE_CPW * cs_lds_per_wave<T>()];
+    k_cell_sum_body<T, VW>(c, st, h, b, shx, shy, sbx, sby, u1, v1, wx, wy, csum, js, lds,
+                           (int)blockIdx.x, (int)gridDim.x);
+}
+#endif
+
+// Sums of interior face g: the left cell's forward bin plus the right
+// cell's backward bin, or K1's pass over the face when a node flips.
+template <typename T>
+__device__ __forceinline__ void k_fcombine_body(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ und, const double* __restrict__ vnd,
+    const double* __restrict__ wnd, const double* __restrict__ node4,
+    const double* __restrict__ acc_v, const double* __restrict__ acc_h,
+    const double* __restrict__ csum, double* __restrict__ fsum, int nvi, int nint, int bid) {
+    using A = typename Arith<T>::A;
+    if (st->stopped) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int g = __builtin_amdgcn_readfirstlane(bid * SPLIT_FPW + (int)(threadIdx.x / WAVE));
+    if (g >= nint) return;
+    const FaceGeo fg = face_geo(c, g, nvi);
+    const double* acc = fg.dir == 0 ? acc_v : acc_h;
+    const double vshift = 0.5 * acc[2 * fg.face] * st->dt;
+    const A vs = (A)vshift;
+    bool flip = false;
+    if (lane < c.nv) {
+        const A x = (A)(fg.dir == 0 ? u1[lane] : v1[lane]);
+        flip = upwind_delta(x - vs) != upwind_delta(x);
+    }
+    double* out = fsum + (size_t)g * FSUM_LD;
+    if (__ballot(flip) != 0) {
+        const Upwind<T, false> up = make_upwind<T, false>(c, fg, h, b, shx, shy, sbx, sby);
+        const NodeTabs nt{und, vnd, wnd, node4};
+        if (fg.dir == 0)
+            fsum_face<0, T, false>(c, vshift, up, nt, out);
+        else
+            fsum_face<1, T, false>(c, vshift, up, nt, out);
+        return;
+    }
+    const double* bL = csum + ((size_t)fg.cL * CSUM_BINS + (fg.dir ? 3 : 1)) * CSUM_LD;
+    const double* bR = csum + ((size_t)fg.cR * CSUM_BINS + (fg.dir ? 2 : 0)) * CSUM_LD;
+    if (lane < FSUM_N) out[lane] = bL[lane] + bR[lane];
+}
+#ifndef HUGKS_BODIES_ONLY
+template <typename T>
+__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fcombine(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ und, const double* __restrict__ vnd,
+    const double* __restrict__ wnd, const double* __restrict__ node4,
+    const double* __restrict__ acc_v, const double* __restrict__ acc_h,
+    const double* __restrict__ csum, double* __restrict__ fsum, int nvi, int nint) {
+    k_fcombine_body<T>(c, st, h, b, shx, shy, sbx, sby, u1, v1, und, vnd, wnd, node4, acc_v,
+                       acc_h, csum, fsum, nvi, nint, (int)blockIdx.x);
+}
+#endif
+
+}  // namespace hugks

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void kg_cell_pre(const int* __restrict__ tab,
 
 template <typename T, int NW, int MAXP, int NVC>
 __global__ __launch_bounds__(NW * WAVE)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void kg_update(
+HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? 3 : 1) void kg_update(
     const int* __restrict__ tab, const Launch2D* __restrict__ Ls) {
     const GroupIdx g = group_index(tab);
     const Launch2D& L = Ls[g.cs];

@@ -505,7 +505,7 @@ __device__ __forceinline__ void k_update2d_mfma_body(
 // FP32 storage: at least five wavefronts per SIMD (the LDS of a cell allows
 // five workgroups per CU at nv = 40).
 template <typename T, int EPT, int NT, int NVC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+__global__ __launch_bounds__(NT) HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? 5 : 1)
 void k_update2d_mfma(
     Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,

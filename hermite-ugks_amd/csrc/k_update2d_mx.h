@@ -609,7 +609,7 @@ __device__ __forceinline__ void k_update2d_mx_body(
 // at nv = 40, allows five three-wavefront cells per CU).
 template <typename T, int NW, int MAXP, int NVC, bool SB3 = false>
 __global__ __launch_bounds__(NW * WAVE)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_update2d_mx(
+HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? 3 : 1) void k_update2d_mx(
     Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb,

@@ -2,6 +2,7 @@
 #include "k_flux2d.h"
 #include "k_flux_face.h"
 #include "k_flux_split.h"
+#include "k_flux_cellsum.h"
 
 namespace hugks {
 
@@ -9,13 +10,24 @@ template <typename T>
 static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     const Consts& c = L.c;
     const size_t ncell = (size_t)c.nx * c.ny;
+    const bool cs = L.flux_mode == HUGKS_FLUX_CELLSUM;
     if (L.flux_mode != HUGKS_FLUX_STENCIL) {
         HookScope hs(L, "slope2d");
         hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)((ncell + SLOPE_CPW - 1) / SLOPE_CPW)),
                            dim3(WAVE * SLOPE_CPW), 0, s, c,
                            L.st, (const T*)L.h, (const T*)L.b, (T*)L.shx, (T*)L.shy, (T*)L.sbx,
                            (T*)L.sby, 0);
-    } else if (L.nwall) {
+    }
+    if (cs) {
+        // 16-byte pieces where a row is a multiple of 16 bytes
+        constexpr int VW = 16 / (int)sizeof(T);
+        HookScope hs(L, "cell_sum");
+        auto kc = (c.nv * (int)sizeof(T)) % 16 == 0 ? k_cell_sum<T, VW> : k_cell_sum<T, 1>;
+        hipLaunchKernelGGL(kc, dim3((unsigned)((ncell + SLOPE_CPW - 1) / SLOPE_CPW)),
+                           dim3(WAVE * SLOPE_CPW), 0, s, c, L.st, (const T*)L.h, (const T*)L.b,
+                           (const T*)L.shx, (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby,
+                           L.u1, L.v1, L.wx, L.wy, L.csum, L.cs_js);
+    } else if (L.flux_mode == HUGKS_FLUX_STENCIL && L.nwall) {
         // the wall fluxes read the slope arrays of the wall cells
         HookScope hs(L, "slope2d");
         hipLaunchKernelGGL((k_slope2d<T>),
@@ -37,7 +49,14 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
         const bool stn = L.flux_mode == HUGKS_FLUX_STENCIL;
         const unsigned gw = (unsigned)((nint + SPLIT_FPW - 1) / SPLIT_FPW);
         const unsigned gl = (unsigned)((nint + 255) / 256);
-        {
+        if (cs) {
+            HookScope hs(L, "flux_combine");
+            hipLaunchKernelGGL((k_fcombine<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+                               (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
+                               (const T*)L.sbx, (const T*)L.sby, L.u1, L.v1, L.unode_d,
+                               L.vnode_d, L.wnode_d, L.node4, L.acc_v, L.acc_h, L.csum, L.fsum,
+                               nvf, nint);
+        } else {
             HookScope hs(L, "flux_sum");
             auto kf = stn ? k_fsum<T, true> : k_fsum<T, false>;
             hipLaunchKernelGGL(kf, dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,

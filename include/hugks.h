@@ -268,10 +268,19 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
  * the same step; they differ in speed by case shape.
  *   flux:   HUGKS_FLUX_FUSED    one kernel per face orientation
  *           HUGKS_FLUX_SPLIT    five-kernel pipeline reading the slope arrays
- *                               (default)
+ *                               (default where HUGKS_FLUX_CELLSUM does not
+ *                               apply)
  *           HUGKS_FLUX_STENCIL  five-kernel pipeline forming the van Leer
  *                               slopes from each upwind cell's stencil
  *                               (needs each distribution array under 4 GiB)
+ *           HUGKS_FLUX_CELLSUM  the pipeline with each face's velocity sums
+ *                               added from half-space sums of its two cells
+ *                               (each cell's arrays read once for the sums of
+ *                               both face orientations, not once per
+ *                               orientation); nv <= 64, the v nodes negative
+ *                               before non-negative and weights that factor
+ *                               as w[i + nv j] = wx[i] wy[j] (default where
+ *                               these hold)
  *   update: HUGKS_UPDATE_LDS  Hermite contractions as LDS dot products
  *           HUGKS_UPDATE_MFMA Hermite contractions as matrix tiles
  *                             (nv <= 48, Nm <= 14)
@@ -284,6 +293,7 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
 #define HUGKS_FLUX_FUSED   0
 #define HUGKS_FLUX_SPLIT   1
 #define HUGKS_FLUX_STENCIL 2
+#define HUGKS_FLUX_CELLSUM 3
 #define HUGKS_UPDATE_LDS   0
 #define HUGKS_UPDATE_MFMA  1
 #define HUGKS_UPDATE_MFMA_NODE 2

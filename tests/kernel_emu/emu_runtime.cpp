@@ -14,6 +14,7 @@ unsigned nthreads = 0;
 Bar block_bar = {0, 0};
 Bar wave_bar[16] = {};
 double mfa[1024], mfb[1024];
+float mbfa[1024 * 8], mbfb[1024 * 8];
 bool capturing = false;
 std::vector<std::function<void()>>* capture_list = nullptr;
 unsigned char* dyn_smem = nullptr;

@@ -28,6 +28,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define HUGKS_WAVES_PER_EU(n)
 #define __shared__ static
 #define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(::emu::dyn_smem);
 // wave-uniform values only: the emulation has no scalar register file
@@ -53,6 +54,29 @@ inline uint64_t __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t r, i
     if ((uint64_t)(uint32_t)off + 8 <= r.bytes) memcpy(&v, r.base + (uint32_t)off, 8);
     return v;
 }
+
+inline void __builtin_amdgcn_raw_buffer_store_b32(uint32_t v, __amdgpu_buffer_rsrc_t r, int off, int,
+                                                  int) {
+    if ((uint64_t)(uint32_t)off + 4 <= r.bytes) memcpy(const_cast<char*>(r.base) + (uint32_t)off, &v, 4);
+}
+inline void __builtin_amdgcn_raw_buffer_store_b64(unsigned __attribute__((vector_size(8))) v,
+                                                  __amdgpu_buffer_rsrc_t r, int off, int, int) {
+    if ((uint64_t)(uint32_t)off + 8 <= r.bytes) memcpy(const_cast<char*>(r.base) + (uint32_t)off, &v, 8);
+}
+// v_perm_b32: byte i of the result is byte sel[i] of {b (bytes 0-3), a (4-7)};
+// selector 0x0c gives 0x00, larger ones 0xff (the cases the kernels use: 0-7)
+inline uint32_t __builtin_amdgcn_perm(uint32_t a, uint32_t b, uint32_t sel) {
+    const uint64_t ab = ((uint64_t)a << 32) | b;
+    uint32_t r = 0;
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t s = (sel >> (8 * i)) & 0xffu;
+        const uint32_t byte = s < 8 ? (uint32_t)((ab >> (8 * s)) & 0xffu) : s == 0x0c ? 0u : 0xffu;
+        r |= byte << (8 * i);
+    }
+    return r;
+}
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (reinterpret_cast<const void*>(&(x)))
 
 struct dim3 {
     unsigned x, y, z;
@@ -163,6 +187,10 @@ void launch_now(K kernel, dim3 g, dim3 b, size_t shmem, A... args) {
 #define gridDim (::emu::gdim)
 
 inline void __syncthreads() { emu::barrier(emu::block_bar, 0, emu::nthreads); }
+// wavefront-scope ordering: the lanes of a wavefront run as separate fibers
+// here, so the wave barrier is a real one and the fences are no-ops
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+inline void __builtin_amdgcn_wave_barrier() { emu::wave_barrier(); }
 
 // Wave-level operations exchange through xbuf between two wave barriers
 // (every lane of the wavefront takes part, as on the hardware).
@@ -188,6 +216,18 @@ inline T __shfl(T v, int lane, int width = 64) {
     std::memcpy(&r, emu::xbuf + 8 * src, sizeof(T));
     emu::wave_barrier();
     return r;
+}
+
+// Bit l set where lane l of the wavefront passes a true predicate.
+inline unsigned long long __ballot(int pred) {
+    const unsigned t = threadIdx.x, w0 = t & ~63u;
+    emu::xbuf[8 * t] = pred ? 1 : 0;
+    emu::wave_barrier();
+    unsigned long long m = 0;
+    for (unsigned l = 0; l < 64; ++l)
+        if (emu::xbuf[8 * (w0 + l)]) m |= 1ull << l;
+    emu::wave_barrier();
+    return m;
 }
 
 // v_permlane32_swap / v_permlane16_swap (gfx950): returns {new vdst, new
@@ -254,6 +294,32 @@ inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, emu_f4 c, i
 inline emu_d4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, emu_d4 c, int, int, int) {
     return emu::mfma16x16x4<emu_d4, double>(a, b, c, true);
 }
+// 16x16x32 bf16 tiles: lane l holds A(l%16, 8(l/16)+s) and B(8(l/16)+s, l%16)
+// in slot s; the f32 result layout as 16x16x4f32.
+typedef __bf16 emu_bf8 __attribute__((ext_vector_type(8)));
+namespace emu {
+extern float mbfa[1024 * 8], mbfb[1024 * 8];
+}
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf8 a, emu_bf8 b, emu_f4 c, int, int, int) {
+    const unsigned t = threadIdx.x, w0 = t & ~63u, l = t & 63u;
+    for (int s = 0; s < 8; ++s) {
+        emu::mbfa[8 * t + s] = (float)a[s];
+        emu::mbfb[8 * t + s] = (float)b[s];
+    }
+    emu::wave_barrier();
+    emu_f4 r = c;
+    for (int v = 0; v < 4; ++v) {
+        const unsigned i = 4 * (l >> 4) + v, j = l & 15;
+        float acc = c[v];
+        for (unsigned k = 0; k < 32; ++k) {
+            const unsigned la = w0 + 16 * (k / 8) + i, lb = w0 + 16 * (k / 8) + j;
+            acc = std::fma(emu::mbfa[8 * la + k % 8], emu::mbfb[8 * lb + k % 8], acc);
+        }
+        r[v] = acc;
+    }
+    emu::wave_barrier();
+    return r;
+}
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 
 // ---- runtime API
@@ -262,6 +328,7 @@ typedef enum hipError_t {
     hipErrorInvalidValue = 1,
     hipErrorOutOfMemory = 2,
     hipErrorNoDevice = 100,
+    hipErrorInvalidDevice = 101,
 } hipError_t;
 typedef struct emu_stream* hipStream_t;
 typedef struct emu_event* hipEvent_t;
@@ -276,6 +343,7 @@ struct hipDeviceProp_t {
     size_t maxSharedMemoryPerMultiProcessor;
     size_t totalGlobalMem;
     size_t sharedMemPerBlockOptin;
+    size_t sharedMemPerBlock;
 };
 #define hipLaunchKernelGGL(kernel, g, b, sh, st, ...) ::emu::launch(kernel, g, b, sh, __VA_ARGS__)
 
@@ -283,6 +351,11 @@ inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "s
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipSetDevice(int d) { return d == 0 ? hipSuccess : hipErrorInvalidDevice; }
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = const_cast<void*>(sym);
+    return hipSuccess;
+}
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
     memset(p, 0, sizeof(*p));
     strcpy(p->name, "CPU emulation");
@@ -291,6 +364,7 @@ inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
     p->maxSharedMemoryPerMultiProcessor = 160 * 1024;
     p->totalGlobalMem = (size_t)1 << 34;
     p->sharedMemPerBlockOptin = 160 * 1024;
+    p->sharedMemPerBlock = 64 * 1024;
     return hipSuccess;
 }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {

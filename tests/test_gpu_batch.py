@@ -158,14 +158,18 @@ def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
         assert b.grouped
         b.advance(5)
         wg = [s.macro(with_q=False) for s in b.solvers]
+    # case by case with the schedule of the grouped launches (split flux,
+    # node update), which the batch's graphs capture at create
     monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
+    monkeypatch.setenv("HUGKS_FLUX_KERNEL", "split")
     with capi.Batch(cs) as b:
         assert not b.grouped
         for s in b.solvers:
-            s.set_schedule(capi.FLUX_SPLIT, capi.UPDATE_MFMA)
+            assert s.schedule() == (capi.FLUX_SPLIT, capi.UPDATE_MFMA_NODE)
         b.advance(5)
         wl = [s.macro(with_q=False) for s in b.solvers]
     monkeypatch.delenv("HUGKS_BATCH_GROUPED")
+    monkeypatch.delenv("HUGKS_FLUX_KERNEL")
     for c, a, x in zip(cs, wg, wl):
         assert np.array_equal(a, x)
         o = oracle_py.Oracle(c.with_precision(capi.FP64))

@@ -35,7 +35,8 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FD = sorted(glob.glob(os.path.join(GOLDEN, "*_fd_*.npz")))
 FD2D = [p for p in FD if os.path.basename(p).startswith("p2d")]
-FLUX = {"fused": capi.FLUX_FUSED, "split": capi.FLUX_SPLIT, "stencil": capi.FLUX_STENCIL}
+FLUX = {"fused": capi.FLUX_FUSED, "split": capi.FLUX_SPLIT, "stencil": capi.FLUX_STENCIL,
+        "cellsum": capi.FLUX_CELLSUM}
 
 
 def _ids(paths):

@@ -33,7 +33,8 @@ pytestmark = pytest.mark.gpu
 TOL_FP64 = 1e-10
 TOL_MIXED = 1e-5
 STEPS = 6
-FLUX = {"split": capi.FLUX_SPLIT, "stencil": capi.FLUX_STENCIL, "fused": capi.FLUX_FUSED}
+FLUX = {"split": capi.FLUX_SPLIT, "stencil": capi.FLUX_STENCIL, "fused": capi.FLUX_FUSED,
+        "cellsum": capi.FLUX_CELLSUM}
 UPDATE = {"mfma": capi.UPDATE_MFMA, "lds": capi.UPDATE_LDS, "mfma-node": capi.UPDATE_MFMA_NODE}
 
 

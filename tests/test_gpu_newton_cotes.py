@@ -29,7 +29,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # (the finite-difference fixtures on Newton-Cotes grids are test_gpu_finite_difference.py's)
 NC = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*_nc*.npz")) if "_fd_" not in p)
 NC2D = [p for p in NC if os.path.basename(p).startswith("p2d")]
-SCHED = [(f, u) for f in (capi.FLUX_FUSED, capi.FLUX_SPLIT, capi.FLUX_STENCIL)
+SCHED = [(f, u) for f in (capi.FLUX_FUSED, capi.FLUX_SPLIT, capi.FLUX_STENCIL, capi.FLUX_CELLSUM)
          for u in (capi.UPDATE_LDS, capi.UPDATE_MFMA, capi.UPDATE_MFMA_NODE)]
 
 

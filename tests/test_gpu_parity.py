@@ -101,12 +101,13 @@ def test_advance_until_output_time(have_gpu):
     assert np.all(field_err(w, o2.get()[0]) <= TOL_FP64)
 
 
-@pytest.mark.parametrize("mode", ["fused", "split", "split-nodes", "stencil"])
+@pytest.mark.parametrize("mode", ["fused", "split", "split-nodes", "stencil", "cellsum"])
 def test_flux_kernels_agree(have_gpu, monkeypatch, mode):
     """The interior-face flux implementations (one fused kernel; the
     five-kernel pipeline reading the slope arrays, with the Maxwellian sums
     from 1D moments of the tensor-product quadrature or node by node; the
-    pipeline forming the slopes from the cell stencils) against the oracle,
+    pipeline forming the slopes from the cell stencils; the pipeline with
+    the face sums added from per-cell half-space sums) against the oracle,
     FP64 and mixed."""
     monkeypatch.setenv("HUGKS_FLUX_KERNEL", mode)
     for prec, tol in ((capi.FP64, TOL_FP64), (capi.MIXED_FP32, TOL_MIXED)):
@@ -144,7 +145,7 @@ def test_flux_weights_not_a_tensor_product(have_gpu):
     assert np.all(err <= TOL_FP64), err
 
 
-@pytest.mark.parametrize("mode", ["split", "stencil", "fused"])
+@pytest.mark.parametrize("mode", ["split", "stencil", "fused", "cellsum"])
 def test_odd_velocity_grids(have_gpu, monkeypatch, mode):
     """nv^2 odd (the face kernels read node pairs, the last pair ending on
     the first pad node) and nv = 14 (a pair of nodes on both sides of the
@@ -171,3 +172,22 @@ def test_compiled_in_grid_sizes(have_gpu, nv):
         err = field_err(r["w"], r["wr"])
         print("nv", nv, "prec", prec, "W err", err, flush=True)
         assert np.all(err <= tol), err
+
+
+def test_cellsum_refuses_unsplit_nodes(have_gpu):
+    """The cell-sum schedule needs the v nodes negative before
+    non-negative (its rows are taken in those two passes): a case whose v
+    list is reversed is refused with HUGKS_E_UNSUPPORTED, and the other
+    schedules still run it to the oracle."""
+    case = cases.build_2d(rt(nx=6, ny=8, nv=12, nm=6), capi.FP64, rt_variant="cartesian", seed=4)
+    nv = case.desc.nv
+    case.arrays["v"][:] = case.arrays["v"][::-1].copy()
+    w = case.arrays["weight"].reshape(nv, nv)
+    case.arrays["weight"][:] = w[::-1, :].reshape(-1).copy()
+    with capi.Solver(case) as s:
+        with pytest.raises(capi.HugksError) as ei:
+            s.set_schedule(capi.FLUX_CELLSUM, capi.UPDATE_MFMA_NODE)
+        assert ei.value.code == capi.E_UNSUPPORTED
+    r = run_pair(case, 3)
+    err = field_err(r["w"], r["wr"])
+    assert np.all(err <= TOL_FP64), err
