@@ -70,7 +70,7 @@ def algorithmic_bytes(case, kernel):
     nhf = d.nx * (d.ny + 1) * nn
     if kernel == "slope2d":          # read h, b; write 4 slope arrays
         return 6 * s * N
-    if kernel == "cell_sum":         # read h, b, 4 slope arrays; write 4 bins of 37 sums per cell
+    if kernel == "slope_sum":        # read h, b; write 4 slope arrays and 4 bins of 37 sums per cell
         return 6 * s * N + 8 * 4 * 37 * d.nx * d.ny
     if kernel == "flux_vface":       # read h, b and 4 slopes; write 2 face arrays
         return 6 * s * N + 2 * s * nvf
@@ -258,7 +258,7 @@ def rocprof_summary(dirs):
 TIMER_OF = [("k_update2d_mx", "update2d"), ("k_update2d_mfma", "update2d"),
             ("k_update2d<", "update2d"), ("k_fsum", "flux_sum"), ("k_fnode", "flux_node"),
             ("k_fface", "flux_face"), ("k_slope2d", "slope2d"), ("k_flux_wall", "flux_wall"),
-            ("k_cell_sum", "cell_sum"), ("k_fcombine", "flux_combine"),
+            ("k_cell_sum", "slope_sum"), ("k_fcombine", "flux_combine"),
             ("k_cell_pre", "cell_pre"), ("k_finalize", "finalize"),
             ("k_part_sum", "finalize_partials"), ("k_rate_part", "timestep_rates"),
             ("k_timestep", "timestep")]

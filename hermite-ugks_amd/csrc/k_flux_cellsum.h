@@ -14,9 +14,10 @@
 // cell's u >= 0 (or v >= 0) bin and its right cell's u < 0 (v < 0) bin
 // whenever no node changes side between vn and vn - fn dt/2 ("flips").
 //
-//   k_cell_sum    one wavefront per cell, after k_slope2d: the four
-//                 half-space bins of the cell's 37 sums, in FP64, from h,
-//                 b and the four slope arrays
+//   k_cell_sum    one wavefront per cell, in place of k_slope2d: the van
+//                 Leer slopes as k_slope2d forms and stores them, then the
+//                 four half-space bins of the cell's 37 sums, in FP64,
+//                 from h, b and the slopes in LDS
 //   k_fcombine    one wavefront per interior face: the face's sums as the
 //                 sum of two bins, or, where a node flips, K1's own pass
 //                 over the face (fsum_face)
@@ -31,10 +32,16 @@
 // once per bin and reduced over the wavefront.  Rows are taken in two
 // passes, v < 0 then v >= 0 (rows [0, js) and [js, nv), checked on the
 // host to split the v nodes that way), so a pass's rows all feed one
-// horizontal bin.  The arrays are staged through LDS a chunk of rows at a
-// time with 16-byte loads, which keep several KB per wavefront in flight
-// where the column layout's own loads would keep a few hundred bytes
-// (one-node loads: 3.6 ms at 512^2 cells, GH40, FP32; staged: 2.5 ms).
+// horizontal bin.  The rows go through LDS a chunk at a time: 16-byte
+// loads of the five stencil cells' h and b, the slopes formed and stored in
+// that layout, then h, b and the slopes written to LDS and read back in the
+// column layout.  The 16-byte loads keep several KB per wavefront in flight
+// where the column layout's own loads would keep a few hundred bytes.  At
+// 512^2 cells, GH40 (K1 of the split pipeline: 4.25 ms FP32, 6.19 FP64;
+// the slope kernel 2.47 / 5.01):
+//   sums reading the stored slopes, one node per load   3.6 ms FP32
+//   the same staged through LDS                          2.5 / 3.9 ms
+//   slopes formed here, no slope kernel                  4.0 / 5.9 ms
 #pragma once
 #include "hugks_device.h"
 #include "hugks_kernels.h"
@@ -210,17 +217,17 @@ constexpr int cs_lds_per_wave() {
     return 6 * CS_CHUNK_BYTES + 2 * CS_MAX_ROWS * 8;
 }
 
-// The four half-space bins of one cell, from h, b and the four slope
-// arrays.  The rows are staged through LDS a chunk at a time: every lane
-// first loads its 16-byte pieces of the chunk of all six arrays (VW
-// elements; one element per load where a row is not a multiple of 16
-// bytes), then the lanes read the chunk back one node per lane in the
-// column layout above.
+// The slopes and the four half-space bins of one cell.  The rows are
+// staged through LDS a chunk at a time: every lane first loads its 16-byte
+// pieces of the chunk from the five stencil cells (VW elements; one element
+// per load where a row is not a multiple of 16 bytes), forms and stores the
+// slopes and puts h, b and the slopes in LDS, then the lanes read the chunk
+// back one node per lane in the column layout above.
 template <typename T, int VW>
 __device__ __forceinline__ void k_cell_sum_body(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
-    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    T* __restrict__ shx, T* __restrict__ shy, T* __restrict__ sbx, T* __restrict__ sby,
+    const double* __restrict__ u1, const double* __restrict__ v1,
     const double* __restrict__ wx, const double* __restrict__ wy, double* __restrict__ csum,
     int js, unsigned char* __restrict__ lds, int bid, int nblk) {
     using A = typename Arith<T>::A;
@@ -231,6 +238,20 @@ __device__ __forceinline__ void k_cell_sum_body(
     if (L >= c.nx * c.ny) return;
     const int cell = tiled_cell(L, c.nx, c.ny);
     const size_t o = (size_t)cell * c.ldv;
+    // the slope stencils, as k_slope2d takes them
+    const A smv = (A)c.smv;
+    bool inner[2];
+    A inv_den[2];
+    size_t on[2][2];
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+        const SlopeNb nb = slope_nb(c, cell, dir);
+        const A len = (A)(dir == 0 ? c.dx : c.dy);
+        inner[dir] = nb.inner;
+        inv_den[dir] = (A)1 / ((A)0.5 * len + (A)0.5 * len);
+        on[dir][0] = (size_t)nb.cl * c.ldv;
+        on[dir][1] = (size_t)nb.cr * c.ldv;
+    }
     const int nv = c.nv;
     const int CR = cs_chunk_rows(nv, (int)sizeof(T));
     const int cw = CR * nv;                  // LDS elements per array
@@ -272,10 +293,34 @@ __device__ __forceinline__ void k_cell_sum_body(
                 // bounds) and do not store it
                 const Vec x0 = *reinterpret_cast<const Vec*>(h + g);
                 const Vec x1 = *reinterpret_cast<const Vec*>(b + g);
-                const Vec x2 = *reinterpret_cast<const Vec*>(shx + g);
-                const Vec x3 = *reinterpret_cast<const Vec*>(shy + g);
-                const Vec x4 = *reinterpret_cast<const Vec*>(sbx + g);
-                const Vec x5 = *reinterpret_cast<const Vec*>(sby + g);
+                // the slopes from the five-cell stencils, as k_slope2d forms
+                // them, stored as it stores them
+                Vec x2, x3, x4, x5;
+                {
+                    const size_t gr = g - o;
+                    const Vec hxl = *reinterpret_cast<const Vec*>(h + on[0][0] + gr);
+                    const Vec hxr = *reinterpret_cast<const Vec*>(h + on[0][1] + gr);
+                    const Vec hyl = *reinterpret_cast<const Vec*>(h + on[1][0] + gr);
+                    const Vec hyr = *reinterpret_cast<const Vec*>(h + on[1][1] + gr);
+                    const Vec bxl = *reinterpret_cast<const Vec*>(b + on[0][0] + gr);
+                    const Vec bxr = *reinterpret_cast<const Vec*>(b + on[0][1] + gr);
+                    const Vec byl = *reinterpret_cast<const Vec*>(b + on[1][0] + gr);
+                    const Vec byr = *reinterpret_cast<const Vec*>(b + on[1][1] + gr);
+#pragma unroll
+                    for (int q = 0; q < VW; ++q) {
+                        x2[q] = (T)vl_slope((A)hxl[q], (A)x0[q], (A)hxr[q], inner[0], inv_den[0], smv);
+                        x3[q] = (T)vl_slope((A)hyl[q], (A)x0[q], (A)hyr[q], inner[1], inv_den[1], smv);
+                        x4[q] = (T)vl_slope((A)bxl[q], (A)x1[q], (A)bxr[q], inner[0], inv_den[0], smv);
+                        x5[q] = (T)vl_slope((A)byl[q], (A)x1[q], (A)byr[q], inner[1], inv_den[1], smv);
+                    }
+                    if (in) {
+                        // streamed out: read back only by the face kernels
+                        __builtin_nontemporal_store(x2, reinterpret_cast<Vec*>(shx + g));
+                        __builtin_nontemporal_store(x3, reinterpret_cast<Vec*>(shy + g));
+                        __builtin_nontemporal_store(x4, reinterpret_cast<Vec*>(sbx + g));
+                        __builtin_nontemporal_store(x5, reinterpret_cast<Vec*>(sby + g));
+                    }
+                }
                 if (t == 0) {
                     // the chunk rows' v and weight factor, with the loads
                     const int jr = lane < cr ? j0 + lane : j0;
@@ -296,7 +341,7 @@ __device__ __forceinline__ void k_cell_sum_body(
                 }
             }
             wave_lds_sync();
-#pragma unroll 2
+#pragma unroll 1
             for (int rr = 0; rr < cr; rr += R) {
                 const int jj = rr + r;
                 if (!act || jj >= cr) continue;
@@ -364,8 +409,8 @@ __device__ __forceinline__ void k_cell_sum_body(
 template <typename T, int VW>
 __global__ __launch_bounds__(WAVE * SLOPE_CPW) void k_cell_sum(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
-    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    T* __restrict__ shx, T* __restrict__ shy, T* __restrict__ sbx, T* __restrict__ sby,
+    const double* __restrict__ u1, const double* __restrict__ v1,
     const double* __restrict__ wx, const double* __restrict__ wy, double* __restrict__ csum,
     int js) {
     __shared__ __align__(16) unsigned char lds[SLOPE_CPW * cs_lds_per_wave<T>()];

@@ -11,7 +11,7 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     const Consts& c = L.c;
     const size_t ncell = (size_t)c.nx * c.ny;
     const bool cs = L.flux_mode == HUGKS_FLUX_CELLSUM;
-    if (L.flux_mode != HUGKS_FLUX_STENCIL) {
+    if (L.flux_mode != HUGKS_FLUX_STENCIL && !cs) {
         HookScope hs(L, "slope2d");
         hipLaunchKernelGGL((k_slope2d<T>), dim3((unsigned)((ncell + SLOPE_CPW - 1) / SLOPE_CPW)),
                            dim3(WAVE * SLOPE_CPW), 0, s, c,
@@ -21,12 +21,13 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
     if (cs) {
         // 16-byte pieces where a row is a multiple of 16 bytes
         constexpr int VW = 16 / (int)sizeof(T);
-        HookScope hs(L, "cell_sum");
+        // the slopes and the per-cell sums in one pass
+        HookScope hs(L, "slope_sum");
         auto kc = (c.nv * (int)sizeof(T)) % 16 == 0 ? k_cell_sum<T, VW> : k_cell_sum<T, 1>;
         hipLaunchKernelGGL(kc, dim3((unsigned)((ncell + SLOPE_CPW - 1) / SLOPE_CPW)),
                            dim3(WAVE * SLOPE_CPW), 0, s, c, L.st, (const T*)L.h, (const T*)L.b,
-                           (const T*)L.shx, (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby,
-                           L.u1, L.v1, L.wx, L.wy, L.csum, L.cs_js);
+                           (T*)L.shx, (T*)L.shy, (T*)L.sbx, (T*)L.sby, L.u1, L.v1, L.wx, L.wy,
+                           L.csum, L.cs_js);
     } else if (L.flux_mode == HUGKS_FLUX_STENCIL && L.nwall) {
         // the wall fluxes read the slope arrays of the wall cells
         HookScope hs(L, "slope2d");
