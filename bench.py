@@ -377,7 +377,7 @@ def bench_c4(a):
     # roofline of the dominant grouped kernel: algorithmic bytes of one
     # launch (summed over the cases it steps) over its average duration
     kmap = {"kg_slope": "slope2d", "kg_fsum": "flux_sum", "kg_fnode": "flux_node",
-            "kg_update": "update2d"}
+            "kg_update": "update2d", "kg_cell_sum": "slope_sum", "kg_fcombine": "flux_combine"}
     roof = None
     kt = rm["kt"]
     timed = [k for k in kt if k in kmap]

@@ -1333,6 +1333,12 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
     GroupPlan P{};
     if (upd.size() > sizeof(P.upd) / sizeof(P.upd[0])) return hipErrorInvalidValue;
     P.fp64 = b->cases[ids[0]]->fp64;
+    P.cellsum = b->cases[ids[0]]->L.flux_mode == HUGKS_FLUX_CELLSUM;
+    P.cs_vec = true;
+    for (int i : ids) {
+        const int es = b->cases[i]->fp64 ? 8 : 4;
+        P.cs_vec = P.cs_vec && (b->cases[i]->L.c.nv * es) % 16 == 0;
+    }
     const int k_ts = table(one), k_sl = table(sl), k_fs = table(fs), k_ff = table(ff),
               k_fn = table(fn), k_wl = wl.empty() ? -1 : table(wl), k_cp = table(cp);
     std::vector<int> k_up;
@@ -1389,13 +1395,18 @@ static hipError_t batch_group_plan(hugks_batch* b) {
     b->gmem.push_back(lmem);
     std::vector<Launch2D> Ls(n);
     std::vector<int> ids[2];
-    for (int i = 0; i < n; ++i) {
-        if (b->in_group[i]) {
-            hugks_set_schedule(b->cases[i], HUGKS_FLUX_SPLIT, HUGKS_UPDATE_MFMA_NODE);
-            ids[b->cases[i]->fp64 ? 1 : 0].push_back(i);
-        }
-        Ls[i] = b->cases[i]->L;
+    for (int i = 0; i < n; ++i)
+        if (b->in_group[i]) ids[b->cases[i]->fp64 ? 1 : 0].push_back(i);
+    // the grouped schedule: the cell-sum flux where every case of the
+    // storage type supports it, else the split flux; the MFMA node update
+    for (int t = 0; t < 2; ++t) {
+        bool cs = true;
+        for (int i : ids[t]) cs = cs && b->cases[i]->L.cs_js >= 0;
+        for (int i : ids[t])
+            hugks_set_schedule(b->cases[i], cs ? HUGKS_FLUX_CELLSUM : HUGKS_FLUX_SPLIT,
+                               HUGKS_UPDATE_MFMA_NODE);
     }
+    for (int i = 0; i < n; ++i) Ls[i] = b->cases[i]->L;
     e = hipMemcpy(lmem, Ls.data(), n * sizeof(Launch2D), hipMemcpyHostToDevice);
     for (int t = 0; t < 2 && e == hipSuccess; ++t)
         if (!ids[t].empty()) e = batch_group_plan_of(b, ids[t], static_cast<const Launch2D*>(lmem));

@@ -115,6 +115,8 @@ struct GroupUpd {
 };
 struct GroupPlan {
     bool fp64;
+    bool cellsum;        // the cell-sum pipeline (every case supports it), else the split one
+    bool cs_vec;         // its 16-byte loads (every case's velocity row a multiple of 16 bytes)
     const Launch2D* Ls;  // per-case arguments, device memory
     const int *t_ts, *t_slope, *t_fsum, *t_fface, *t_fnode, *t_wall, *t_cpre, *t_fin;
     int g_ts, g_slope, g_fsum, g_fface, g_fnode, g_wall, g_cpre, g_fin;

@@ -9,6 +9,7 @@
 #pragma once
 #include "k_flux2d.h"
 #include "k_flux_split.h"
+#include "k_flux_cellsum.h"
 #include "k_misc.h"
 #include "k_update2d_mx.h"
 
@@ -70,6 +71,31 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void kg_fsum(const int* __restric
                           (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby, L.unode_d,
                           L.vnode_d, L.wnode_d, L.node4, L.acc_v, L.acc_h, L.fsum, nvf, nint,
                           g.bid, g.nblk);
+}
+
+// the cell-sum pipeline (every case of the plan supports it): slopes and
+// per-cell sums, then each face's sums from two bins
+template <typename T, int VW>
+__global__ __launch_bounds__(WAVE * SLOPE_CPW) void kg_cell_sum(const int* __restrict__ tab,
+                                                                const Launch2D* __restrict__ Ls) {
+    __shared__ __align__(16) unsigned char lds[SLOPE_CPW * cs_lds_per_wave<T>()];
+    const GroupIdx g = group_index(tab);
+    const Launch2D& L = Ls[g.cs];
+    k_cell_sum_body<T, VW>(L.c, L.st, (const T*)L.h, (const T*)L.b, (T*)L.shx, (T*)L.shy,
+                           (T*)L.sbx, (T*)L.sby, L.u1, L.v1, L.wx, L.wy, L.csum, L.cs_js, lds,
+                           g.bid, g.nblk);
+}
+
+template <typename T>
+__global__ __launch_bounds__(WAVE * SPLIT_FPW) void kg_fcombine(const int* __restrict__ tab,
+                                                                const Launch2D* __restrict__ Ls) {
+    const GroupIdx g = group_index(tab);
+    const Launch2D& L = Ls[g.cs];
+    int nvf, nint;
+    face_counts(L.c, nvf, nint);
+    k_fcombine_body<T>(L.c, L.st, (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
+                       (const T*)L.sbx, (const T*)L.sby, L.u1, L.v1, L.unode_d, L.vnode_d,
+                       L.wnode_d, L.node4, L.acc_v, L.acc_h, L.csum, L.fsum, nvf, nint, g.bid);
 }
 
 __global__ __launch_bounds__(256) void kg_fface(const int* __restrict__ tab,

@@ -57,15 +57,30 @@ static hipError_t group_step_t(const GroupPlan& P, hipStream_t s) {
         HookCall hc(P.hook, P.hook_ctx, "kg_timestep");
         hipLaunchKernelGGL((kg_timestep), dim3(P.g_ts), dim3(RED_NT), 0, s, P.t_ts, P.Ls);
     }
-    {
-        HookCall hc(P.hook, P.hook_ctx, "kg_slope");
-        hipLaunchKernelGGL((kg_slope<T>), dim3(P.g_slope), dim3(WAVE * SLOPE_CPW), 0, s,
-                           P.t_slope, P.Ls);
-    }
-    {
-        HookCall hc(P.hook, P.hook_ctx, "kg_fsum");
-        hipLaunchKernelGGL((kg_fsum<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s, P.t_fsum,
-                           P.Ls);
+    if (P.cellsum) {
+        {
+            constexpr int VW = 16 / (int)sizeof(T);
+            HookCall hc(P.hook, P.hook_ctx, "kg_cell_sum");
+            auto kc = P.cs_vec ? kg_cell_sum<T, VW> : kg_cell_sum<T, 1>;
+            hipLaunchKernelGGL(kc, dim3(P.g_slope), dim3(WAVE * SLOPE_CPW), 0, s, P.t_slope,
+                               P.Ls);
+        }
+        {
+            HookCall hc(P.hook, P.hook_ctx, "kg_fcombine");
+            hipLaunchKernelGGL((kg_fcombine<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s,
+                               P.t_fsum, P.Ls);
+        }
+    } else {
+        {
+            HookCall hc(P.hook, P.hook_ctx, "kg_slope");
+            hipLaunchKernelGGL((kg_slope<T>), dim3(P.g_slope), dim3(WAVE * SLOPE_CPW), 0, s,
+                               P.t_slope, P.Ls);
+        }
+        {
+            HookCall hc(P.hook, P.hook_ctx, "kg_fsum");
+            hipLaunchKernelGGL((kg_fsum<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s,
+                               P.t_fsum, P.Ls);
+        }
     }
     {
         HookCall hc(P.hook, P.hook_ctx, "kg_fface");

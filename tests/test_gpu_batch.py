@@ -156,16 +156,21 @@ def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
     cs = grouped_cases(capi.MIXED_FP32)
     with capi.Batch(cs) as b:
         assert b.grouped
+        sched = b.solvers[0].schedule()
+        assert all(s.schedule() == sched for s in b.solvers)
+        assert sched in ((capi.FLUX_CELLSUM, capi.UPDATE_MFMA_NODE),
+                         (capi.FLUX_SPLIT, capi.UPDATE_MFMA_NODE))
         b.advance(5)
         wg = [s.macro(with_q=False) for s in b.solvers]
-    # case by case with the schedule of the grouped launches (split flux,
-    # node update), which the batch's graphs capture at create
+    # case by case with the schedule of the grouped launches, which the
+    # batch's graphs capture at create
     monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
-    monkeypatch.setenv("HUGKS_FLUX_KERNEL", "split")
+    monkeypatch.setenv("HUGKS_FLUX_KERNEL",
+                       "cellsum" if sched[0] == capi.FLUX_CELLSUM else "split")
     with capi.Batch(cs) as b:
         assert not b.grouped
         for s in b.solvers:
-            assert s.schedule() == (capi.FLUX_SPLIT, capi.UPDATE_MFMA_NODE)
+            assert s.schedule() == sched
         b.advance(5)
         wl = [s.macro(with_q=False) for s in b.solvers]
     monkeypatch.delenv("HUGKS_BATCH_GROUPED")
