@@ -134,6 +134,7 @@ struct hugks_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t ncell = 0;
     uint64_t run_hash = 0;   // of the inputs that define the run (checkpoints)
+    bool in_batch = false;   // a batch's graphs hold this case's schedule
 };
 
 // FNV-1a, 64-bit, over byte ranges.
@@ -1046,6 +1047,8 @@ int hugks_set_timing(hugks_handle* h, int32_t on) {
 
 int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kernel) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    if (h->in_batch)
+        return fail(h, HUGKS_E_ARG, "the case belongs to a batch, whose graphs fix its schedule");
     if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_CELLSUM ||
         update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA_NODE)
         return fail(h, HUGKS_E_ARG, "unknown kernel choice");
@@ -1074,6 +1077,8 @@ int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kern
 
 int hugks_set_update_waves(hugks_handle* h, int32_t waves) {
     if (!h) return fail(nullptr, HUGKS_E_ARG, "null handle");
+    if (h->in_batch)
+        return fail(h, HUGKS_E_ARG, "the case belongs to a batch, whose graphs fix its schedule");
     if (h->dim != 2) return HUGKS_OK;
     const int nw = waves > 0 ? waves : update2d_mx_default_waves(h->L.c);
     if (nw > 16 || !update2d_mx_supported(h->L.c, h->fp64, h->b3, h->L.lds_limit, nw))
@@ -1522,6 +1527,8 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
         batch_release(b);
         return batch_fail(nullptr, e == hipErrorOutOfMemory ? HUGKS_E_NOMEM : HUGKS_E_DEVICE, m);
     }
+    // the graphs hold each case's launches: its schedule is fixed from here
+    for (hugks_handle* h : b->cases) h->in_batch = true;
     *out = b;
     return HUGKS_OK;
 }

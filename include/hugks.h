@@ -297,6 +297,9 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
 #define HUGKS_UPDATE_LDS   0
 #define HUGKS_UPDATE_MFMA  1
 #define HUGKS_UPDATE_MFMA_NODE 2
+/* A handle that belongs to a batch (hugks_batch_handle) keeps the schedule
+ * and tile its batch captured at create: setting either returns
+ * HUGKS_E_ARG. */
 int hugks_set_schedule(hugks_handle *h, int32_t flux_kernel, int32_t update_kernel);
 int hugks_get_schedule(hugks_handle *h, int32_t *flux_kernel, int32_t *update_kernel);
 /* Tile parameter of HUGKS_UPDATE_MFMA_NODE: wavefronts per cell, 1 to 16

@@ -236,3 +236,20 @@ def test_tune_tiles_picks_a_compiled_tile(have_gpu):
             assert s.update_waves() == nw
     again = capi.tune_tiles(c)
     assert again[:3] == (fk, uk, nw) and again[3] == -1.0
+
+
+def test_batch_cases_refuse_schedule_changes(have_gpu):
+    """A batch's graphs hold its cases' launches from create: setting the
+    schedule or the node-update tile of one of its handles is refused
+    (the schedules they report stay those of the graphs)."""
+    cs = grouped_cases(capi.MIXED_FP32)
+    with capi.Batch(cs) as b:
+        s = b.solvers[0]
+        before = s.schedule()
+        with pytest.raises(capi.HugksError) as ei:
+            s.set_schedule(capi.FLUX_SPLIT, capi.UPDATE_LDS)
+        assert ei.value.code == capi.E_ARG
+        with pytest.raises(capi.HugksError) as ei:
+            s.set_update_waves(1)
+        assert ei.value.code == capi.E_ARG
+        assert s.schedule() == before
