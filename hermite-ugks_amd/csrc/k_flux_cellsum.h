@@ -294,7 +294,10 @@ __device__ __forceinline__ void k_cell_sum_body(
                 const Vec x0 = *reinterpret_cast<const Vec*>(h + g);
                 const Vec x1 = *reinterpret_cast<const Vec*>(b + g);
                 // the slopes from the five-cell stencils, as k_slope2d forms
-                // them, stored as it stores them
+                // them but with the limiter's quotient from the hardware
+                // reciprocal for FP32 (within 1 ulp; the IEEE division was
+                // a third of this kernel's VALU instructions: 4.0 -> 3.8 ms
+                // at 512^2 cells, GH40), stored as k_slope2d stores them
                 Vec x2, x3, x4, x5;
                 {
                     const size_t gr = g - o;
@@ -308,10 +311,14 @@ __device__ __forceinline__ void k_cell_sum_body(
                     const Vec byr = *reinterpret_cast<const Vec*>(b + on[1][1] + gr);
 #pragma unroll
                     for (int q = 0; q < VW; ++q) {
-                        x2[q] = (T)vl_slope((A)hxl[q], (A)x0[q], (A)hxr[q], inner[0], inv_den[0], smv);
-                        x3[q] = (T)vl_slope((A)hyl[q], (A)x0[q], (A)hyr[q], inner[1], inv_den[1], smv);
-                        x4[q] = (T)vl_slope((A)bxl[q], (A)x1[q], (A)bxr[q], inner[0], inv_den[0], smv);
-                        x5[q] = (T)vl_slope((A)byl[q], (A)x1[q], (A)byr[q], inner[1], inv_den[1], smv);
+                        x2[q] = (T)vl_slope<A, true>((A)hxl[q], (A)x0[q], (A)hxr[q], inner[0],
+                                                     inv_den[0], smv);
+                        x3[q] = (T)vl_slope<A, true>((A)hyl[q], (A)x0[q], (A)hyr[q], inner[1],
+                                                     inv_den[1], smv);
+                        x4[q] = (T)vl_slope<A, true>((A)bxl[q], (A)x1[q], (A)bxr[q], inner[0],
+                                                     inv_den[0], smv);
+                        x5[q] = (T)vl_slope<A, true>((A)byl[q], (A)x1[q], (A)byr[q], inner[1],
+                                                     inv_den[1], smv);
                     }
                     if (in) {
                         // streamed out: read back only by the face kernels
