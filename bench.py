@@ -85,8 +85,6 @@ def algorithmic_bytes(case, kernel):
         return 6 * s * nint * nn + 8 * 37 * nint
     if kernel == "flux_node":        # read 6 upwind arrays, write 2 face arrays per face node
         return 8 * s * nint * nn
-    if kernel == "flux_combine":     # read two bins of 37 sums per face, write 37 sums
-        return 3 * 8 * 37 * nint
     return None
 
 
@@ -258,7 +256,7 @@ def rocprof_summary(dirs):
 TIMER_OF = [("k_update2d_mx", "update2d"), ("k_update2d_mfma", "update2d"),
             ("k_update2d<", "update2d"), ("k_fsum", "flux_sum"), ("k_fnode", "flux_node"),
             ("k_fface", "flux_face"), ("k_slope2d", "slope2d"), ("k_flux_wall", "flux_wall"),
-            ("k_cell_sum", "slope_sum"), ("k_fcombine", "flux_combine"),
+            ("k_cell_sum", "slope_sum"),
             ("k_cell_pre", "cell_pre"), ("k_finalize", "finalize"),
             ("k_part_sum", "finalize_partials"), ("k_rate_part", "timestep_rates"),
             ("k_timestep", "timestep")]
@@ -377,7 +375,7 @@ def bench_c4(a):
     # roofline of the dominant grouped kernel: algorithmic bytes of one
     # launch (summed over the cases it steps) over its average duration
     kmap = {"kg_slope": "slope2d", "kg_fsum": "flux_sum", "kg_fnode": "flux_node",
-            "kg_update": "update2d", "kg_cell_sum": "slope_sum", "kg_fcombine": "flux_combine"}
+            "kg_update": "update2d", "kg_cell_sum": "slope_sum"}
     roof = None
     kt = rm["kt"]
     timed = [k for k in kt if k in kmap]

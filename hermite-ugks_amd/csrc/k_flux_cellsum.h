@@ -18,9 +18,10 @@
 //                 Leer slopes as k_slope2d forms and stores them, then the
 //                 four half-space bins of the cell's 37 sums, in FP64,
 //                 from h, b and the slopes in LDS
-//   k_fcombine    one wavefront per interior face: the face's sums as the
-//                 sum of two bins, or, where a node flips, K1's own pass
-//                 over the face (fsum_face)
+//   k_fface_cs    one lane per interior face: the face's sums as the sum
+//                 of two bins, or, where a node flips, K1's own pass over
+//                 the face (fsum_face, by the lane's wavefront), then the
+//                 face state, Maxwellian sums and conserved flux as k_fface
 //
 // Lane mapping of k_cell_sum: lane l takes column iu = l % nv of row
 // jv = j0 + l / nv (64 / nv rows per pass; nv <= 64), so each lane keeps
@@ -50,9 +51,15 @@
 
 namespace hugks {
 
-// bins of a cell: 0 u < 0, 1 u >= 0, 2 v < 0, 3 v >= 0 (as upwind_delta)
+// bins of a cell: 0 u < 0, 1 u >= 0, 2 v < 0, 3 v >= 0 (as upwind_delta),
+// stored by cell: csum[(cell * CSUM_BINS + bin) * CELL_SUM_LD + q] (stored
+// by sum instead, with the cells fastest, the bin stores of the cell
+// kernel scatter: slopes and sums 3.71 -> 3.86 ms FP32, 5.9 -> 6.4 ms FP64
+// at 512^2 cells, GH40)
 constexpr int CSUM_BINS = 4;
-constexpr int CSUM_LD = CELL_SUM_LD;
+__host__ __device__ inline size_t csum_at(int bin, int q, size_t cell) {
+    return (cell * CSUM_BINS + (size_t)bin) * CELL_SUM_LD + (size_t)q;
+}
 
 // The 37 sums of a vertical face (vn = u, vt = v) from a lane's
 // accumulators over its column: x[0..3] = sum a v^b, x[4..6] = e v^b,
@@ -165,10 +172,10 @@ __device__ __forceinline__ void cs_expand_h(const double (&x)[18], double u,
 
 // One bin: the 37 sums expanded from a lane's accumulators (vertical V,
 // else horizontal) and reduced over the wavefront in two halves (fewer
-// values live at once), written to out[0..36].
+// values live at once), written to out[q * stride], q = 0..36.
 template <bool V, int N>
 __device__ __forceinline__ void cs_reduce_bin(const double (&x0)[N], double u, double wu, bool on,
-                                              double* __restrict__ out) {
+                                              double* __restrict__ out, size_t stride) {
     // the column's weight factor, left out of the per-node products
     double x[N];
 #pragma unroll
@@ -190,13 +197,13 @@ __device__ __forceinline__ void cs_reduce_bin(const double (&x0)[N], double u, d
 #pragma unroll
             for (int i = 0; i < H1; ++i) t[i] = s[i];
             const double y = wave_reduce_scatter<H1>(t, q);
-            if (q >= 0) out[q] = y;
+            if (q >= 0) out[(size_t)q * stride] = y;
         } else {
             double t[H2];
 #pragma unroll
             for (int i = 0; i < H2; ++i) t[i] = s[H1 + i];
             const double y = wave_reduce_scatter<H2>(t, q);
-            if (q >= 0) out[H1 + q] = y;
+            if (q >= 0) out[(size_t)(H1 + q) * stride] = y;
         }
     }
 }
@@ -271,7 +278,6 @@ __device__ __forceinline__ void k_cell_sum_body(
     for (int q = 0; q < 16; ++q) av[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < 18; ++q) ah[q] = 0.0;
-    double* out = csum + (size_t)cell * CSUM_BINS * CSUM_LD;
     typedef T Vec __attribute__((ext_vector_type(VW)));
 #pragma unroll 1
     for (int p = 0; p < 2; ++p) {
@@ -404,13 +410,13 @@ __device__ __forceinline__ void k_cell_sum_body(
             }
         }
         // the horizontal bin of this pass's rows
-        cs_reduce_bin<false>(ah, u, wu, true, out + (2 + p) * CSUM_LD);
+        cs_reduce_bin<false>(ah, u, wu, true, csum + csum_at(2 + p, 0, cell), 1);
 #pragma unroll
         for (int t = 0; t < 18; ++t) ah[t] = 0.0;
     }
 #pragma unroll 1
     for (int side = 0; side < 2; ++side)
-        cs_reduce_bin<true>(av, u, wu, act && su == side, out + side * CSUM_LD);
+        cs_reduce_bin<true>(av, u, wu, act && su == side, csum + csum_at(side, 0, cell), 1);
 }
 #ifndef HUGKS_BODIES_ONLY
 template <typename T, int VW>
@@ -426,57 +432,103 @@ __global__ __launch_bounds__(WAVE * SLOPE_CPW) void k_cell_sum(
 }
 #endif
 
-// Sums of interior face g: the left cell's forward bin plus the right
-// cell's backward bin, or K1's pass over the face when a node flips.
+// Whether a node of face fg changes upwind side between vn and
+// vn - fn dt/2 (a "flip"): then the face's sums are not the sum of two
+// bins.  The test runs in the arithmetic type, as the upwind choice of
+// fsum_face and fnode_face does.
+template <typename A>
+__device__ __forceinline__ bool face_flips(const Consts& c, const FaceGeo& fg,
+                                           const double* __restrict__ u1,
+                                           const double* __restrict__ v1,
+                                           const double* __restrict__ acc_v,
+                                           const double* __restrict__ acc_h, double dt) {
+    const double* acc = fg.dir == 0 ? acc_v : acc_h;
+    const double* x = fg.dir == 0 ? u1 : v1;
+    const A vs = (A)(0.5 * acc[2 * fg.face] * dt);
+    bool f = false;
+    for (int i = 0; i < c.nv; ++i) {
+        const A xi = (A)x[i];
+        f = f || upwind_delta(xi - vs) != upwind_delta(xi);
+    }
+    return f;
+}
+
+// K2, K3 (tensor-product quadrature) and K4 of one face on one lane, as
+// k_fface, with the face's 37 sums taken first: its left cell's forward
+// bin plus its right cell's backward bin, or, for a face with a flip,
+// K1's pass over the face (fsum_face) by the whole wavefront, for each
+// such face of the wavefront in turn.
 template <typename T>
-__device__ __forceinline__ void k_fcombine_body(
-    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
-    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+__device__ __forceinline__ void k_fface_cs_body(
+    Consts c, const DevState* __restrict__ st, const double* __restrict__ W,
+    const T* __restrict__ h, const T* __restrict__ b, const T* __restrict__ shx,
+    const T* __restrict__ shy, const T* __restrict__ sbx, const T* __restrict__ sby,
+    const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ wx, const double* __restrict__ wy,
     const double* __restrict__ und, const double* __restrict__ vnd,
     const double* __restrict__ wnd, const double* __restrict__ node4,
     const double* __restrict__ acc_v, const double* __restrict__ acc_h,
-    const double* __restrict__ csum, double* __restrict__ fsum, int nvi, int nint, int bid) {
+    const double* __restrict__ csum, double* __restrict__ fsum, double* __restrict__ fcoef,
+    double* __restrict__ vflux, double* __restrict__ hflux, int nvi, int nint, int bid) {
     using A = typename Arith<T>::A;
     if (st->stopped) return;
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int g = __builtin_amdgcn_readfirstlane(bid * SPLIT_FPW + (int)(threadIdx.x / WAVE));
-    if (g >= nint) return;
-    const FaceGeo fg = face_geo(c, g, nvi);
-    const double* acc = fg.dir == 0 ? acc_v : acc_h;
-    const double vshift = 0.5 * acc[2 * fg.face] * st->dt;
-    const A vs = (A)vshift;
+    const int g = bid * 256 + (int)threadIdx.x;
+    const bool have = g < nint;
+    const double dt = st->dt;
+    FaceGeo fg{};
     bool flip = false;
-    if (lane < c.nv) {
-        const A x = (A)(fg.dir == 0 ? u1[lane] : v1[lane]);
-        flip = upwind_delta(x - vs) != upwind_delta(x);
+    if (have) {
+        fg = face_geo(c, g, nvi);
+        flip = face_flips<A>(c, fg, u1, v1, acc_v, acc_h, dt);
     }
-    double* out = fsum + (size_t)g * FSUM_LD;
-    if (__ballot(flip) != 0) {
-        const Upwind<T, false> up = make_upwind<T, false>(c, fg, h, b, shx, shy, sbx, sby);
-        const NodeTabs nt{und, vnd, wnd, node4};
-        if (fg.dir == 0)
-            fsum_face<0, T, false>(c, vshift, up, nt, out);
-        else
-            fsum_face<1, T, false>(c, vshift, up, nt, out);
-        return;
+    // the wavefront's faces with a flip, one after the other, by all lanes
+    unsigned long long fm = __ballot(flip);
+    if (fm != 0) {
+        while (fm != 0) {
+            const int l = __builtin_ctzll(fm);
+            fm &= fm - 1;
+            const int gf = __shfl(g, l, WAVE);
+            const FaceGeo ff = face_geo(c, gf, nvi);
+            const double* acc = ff.dir == 0 ? acc_v : acc_h;
+            const double vshift = 0.5 * acc[2 * ff.face] * dt;
+            const Upwind<T, false> up = make_upwind<T, false>(c, ff, h, b, shx, shy, sbx, sby);
+            const NodeTabs nt{und, vnd, wnd, node4};
+            if (ff.dir == 0)
+                fsum_face<0, T, false>(c, vshift, up, nt, fsum + (size_t)gf * FSUM_LD);
+            else
+                fsum_face<1, T, false>(c, vshift, up, nt, fsum + (size_t)gf * FSUM_LD);
+        }
+        __threadfence();   // those sums, written by other lanes, before their faces read them
     }
-    const double* bL = csum + ((size_t)fg.cL * CSUM_BINS + (fg.dir ? 3 : 1)) * CSUM_LD;
-    const double* bR = csum + ((size_t)fg.cR * CSUM_BINS + (fg.dir ? 2 : 0)) * CSUM_LD;
-    if (lane < FSUM_N) out[lane] = bL[lane] + bR[lane];
+    if (!have) return;
+    if (!flip) {
+        const double* bL = csum + csum_at(fg.dir ? 3 : 1, 0, (size_t)fg.cL);
+        const double* bR = csum + csum_at(fg.dir ? 2 : 0, 0, (size_t)fg.cR);
+        double* out = fsum + (size_t)g * FSUM_LD;
+#pragma unroll
+        for (int q = 0; q < FSUM_N; ++q) out[q] = bL[q] + bR[q];
+    }
+    fstate_lane(c, dt, W, acc_v, acc_h, fsum, fcoef, fg, g);
+    double s2[FMAXW_N];
+    fmaxw_sep_lane(c, u1, v1, wx, wy, acc_v, acc_h, fcoef, fg, g, s2);
+    fslope_t_lane(c, acc_v, acc_h, s2, fcoef, vflux, hflux, fg, g);
 }
 #ifndef HUGKS_BODIES_ONLY
 template <typename T>
-__global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fcombine(
-    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
-    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
-    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+__global__ __launch_bounds__(256) void k_fface_cs(
+    Consts c, const DevState* __restrict__ st, const double* __restrict__ W,
+    const T* __restrict__ h, const T* __restrict__ b, const T* __restrict__ shx,
+    const T* __restrict__ shy, const T* __restrict__ sbx, const T* __restrict__ sby,
+    const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ wx, const double* __restrict__ wy,
     const double* __restrict__ und, const double* __restrict__ vnd,
     const double* __restrict__ wnd, const double* __restrict__ node4,
     const double* __restrict__ acc_v, const double* __restrict__ acc_h,
-    const double* __restrict__ csum, double* __restrict__ fsum, int nvi, int nint) {
-    k_fcombine_body<T>(c, st, h, b, shx, shy, sbx, sby, u1, v1, und, vnd, wnd, node4, acc_v,
-                       acc_h, csum, fsum, nvi, nint, (int)blockIdx.x);
+    const double* __restrict__ csum, double* __restrict__ fsum, double* __restrict__ fcoef,
+    double* __restrict__ vflux, double* __restrict__ hflux, int nvi, int nint) {
+    k_fface_cs_body<T>(c, st, W, h, b, shx, shy, sbx, sby, u1, v1, wx, wy, und, vnd, wnd, node4,
+                       acc_v, acc_h, csum, fsum, fcoef, vflux, hflux, nvi, nint,
+                       (int)blockIdx.x);
 }
 #endif
 

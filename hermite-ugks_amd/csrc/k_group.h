@@ -87,15 +87,16 @@ __global__ __launch_bounds__(WAVE * SLOPE_CPW) void kg_cell_sum(const int* __res
 }
 
 template <typename T>
-__global__ __launch_bounds__(WAVE * SPLIT_FPW) void kg_fcombine(const int* __restrict__ tab,
-                                                                const Launch2D* __restrict__ Ls) {
+__global__ __launch_bounds__(256) void kg_fface_cs(const int* __restrict__ tab,
+                                                   const Launch2D* __restrict__ Ls) {
     const GroupIdx g = group_index(tab);
     const Launch2D& L = Ls[g.cs];
     int nvf, nint;
     face_counts(L.c, nvf, nint);
-    k_fcombine_body<T>(L.c, L.st, (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
-                       (const T*)L.sbx, (const T*)L.sby, L.u1, L.v1, L.unode_d, L.vnode_d,
-                       L.wnode_d, L.node4, L.acc_v, L.acc_h, L.csum, L.fsum, nvf, nint, g.bid);
+    k_fface_cs_body<T>(L.c, L.st, L.W, (const T*)L.h, (const T*)L.b, (const T*)L.shx,
+                       (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby, L.u1, L.v1, L.wx, L.wy,
+                       L.unode_d, L.vnode_d, L.wnode_d, L.node4, L.acc_v, L.acc_h, L.csum,
+                       L.fsum, L.fcoef, L.vflux, L.hflux, nvf, nint, g.bid);
 }
 
 __global__ __launch_bounds__(256) void kg_fface(const int* __restrict__ tab,

@@ -51,12 +51,14 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
         const unsigned gw = (unsigned)((nint + SPLIT_FPW - 1) / SPLIT_FPW);
         const unsigned gl = (unsigned)((nint + 255) / 256);
         if (cs) {
-            HookScope hs(L, "flux_combine");
-            hipLaunchKernelGGL((k_fcombine<T>), dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
+            // the face sums from the cell bins inside the lane-per-face
+            // kernel (the cell-sum flux needs a tensor-product quadrature)
+            HookScope hs(L, "flux_face");
+            hipLaunchKernelGGL((k_fface_cs<T>), dim3(gl), dim3(256), 0, s, c, L.st, L.W,
                                (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
-                               (const T*)L.sbx, (const T*)L.sby, L.u1, L.v1, L.unode_d,
-                               L.vnode_d, L.wnode_d, L.node4, L.acc_v, L.acc_h, L.csum, L.fsum,
-                               nvf, nint);
+                               (const T*)L.sbx, (const T*)L.sby, L.u1, L.v1, L.wx, L.wy,
+                               L.unode_d, L.vnode_d, L.wnode_d, L.node4, L.acc_v, L.acc_h,
+                               L.csum, L.fsum, L.fcoef, L.vflux, L.hflux, nvf, nint);
         } else {
             HookScope hs(L, "flux_sum");
             auto kf = stn ? k_fsum<T, true> : k_fsum<T, false>;
@@ -65,14 +67,14 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
                                (const T*)L.sbx, (const T*)L.sby, L.unode_d, L.vnode_d,
                                L.wnode_d, L.node4, L.acc_v, L.acc_h, L.fsum, nvf, nint);
         }
-        if (L.wx) {
+        if (!cs && L.wx) {
             // tensor-product quadrature: face state, Maxwellian sums and
             // conserved flux in one lane-per-face kernel
             HookScope hs(L, "flux_face");
             hipLaunchKernelGGL((k_fface), dim3(gl), dim3(256), 0, s, c, L.st, L.W, L.u1, L.v1,
                                L.wx, L.wy, L.acc_v, L.acc_h, L.fsum, L.fcoef, L.vflux, L.hflux,
                                nvf, nint);
-        } else {
+        } else if (!cs) {
             {
                 HookScope hs(L, "flux_state");
                 hipLaunchKernelGGL((k_fstate), dim3(gl), dim3(256), 0, s, c, L.st, L.W, L.acc_v,

@@ -66,9 +66,9 @@ static hipError_t group_step_t(const GroupPlan& P, hipStream_t s) {
                                P.Ls);
         }
         {
-            HookCall hc(P.hook, P.hook_ctx, "kg_fcombine");
-            hipLaunchKernelGGL((kg_fcombine<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s,
-                               P.t_fsum, P.Ls);
+            HookCall hc(P.hook, P.hook_ctx, "kg_fface");
+            hipLaunchKernelGGL((kg_fface_cs<T>), dim3(P.g_fface), dim3(256), 0, s, P.t_fface,
+                               P.Ls);
         }
     } else {
         {
@@ -81,10 +81,10 @@ static hipError_t group_step_t(const GroupPlan& P, hipStream_t s) {
             hipLaunchKernelGGL((kg_fsum<T>), dim3(P.g_fsum), dim3(WAVE * SPLIT_FPW), 0, s,
                                P.t_fsum, P.Ls);
         }
-    }
-    {
-        HookCall hc(P.hook, P.hook_ctx, "kg_fface");
-        hipLaunchKernelGGL((kg_fface), dim3(P.g_fface), dim3(256), 0, s, P.t_fface, P.Ls);
+        {
+            HookCall hc(P.hook, P.hook_ctx, "kg_fface");
+            hipLaunchKernelGGL((kg_fface), dim3(P.g_fface), dim3(256), 0, s, P.t_fface, P.Ls);
+        }
     }
     {
         HookCall hc(P.hook, P.hook_ctx, "kg_fnode");

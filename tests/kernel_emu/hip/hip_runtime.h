@@ -190,6 +190,9 @@ inline void __syncthreads() { emu::barrier(emu::block_bar, 0, emu::nthreads); }
 // wavefront-scope ordering: the lanes of a wavefront run as separate fibers
 // here, so the wave barrier is a real one and the fences are no-ops
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
+// the lanes' stores are in host memory before any later load: a wave
+// barrier orders them across the lanes (fibers) of the wavefront
+inline void __threadfence() { emu::wave_barrier(); }
 inline void __builtin_amdgcn_wave_barrier() { emu::wave_barrier(); }
 
 // Wave-level operations exchange through xbuf between two wave barriers
