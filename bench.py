@@ -59,9 +59,18 @@ NAMES = {"c5": "rt2d_512x512_gh40_nm12", "c3": "rt2d_120x120_gh28_nm10",
          "c2": "poiseuille2d_4x64_gh24_nm8"}
 
 
-def algorithmic_bytes(case, kernel):
+SCHED_NAMES = {"flux": {capi.FLUX_FUSED: "fused", capi.FLUX_SPLIT: "split",
+                        capi.FLUX_STENCIL: "stencil", capi.FLUX_CELLSUM: "cellsum"},
+               "update": {capi.UPDATE_LDS: "lds", capi.UPDATE_MFMA: "mfma",
+                          capi.UPDATE_MFMA_NODE: "mfma-node",
+                          capi.UPDATE_MFMA_FACE: "mfma-face"}}
+
+
+def algorithmic_bytes(case, kernel, update=None):
     """Algorithmic HBM bytes of one launch: each array the kernel must
-    touch counted once (N = cells*nv^2, s = bytes per stored element)."""
+    touch counted once (N = cells*nv^2, s = bytes per stored element).
+    update: the case's update kernel (the face-forming one reads per-cell
+    divergence arrays instead of the face arrays)."""
     d = case.desc
     s = 8 if d.precision == capi.FP64 else 4
     nn = d.nv * d.nv
@@ -76,8 +85,12 @@ def algorithmic_bytes(case, kernel):
         return 6 * s * N + 2 * s * nvf
     if kernel == "flux_hface":
         return 6 * s * N + 2 * s * nhf
+    if kernel == "update2d" and update == capi.UPDATE_MFMA_FACE:
+        return 6 * s * N             # read h, b and 2 divergence arrays; write h, b
     if kernel == "update2d":         # read h, b, 4 face arrays; write h, b
         return 4 * s * N + 2 * s * (nvf + nhf)
+    if kernel == "flux_div":         # read h, b and 4 slopes; write 2 divergence arrays
+        return 8 * s * N
     # interior-face pipeline: both orientations per launch
     xper, yper = d.bc[0] == capi.BC_PERIODIC, d.bc[2] == capi.BC_PERIODIC
     nint = (d.nx + 1 if xper else d.nx - 1) * d.ny + d.nx * (d.ny + 1 if yper else d.ny - 1)
@@ -111,8 +124,9 @@ def run_gpu(case, warmup, steps, kernel_steps=3):
         s.reset_kernel_times()
         s.advance(kernel_steps)
         kt = s.kernel_times()
+    sched = s.schedule()
     s.close()
-    return dict(seconds=t1 - t0, info=info, kernels=kt, w=w, q=q, w0=w0)
+    return dict(seconds=t1 - t0, info=info, kernels=kt, w=w, q=q, w0=w0, schedule=sched)
 
 
 def host_info():
@@ -256,6 +270,7 @@ def rocprof_summary(dirs):
 TIMER_OF = [("k_update2d_mx", "update2d"), ("k_update2d_mfma", "update2d"),
             ("k_update2d<", "update2d"), ("k_fsum", "flux_sum"), ("k_fnode", "flux_node"),
             ("k_fface", "flux_face"), ("k_slope2d", "slope2d"), ("k_flux_wall", "flux_wall"),
+            ("k_fdiv", "flux_div"),
             ("k_cell_sum", "slope_sum"),
             ("k_cell_pre", "cell_pre"), ("k_finalize", "finalize"),
             ("k_part_sum", "finalize_partials"), ("k_rate_part", "timestep_rates"),
@@ -292,7 +307,8 @@ def traffic_summary(workload, dirs):
             continue
         fb = v["FETCH_SIZE_avg"] * 1024 * cal["fetch_scale"]
         wb = write[name]["WRITE_SIZE_avg"] * 1024 * cal["write_scale"]
-        ab = algorithmic_bytes(case, timer)
+        ab = algorithmic_bytes(case, timer,
+                               capi.UPDATE_MFMA_FACE if "k_update2d_mxf" in name else None)
         ms = trace.get(name, {}).get("avg_ms")
         kern[timer] = {"rocprof_name": name, "FETCH_SIZE_kib": v["FETCH_SIZE_avg"],
                        "WRITE_SIZE_kib": write[name]["WRITE_SIZE_avg"],
@@ -539,7 +555,7 @@ def main():
     kt = rm["kernels"]
     dom = max(kt, key=lambda k: kt[k][0])
     avg_ms = kt[dom][0] / kt[dom][1]
-    ab = algorithmic_bytes(mixed, dom)
+    ab = algorithmic_bytes(mixed, dom, rm["schedule"][1])
     roof = None
     if ab is not None:
         gbs = ab / (avg_ms * 1e-3) / 1e9
@@ -578,6 +594,8 @@ def main():
         "config": {"workload": NAMES[wl], "cells": [params["nx"], params["ny"]],
                    "nv": params["nv"], "Nm": params["Nm"], "knudsen": params["knudsen"],
                    "phase_space_points": N, "parallelism": "single GPU"},
+        "schedule": {"flux": SCHED_NAMES["flux"].get(rm["schedule"][0]),
+                     "update": SCHED_NAMES["update"].get(rm["schedule"][1])},
         "roofline": roof,
         "mfma": mfma,
         "cpu_baseline": cpu,

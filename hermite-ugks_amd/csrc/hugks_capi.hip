@@ -580,6 +580,11 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
             }
         }
         if (L.flux_mode == HUGKS_FLUX_CELLSUM && L.cs_js < 0) L.flux_mode = HUGKS_FLUX_SPLIT;
+        // the per-cell face divergence of HUGKS_UPDATE_MFMA_FACE (k_fdiv)
+        if (!h->b3 && update2d_mfma_supported(c) && fsz < (size_t)0xFFFFFFF0u) {
+            ALLOC(L.dvh, fsz);
+            ALLOC(L.dvb, fsz);
+        }
         ALLOC(L.fcoef, (nint ? nint : 1) * FACE_COEF_LD * sizeof(double));
         ALLOC(L.fmaxw, (nint ? nint : 1) * FACE_MAXW_LD * sizeof(double));
         ALLOC(L.cellpre, ncell * 24 * sizeof(double));
@@ -589,8 +594,6 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
         L.lds_limit = prop.sharedMemPerBlockOptin > 0 ? (size_t)prop.sharedMemPerBlockOptin
                                                       : (size_t)prop.sharedMemPerBlock;
         L.update_nw = update2d_mx_default_waves(c);
-        // default: the MFMA node update, else the MFMA Hermite tiles, else
-        // the LDS kernel
         const bool mx = update2d_mx_supported(c, h->fp64, h->b3, L.lds_limit, L.update_nw);
         if (h->b3 && !mx) {
             release(h);
@@ -598,11 +601,18 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
             return fail(nullptr, HUGKS_E_UNSUPPORTED,
                         "HUGKS_MIXED_BF16X3 needs the MFMA node update (nv <= 48, Nm <= 14)");
         }
-        L.update_mode = mx ? 2 : update2d_mfma_supported(c) ? 1 : 0;
+        // default: the node update forming the interior faces' equilibrium
+        // flux where it applies, else the MFMA node update, else the MFMA
+        // Hermite tiles, else the LDS kernel
+        const bool mxf =
+            (L.flux_mode == HUGKS_FLUX_CELLSUM || L.flux_mode == HUGKS_FLUX_SPLIT) &&
+            update2d_mxf_supported(L, L.update_nw);
+        L.update_mode = mxf ? 3 : mx ? 2 : update2d_mfma_supported(c) ? 1 : 0;
         if (const char* um = getenv("HUGKS_UPDATE_KERNEL"); um && !h->b3) {
             if (strcmp(um, "lds") == 0) L.update_mode = 0;
             if (strcmp(um, "mfma") == 0 && update2d_mfma_supported(c)) L.update_mode = 1;
             if (strcmp(um, "mfma-node") == 0 && mx) L.update_mode = 2;
+            if (strcmp(um, "mfma-face") == 0 && mxf) L.update_mode = 3;
         }
         if (L.update_lds > (size_t)prop.sharedMemPerBlockOptin && prop.sharedMemPerBlockOptin > 0) {
             release(h);
@@ -1050,7 +1060,7 @@ int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kern
     if (h->in_batch)
         return fail(h, HUGKS_E_ARG, "the case belongs to a batch, whose graphs fix its schedule");
     if (flux_kernel < HUGKS_FLUX_FUSED || flux_kernel > HUGKS_FLUX_CELLSUM ||
-        update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA_NODE)
+        update_kernel < HUGKS_UPDATE_LDS || update_kernel > HUGKS_UPDATE_MFMA_FACE)
         return fail(h, HUGKS_E_ARG, "unknown kernel choice");
     if (h->dim != 2) return HUGKS_OK;   // 1D has one kernel of each kind
     if (update_kernel == HUGKS_UPDATE_MFMA && !update2d_mfma_supported(h->L.c))
@@ -1065,6 +1075,13 @@ int hugks_set_schedule(hugks_handle* h, int32_t flux_kernel, int32_t update_kern
         !update2d_mx_supported(h->L.c, h->fp64, h->b3, h->L.lds_limit, h->L.update_nw))
         return fail(h, HUGKS_E_UNSUPPORTED,
                     "MFMA node update needs nv <= 48, Nm <= 14 and its LDS on the device");
+    if (update_kernel == HUGKS_UPDATE_MFMA_FACE &&
+        ((flux_kernel != HUGKS_FLUX_CELLSUM && flux_kernel != HUGKS_FLUX_SPLIT) ||
+         !update2d_mxf_supported(h->L, h->L.update_nw)))
+        return fail(h, HUGKS_E_UNSUPPORTED,
+                    "HUGKS_UPDATE_MFMA_FACE needs HUGKS_FLUX_CELLSUM or HUGKS_FLUX_SPLIT, FP32 "
+                    "or FP64 storage, the node update's conditions and distribution arrays "
+                    "under 4 GiB");
     if (flux_kernel == HUGKS_FLUX_STENCIL && !h->L.stencil_ok)
         return fail(h, HUGKS_E_UNSUPPORTED, "stencil flux needs distribution arrays under 4 GiB");
     if (flux_kernel == HUGKS_FLUX_CELLSUM && h->L.cs_js < 0)
@@ -1081,7 +1098,8 @@ int hugks_set_update_waves(hugks_handle* h, int32_t waves) {
         return fail(h, HUGKS_E_ARG, "the case belongs to a batch, whose graphs fix its schedule");
     if (h->dim != 2) return HUGKS_OK;
     const int nw = waves > 0 ? waves : update2d_mx_default_waves(h->L.c);
-    if (nw > 16 || !update2d_mx_supported(h->L.c, h->fp64, h->b3, h->L.lds_limit, nw))
+    if (nw > 16 || !update2d_mx_supported(h->L.c, h->fp64, h->b3, h->L.lds_limit, nw) ||
+        (h->L.update_mode == HUGKS_UPDATE_MFMA_FACE && !update2d_mxf_supported(h->L, nw)))
         return fail(h, HUGKS_E_UNSUPPORTED,
                     "no MFMA node-update kernel with " + std::to_string(nw) +
                         " wavefronts per cell for this velocity grid");
@@ -1165,9 +1183,10 @@ static int tune_case(const hugks_desc* d, bool grouped, int* flux_kernel, int* u
         const int f1 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_CELLSUM;
         const int u0 = grouped ? HUGKS_UPDATE_MFMA_NODE : HUGKS_UPDATE_LDS;
         for (int fk = f0; fk <= f1; ++fk)
-            for (int uk = u0; uk <= HUGKS_UPDATE_MFMA_NODE; ++uk) {
+            for (int uk = u0; uk <= (grouped ? HUGKS_UPDATE_MFMA_NODE : HUGKS_UPDATE_MFMA_FACE);
+                 ++uk) {
                 if (hugks_set_schedule(h, fk, uk) != HUGKS_OK) continue;
-                const bool node = uk == HUGKS_UPDATE_MFMA_NODE;
+                const bool node = uk >= HUGKS_UPDATE_MFMA_NODE;
                 for (int nw = node ? 1 : 0; nw <= (node ? 16 : 0); ++nw) {
                     if (node && hugks_set_update_waves(h, nw) != HUGKS_OK) continue;
                     float ms = 0.f;

@@ -299,6 +299,12 @@ __device__ __forceinline__ int tiled_cell(int L, int nx, int ny) {
     const int jj = rem / ws, ii = rem - jj * ws;
     return s * TILE_W + ii + nx * jj;
 }
+// The inverse of tiled_cell: the walk position of cell (i, j) of an nx x ny grid.
+__device__ __forceinline__ int tiled_index(int i, int j, int nx, int ny) {
+    const int s = i / TILE_W, ii = i - s * TILE_W;
+    const int ws = nx - s * TILE_W < TILE_W ? nx - s * TILE_W : TILE_W;
+    return s * TILE_W * ny + j * ws + ii;
+}
 
 // ----------------------------------------------------------- slope stencil
 // Cells of the slope of `cell` along dir (0: x, 1: y), as interp_inner /

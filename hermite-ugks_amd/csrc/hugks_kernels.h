@@ -21,6 +21,8 @@ struct Launch2D {
     double* part;        // per-cell residual partials [5*ncell]
     void *h, *b, *shx, *shy, *sbx, *sby;
     void *vfh, *vfb, *hfh, *hfb;
+    void *dvh, *dvb;     // update mode 3: per-cell face divergence of the free transport
+                         // and the walls (k_fdiv; [ncell][ldv], or null)
     double *vflux, *hflux;
     const void *unode, *vnode, *wnode;       // per-node tables in the arithmetic type
     const double *unode_d, *vnode_d, *wnode_d;  // FP64 tables (moments)
@@ -38,7 +40,9 @@ struct Launch2D {
                          // -1 where the mode is unavailable (nv > 64, unsorted v)
     int stencil_ok;      // distribution arrays under 4 GiB (32-bit byte offsets)
     int update_mode;     // cell update: 0 LDS dot products, 1 MFMA Hermite tiles,
-                         // 2 MFMA node update (force and collision on tiles)
+                         // 2 MFMA node update (force and collision on tiles), 3 mode 2
+                         // forming the equilibrium part of the interior faces' node
+                         // fluxes too (flux modes 1 and 3; k_fdiv replaces K5)
     int update_nw;       // wavefronts per cell of update mode 2
     size_t lds_limit;    // largest dynamic LDS of one workgroup on the device
     double *fsum, *fcoef, *fmaxw;  // per-face scratch of the pipeline
@@ -56,6 +60,20 @@ struct Launch2D {
 constexpr int FACE_SUM_LD = 40;
 constexpr int FACE_COEF_LD = 40;
 constexpr int FACE_MAXW_LD = 16;
+// Coefficient slots of one face in the per-face scratch fcoef (FACE_COEF_LD
+// doubles per face), written by the face kernels of k_flux_split.h and
+// k_flux_cellsum.h.
+enum : int {
+    FC_P = 0,      // prim at the face, 4
+    FC_TAU = 4,
+    FC_MT = 5,     // Mt(1..6) at 5..10
+    FC_AL = 11,    // 4
+    FC_AR = 15,    // 4
+    FC_AB = 19,    // 4
+    FC_FL = 23,    // 4: f0 and force part of the macro flux
+    FC_Q = 27,     // heat flux, 2
+    FC_AT = 29,    // 4 (K4)
+};
 // Leading dimension of one half-space bin of the per-cell sums (flux mode 3).
 constexpr int CELL_SUM_LD = 40;
 // Grids below this many cells reduce over their cells in one workgroup (one
@@ -146,6 +164,10 @@ bool update2d_mfma_supported(const Consts& c);
 // FP32 storage only)
 bool update2d_mx_supported(const Consts& c, bool fp64, bool b3, size_t lds_limit, int nw);
 int update2d_mx_default_waves(const Consts& c);
+// update mode 3 with nw wavefronts per cell: as mode 2 (not b3), the LDS of its
+// face rows within the limit, the divergence arrays allocated and each
+// distribution array under 4 GiB - 16 bytes (32-bit offsets)
+bool update2d_mxf_supported(const Launch2D& L, int nw);
 size_t update2d_mfma_lds_bytes(const Consts& c, bool fp64);
 
 }  // namespace hugks

@@ -349,18 +349,7 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fsum(
 }
 
 // ------------------------------------------------------------------ K2
-// Coefficient slots of one face.
-enum : int {
-    FC_P = 0,      // prim at the face, 4
-    FC_TAU = 4,
-    FC_MT = 5,     // Mt(1..6) at 5..10
-    FC_AL = 11,    // 4
-    FC_AR = 15,    // 4
-    FC_AB = 19,    // 4
-    FC_FL = 23,    // 4: f0 and force part of the macro flux
-    FC_Q = 27,     // heat flux, 2
-    FC_AT = 29,    // 4 (K4)
-};
+// Coefficient slots of one face: FC_* (hugks_kernels.h).
 
 __device__ __forceinline__ void fstate_lane(const Consts& c, double dt,
                                             const double* __restrict__ W,
@@ -884,6 +873,214 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void k_fnode(
     T* __restrict__ vfb, T* __restrict__ hfh, T* __restrict__ hfb, int nvi, int nint) {
     k_fnode_body<T, ST>(c, st, h, b, shx, shy, sbx, sby, uvnode, acc_v, acc_h, fcoef, vfh,
         vfb, hfh, hfb, nvi, nint, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// ------------------------------------------------------- K5 as a divergence
+// With the node update forming the equilibrium part of the interior faces'
+// node fluxes on its matrix tiles (update mode 3), this kernel takes K5's
+// place: per cell and node the rest of the face divergence,
+//   d = F_L - F_R + F_D - F_U,
+// where F is, on an interior face, the free-transport part of the node flux
+// (flux.f90:207-236; the terms of the upwind cell's f and slopes):
+//   l vn (mt4 (f + s len/2 f_n) + (mt6 fn - mt5 vn) f_n + t (mt6 ft - mt5 vt) f_t)
+// (s = +1 when the left/lower cell is upwind, else -1; t the frame sign of the
+// face, +1 on x faces, -1 on y faces; vt = v on x faces, -u on y faces), and on
+// a wall face the whole node flux of k_flux_wall.  The upwind cell of a node is
+// chosen as K5 chooses it (the sign of vn - fn dt/2 in the node arithmetic).
+// One wavefront per cell; a lane takes VW consecutive nodes of a row (16-byte
+// loads and stores where a row is a multiple of 16 bytes, else VW = 1).  Raw
+// buffer loads on whole arrays with 32-bit byte offsets: the host runs this
+// mode only for distribution arrays under 4 GiB - 16 bytes.
+constexpr int DIV_CPW = 4;   // cells (wavefronts) per workgroup of k_fdiv
+
+template <typename T, int VW>
+struct DivVec {
+    using V = T __attribute__((ext_vector_type(VW)));
+    static __device__ __forceinline__ V ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        if constexpr (sizeof(V) == 16)
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+        else if constexpr (sizeof(V) == 8)
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+        else
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    }
+};
+
+template <typename T, int VW>
+__device__ __forceinline__ void k_fdiv_body(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ acc_v, const double* __restrict__ acc_h,
+    const double* __restrict__ fcoef, const T* __restrict__ vfh, const T* __restrict__ vfb,
+    const T* __restrict__ hfh, const T* __restrict__ hfb, T* __restrict__ dvh,
+    T* __restrict__ dvb, int bid, int nblk) {
+    using A = typename Arith<T>::A;
+    using DV = DivVec<T, VW>;
+    using V = typename DV::V;
+    if (st->stopped) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nx = c.nx, ny = c.ny, nv = c.nv, nn = c.nn;
+    const int L = __builtin_amdgcn_readfirstlane(
+        (int)xcd_block(bid, nblk) * DIV_CPW + (int)(threadIdx.x / WAVE));
+    if (L >= nx * ny) return;
+    const int cell = tiled_cell(L, nx, ny);
+    const int i = cell % nx, j = cell / nx;
+    const double dt = st->dt;
+    const bool xper = c.bc[0] == HUGKS_BC_PERIODIC, yper = c.bc[2] == HUGKS_BC_PERIODIC;
+    const bool inL = xper || i > 0, inR = xper || i < nx - 1;
+    const bool inD = yper || j > 0, inU = yper || j < ny - 1;
+    const int nf = xper ? nx + 1 : nx - 1, nr = yper ? ny + 1 : ny - 1;
+    const int ox = xper ? 0 : 1, oy = yper ? 0 : 1;
+    const int vl = i + (nx + 1) * j, vr = vl + 1, hd = i + nx * j, hu = i + nx * (j + 1);
+    // node-independent factors of a face's free-transport flux:
+    // vs = fn dt/2, f4 = l mt4, h4 = l mt4 len/2, f6n = l mt6 fn, f5 = l mt5,
+    // f6t = t l mt6 ft, f5t = t l mt5 (zero on a wall face)
+    struct FK {
+        A vs, f4, h4, f6n, f5, f6t, f5t;
+    };
+    auto face_k = [&](bool in, bool isx, int g, int slot) -> FK {
+        FK k{0, 0, 0, 0, 0, 0, 0};
+        if (!in) return k;
+        const double* fc = fcoef + (size_t)g * FACE_COEF_LD;
+        const double* ac = isx ? acc_v : acc_h;
+        const double fn = ac[2 * slot], ft = ac[2 * slot + 1];
+        const double fl = isx ? c.dy : c.dx, len = isx ? c.dx : c.dy, ts = isx ? 1.0 : -1.0;
+        const double mt4 = fc[FC_MT + 3], mt5 = fc[FC_MT + 4], mt6 = fc[FC_MT + 5];
+        k.vs = (A)(0.5 * fn * dt);
+        k.f4 = (A)(fl * mt4);
+        k.h4 = (A)(fl * mt4 * (0.5 * len));
+        k.f6n = (A)(fl * mt6 * fn);
+        k.f5 = (A)(fl * mt5);
+        k.f6t = (A)(ts * fl * mt6 * ft);
+        k.f5t = (A)(ts * fl * mt5);
+        return k;
+    };
+    const FK kL = face_k(inL, true, tiled_index(i - ox, j, nf, ny), vl);
+    const FK kR = face_k(inR, true, tiled_index(i + 1 - ox, j, nf, ny), vr);
+    const FK kD = face_k(inD, false, nf * ny + tiled_index(i, j - oy, nx, nr), hd);
+    const FK kU = face_k(inU, false, nf * ny + tiled_index(i, j + 1 - oy, nx, nr), hu);
+    // one face's flux of h and b at a node: normal and tangential velocity,
+    // upwind side, the upwind f and slopes (normal first)
+    auto flux = [](const FK& k, A vn, A vt, int d, A hu_, A sn, A st_, A bu, A bn, A bt, A& fh,
+                   A& fb) {
+        const A ch = vn * k.f4;
+        const A cn = vn * ((d ? k.h4 : -k.h4) + k.f6n - k.f5 * vn);
+        const A ct = vn * (k.f6t - k.f5t * vt);
+        fh = ch * hu_ + cn * sn + ct * st_;
+        fb = ch * bu + cn * bn + ct * bt;
+    };
+    const size_t ldv = (size_t)c.ldv;
+    const uint32_t rowB = (uint32_t)(ldv * sizeof(T));
+    const uint32_t nbytes = (uint32_t)((size_t)nx * ny * ldv * sizeof(T));
+    constexpr uint32_t OUTF = 0xFFFFFFF0u;   // past every array (host check)
+    const __amdgpu_buffer_rsrc_t rH = buf_rsrc(h, nbytes), rB = buf_rsrc(b, nbytes);
+    const __amdgpu_buffer_rsrc_t rSX = buf_rsrc(shx, nbytes), rSY = buf_rsrc(shy, nbytes);
+    const __amdgpu_buffer_rsrc_t rBX = buf_rsrc(sbx, nbytes), rBY = buf_rsrc(sby, nbytes);
+    const uint32_t bO = (uint32_t)cell * rowB;
+    const uint32_t bL = (uint32_t)((i - 1 + nx) % nx + nx * j) * rowB;
+    const uint32_t bR = (uint32_t)((i + 1) % nx + nx * j) * rowB;
+    const uint32_t bD = (uint32_t)(i + nx * ((j - 1 + ny) % ny)) * rowB;
+    const uint32_t bU = (uint32_t)(i + nx * ((j + 1) % ny)) * rowB;
+    const uint32_t rowb = (uint32_t)(nn * (int)sizeof(T));
+    // the wall faces' node-flux rows (h, b) of the cell's four faces
+    const __amdgpu_buffer_rsrc_t rW[8] = {
+        buf_rsrc(vfh + (size_t)vl * ldv, rowb), buf_rsrc(vfh + (size_t)vr * ldv, rowb),
+        buf_rsrc(hfh + (size_t)hd * ldv, rowb), buf_rsrc(hfh + (size_t)hu * ldv, rowb),
+        buf_rsrc(vfb + (size_t)vl * ldv, rowb), buf_rsrc(vfb + (size_t)vr * ldv, rowb),
+        buf_rsrc(hfb + (size_t)hd * ldv, rowb), buf_rsrc(hfb + (size_t)hu * ldv, rowb)};
+    // f and the x and y slopes of h, then of b, at VW nodes
+    struct Up {
+        V h, sx, sy, b, bx, by;
+    };
+    auto ld = [&](uint32_t off) -> Up {
+        return Up{DV::ld(rH, off), DV::ld(rSX, off), DV::ld(rSY, off),
+                  DV::ld(rB, off), DV::ld(rBX, off), DV::ld(rBY, off)};
+    };
+    T* const oh = dvh + (size_t)cell * ldv;
+    T* const ob = dvb + (size_t)cell * ldv;
+#pragma unroll 1
+    for (int e = lane * VW; e < nn; e += WAVE * VW) {
+        const int jv = e / nv, iu = e - jv * nv;   // VW divides nv: one row
+        const uint32_t oe = (uint32_t)e * (uint32_t)sizeof(T);
+        const A vv = (A)v1[jv];
+        A uu[VW];
+#pragma unroll
+        for (int q = 0; q < VW; ++q) uu[q] = (A)u1[iu + q];
+        const Up me = ld(bO + oe);
+        A th[VW], tb[VW];
+#pragma unroll
+        for (int q = 0; q < VW; ++q) th[q] = tb[q] = (A)0;
+        // one face at a time: the upwind side of each node, the neighbour's
+        // values where the neighbour is upwind (not loaded where no node of
+        // the wavefront needs them), the flux added with the divergence
+        // sign; a wall face's node fluxes as the wall kernel stored them
+#pragma unroll
+        for (int fa = 0; fa < 4; ++fa) {
+            const bool isx = fa < 2, lo = (fa & 1) == 0;   // left/lower face of the cell
+            const bool in = fa == 0 ? inL : fa == 1 ? inR : fa == 2 ? inD : inU;
+            const FK& k = fa == 0 ? kL : fa == 1 ? kR : fa == 2 ? kD : kU;
+            const uint32_t bn = fa == 0 ? bL : fa == 1 ? bR : fa == 2 ? bD : bU;
+            const A sg = lo ? (A)1 : (A)-1;
+            if (in) {
+                int d[VW];
+                bool nb[VW];
+                bool any = false;
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    d[q] = upwind_delta((isx ? uu[q] : vv) - k.vs);
+                    // the neighbour is upwind of the left/lower face when the
+                    // left/lower cell is (d = 1), of the right/upper one when
+                    // the right/upper cell is (d = 0)
+                    nb[q] = lo ? d[q] != 0 : d[q] == 0;
+                    any = any || nb[q];
+                }
+                Up w = me;
+                if (__any(any)) w = ld(any ? bn + oe : OUTF);
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    const Up& x = nb[q] ? w : me;
+                    A fh, fb;
+                    if (isx)
+                        flux(k, uu[q], vv, d[q], (A)x.h[q], (A)x.sx[q], (A)x.sy[q], (A)x.b[q],
+                             (A)x.bx[q], (A)x.by[q], fh, fb);
+                    else
+                        flux(k, vv, -uu[q], d[q], (A)x.h[q], (A)x.sy[q], (A)x.sx[q], (A)x.b[q],
+                             (A)x.by[q], (A)x.bx[q], fh, fb);
+                    th[q] += sg * fh;
+                    tb[q] += sg * fb;
+                }
+            } else {
+                const V wh = DV::ld(rW[fa], oe), wb = DV::ld(rW[4 + fa], oe);
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    th[q] += sg * (A)wh[q];
+                    tb[q] += sg * (A)wb[q];
+                }
+            }
+        }
+        V sh, sb;
+#pragma unroll
+        for (int q = 0; q < VW; ++q) {
+            sh[q] = (T)th[q];
+            sb[q] = (T)tb[q];
+        }
+        // streamed out: read back only by the cell update
+        __builtin_nontemporal_store(sh, reinterpret_cast<V*>(oh + e));
+        __builtin_nontemporal_store(sb, reinterpret_cast<V*>(ob + e));
+    }
+}
+template <typename T, int VW>
+__global__ __launch_bounds__(WAVE * DIV_CPW) HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? 4 : 3) void k_fdiv(
+    Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
+    const T* __restrict__ shx, const T* __restrict__ shy, const T* __restrict__ sbx,
+    const T* __restrict__ sby, const double* __restrict__ u1, const double* __restrict__ v1,
+    const double* __restrict__ acc_v, const double* __restrict__ acc_h,
+    const double* __restrict__ fcoef, const T* __restrict__ vfh, const T* __restrict__ vfb,
+    const T* __restrict__ hfh, const T* __restrict__ hfb, T* __restrict__ dvh,
+    T* __restrict__ dvb) {
+    k_fdiv_body<T, VW>(c, st, h, b, shx, shy, sbx, sby, u1, v1, acc_v, acc_h, fcoef, vfh, vfb,
+                       hfh, hfb, dvh, dvb, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace hugks

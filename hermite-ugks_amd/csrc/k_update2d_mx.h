@@ -68,9 +68,13 @@ static __constant__ const double kInvFact[MF_NB] = {
 // its twenty scattered loads per tile job outweigh the occupancy).
 struct MxLayout {
     int np, sH, sR, sD, ldG;
-    int oD, oL, oRc, oHx, oHo, oHy, oT, oOM, end;
+    int oD, oL, oRc, oFA, oFB, oHx, oHo, oHy, oT, oOM, end;
 };
-__host__ __device__ inline MxLayout mx_layout(int nv, bool stage_g) {
+// With the interior face fluxes on the tiles (ff), the face rows (k_update2d_mx_body)
+// take 24 rows of stride sR and 8 of stride sH after Rc, inside G's region where
+// G is staged (G is dead once Tm is formed), and 16 rows of stride sH in Hx's
+// place (Hx is dead then too).
+__host__ __device__ inline MxLayout mx_layout(int nv, bool stage_g, bool ff = false) {
     MxLayout L;
     L.np = mf_np(nv);
     L.sH = L.np + 4;
@@ -81,7 +85,12 @@ __host__ __device__ inline MxLayout mx_layout(int nv, bool stage_g) {
     L.oD = 0;
     L.oL = L.oD + 2 * MF_NB * L.sD;
     L.oRc = L.oL + 2 * 8 * L.sH;
-    const int r0 = L.oRc + 8 * L.sR;
+    int r0 = L.oRc + 8 * L.sR;
+    L.oFA = L.oFB = r0;
+    if (ff) {
+        L.oFB = r0 + 24 * L.sR;
+        r0 = L.oFB + 8 * L.sH;
+    }
     const int g = stage_g ? 2 * nv * L.ldG : 0;
     L.oHx = g > r0 ? g : r0;
     L.oHo = L.oHx + MF_NB * L.sH;
@@ -92,8 +101,8 @@ __host__ __device__ inline MxLayout mx_layout(int nv, bool stage_g) {
     return L;
 }
 // G is staged exactly when the operand element is 4 bytes
-__host__ __device__ inline size_t mx_lds_bytes(int nv, size_t tsize) {
-    return (size_t)mx_layout(nv, tsize == 4).end * tsize;
+__host__ __device__ inline size_t mx_lds_bytes(int nv, size_t tsize, bool ff = false) {
+    return (size_t)mx_layout(nv, tsize == 4, ff).end * tsize;
 }
 
 // Compiled instances (wavefronts per cell, tile positions per wavefront,
@@ -196,13 +205,23 @@ __device__ __forceinline__ X mx_opaque(X x) {
     return x;
 }
 
-template <typename T, int NW, int MAXP, int NVC, bool SB3 = false>
+// Inputs of the face fluxes when the update forms their equilibrium part
+// itself (FF): the per-cell divergence of the rest (free transport of the
+// interior faces, whole node fluxes of the wall faces; k_fdiv), the
+// per-face coefficients of the flux pipeline (FC_*, indexed like its
+// interior faces) and the face accelerations.
+struct MxFace {
+    const void *dvh, *dvb;   // [ncell][ldv]
+    const double *fcoef, *acc_v, *acc_h;
+};
+
+template <typename T, int NW, int MAXP, int NVC, bool SB3 = false, bool FF = false>
 __device__ __forceinline__ void k_update2d_mx_body(
     const Consts& c, const DevState* __restrict__ st, const double* __restrict__ cp,
     T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
     const T* __restrict__ hfh, const T* __restrict__ hfb,
     const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ u1,
-    const double* __restrict__ v1, int bid, int nblk) {
+    const double* __restrict__ v1, int bid, int nblk, MxFace fx = {}) {
     using A = typename Arith<T>::A;
     using O = MxOps<T, SB3>;
     using E = typename O::E;
@@ -212,6 +231,8 @@ __device__ __forceinline__ void k_update2d_mx_body(
     HIP_DYNAMIC_SHARED(double, smem_raw)
     // unscaled coefficients C10, C30, C12, C01, C03, C21 of h, C10, C01 of b
     __shared__ double qm[8];
+    // FF: the force shift 0.5 fn dt of each face (left, right, down, up)
+    __shared__ typename Arith<T>::A fk[4];
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const int cell = tiled_cell((int)xcd_block(bid, nblk), c.nx, c.ny);
@@ -232,7 +253,7 @@ __device__ __forceinline__ void k_update2d_mx_body(
     const double sqrt_rt = sqrt(rt);
 
     constexpr bool STAGE_G = sizeof(E) == 4;
-    const MxLayout ly = mx_layout(nv, STAGE_G);
+    const MxLayout ly = mx_layout(nv, STAGE_G, FF);
     const int sH = ly.sH, sR = ly.sR, sD = ly.sD, ldG = ly.ldG, GS = nv * ly.ldG;
     E* const sm = reinterpret_cast<E*>(smem_raw);
     E* const G = sm;               // staged G (4-byte operands)
@@ -269,6 +290,26 @@ __device__ __forceinline__ void k_update2d_mx_body(
         return (p < npos && iu < nv && jv < nv) ? (uint32_t)((iu + nv * jv) * (int)sizeof(T))
                                                 : OUT;
     };
+
+    // ---- FF: the cell's four faces (0 left, 1 right, 2 down, 3 up).  An
+    // interior face (not a wall) has the slot g of the flux pipeline's
+    // per-face scratch (its walk order, face_geo).
+    const int ny = c.ny;
+    const bool xper = c.bc[0] == HUGKS_BC_PERIODIC, yper = c.bc[2] == HUGKS_BC_PERIODIC;
+    const bool inL = xper || i > 0, inR = xper || i < nx - 1;
+    const bool inD = yper || j > 0, inU = yper || j < ny - 1;
+    auto face_pick = [](int fa, auto a0, auto a1, auto a2, auto a3) {
+        return fa == 0 ? a0 : fa == 1 ? a1 : fa == 2 ? a2 : a3;
+    };
+    if constexpr (FF) {
+        if (tid < 4) {
+            const int fa = tid;
+            const bool in = face_pick(fa, inL, inR, inD, inU);
+            const int slot = face_pick(fa, vl, vr, hd, hu);
+            const double* ac = fa < 2 ? fx.acc_v : fx.acc_h;
+            fk[fa] = in ? (A)(0.5 * ac[2 * slot] * dt) : (A)0;
+        }
+    }
 
     // ---- 8-byte operands: the B operand of Tm = Hx G, G = w f, for job
     // (f, jt): node (iu = 4 ks + lk, jv = 16 jt + lr) of K step ks, loaded
@@ -374,6 +415,9 @@ __device__ __forceinline__ void k_update2d_mx_body(
         buf_rsrc(hfh + (size_t)hd * ldv, rowb), buf_rsrc(hfh + (size_t)hu * ldv, rowb),
         buf_rsrc(vfb + (size_t)vl * ldv, rowb), buf_rsrc(vfb + (size_t)vr * ldv, rowb),
         buf_rsrc(hfb + (size_t)hd * ldv, rowb), buf_rsrc(hfb + (size_t)hu * ldv, rowb)};
+    // FF: the cell's rows of the divergence arrays
+    const __amdgpu_buffer_rsrc_t rsdh = buf_rsrc((const T*)fx.dvh + (size_t)cell * ldv, rowb);
+    const __amdgpu_buffer_rsrc_t rsdb = buf_rsrc((const T*)fx.dvb + (size_t)cell * ldv, rowb);
     // one tile position's face fluxes and, for 8-byte operands, f^n, b^n
     // of its nodes (read a second time: the cell's rows are in L2 from the
     // Tm contraction)
@@ -381,8 +425,13 @@ __device__ __forceinline__ void k_update2d_mx_body(
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const uint32_t o = node_off(pi, v);
+            if constexpr (FF) {
+                fvo[0][v] = (A)buf_ld<T>(rsdh, o);
+                fvo[4][v] = (A)buf_ld<T>(rsdb, o);
+            } else {
 #pragma unroll
-            for (int a = 0; a < 8; ++a) fvo[a][v] = (A)buf_ld<T>(rsf[a], o);
+                for (int a = 0; a < 8; ++a) fvo[a][v] = (A)buf_ld<T>(rsf[a], o);
+            }
             if constexpr (!STAGE_G) {
                 rho[v] = (A)buf_ld<T>(rsh, o);
                 rbo[v] = (A)buf_ld<T>(rsb, o);
@@ -514,6 +563,121 @@ __device__ __forceinline__ void k_update2d_mx_body(
             }
         }
     }
+    // ---- FF: the equilibrium part of the interior faces' node fluxes as
+    // K rows of the F^T tiles.  On a face with normal velocity vn and
+    // tangential vt (x faces: u, v; y faces: v, -u) that part of the
+    // flux of h is l vn (rho lam/pi) exp(-lam (cu^2 + cv^2)) Q_h(vn, vt)
+    // (flux.f90:207-236; cu = vn - U, cv = vt - V in the face's frame), and
+    // Q_h is a cubic in cv whose coefficients are functions of vn alone
+    // (its upwind side included):
+    //   flux = sum_k N_k(vn) T_k(vt),  T_k = exp(-lam cv^2) cv^k, k = 0..3,
+    // and the same for b with its own N_k.  With the divergence sign and
+    // 1/(area (1 + dt/2tau)) folded into N_k, the x faces add 8 rows T_k(v)
+    // (A operand, shared) x N_k(u) (B operand, per distribution) and the y
+    // faces 8 rows N_k(v) (per distribution) x T_k(-u) (shared).
+    E* const TxA = sm + ly.oFA;    // [8][sR]: x faces' T_k over v
+    E* const NyA = TxA + 8 * sR;   // [2][8][sR]: y faces' N_k over v, h then b
+    E* const TyB = sm + ly.oFB;    // [8][sH]: y faces' T_k over u
+    E* const NxB = Hx;             // [2][8][sH]: x faces' N_k over u (Hx is dead)
+    if constexpr (FF) {
+        const double cdd = iden / (c.dx * c.dy);
+        for (int t = tid; t < 4 * np; t += NT) {
+            const int fa = t / np, idx = t - fa * np;
+            const bool isx = fa < 2;
+            const bool in = face_pick(fa, inL, inR, inD, inU);
+            A nh[4] = {0, 0, 0, 0}, nb[4] = {0, 0, 0, 0}, tk[4] = {0, 0, 0, 0};
+            if (in && idx < nv) {
+                const int nf = xper ? nx + 1 : nx - 1, nr = yper ? ny + 1 : ny - 1;
+                const int ox = xper ? 0 : 1, oy = yper ? 0 : 1;
+                const int g = face_pick(fa, tiled_index(i - ox, j, nf, ny),
+                                        tiled_index(i + 1 - ox, j, nf, ny),
+                                        nf * ny + tiled_index(i, j - oy, nx, nr),
+                                        nf * ny + tiled_index(i, j + 1 - oy, nx, nr));
+                const int slot = face_pick(fa, vl, vr, hd, hu);
+                const double* fc = fx.fcoef + (size_t)g * FACE_COEF_LD;
+                const double* ac = isx ? fx.acc_v : fx.acc_h;
+                const A fl = (A)(isx ? c.dy : c.dx);
+                const A sg = (A)((fa & 1) ? -cdd : cdd);
+                // the row arithmetic in the node arithmetic, as the per-node
+                // flux forms these terms
+                const A U = (A)fc[FC_P + 1], V = (A)fc[FC_P + 2], lam = (A)fc[FC_P + 3];
+                const A mt1 = (A)fc[FC_MT], mt2 = (A)fc[FC_MT + 1], mt3 = (A)fc[FC_MT + 2];
+                const A K = (A)c.K;
+                const A kfac = (A)(c.K / (2.0 * fc[FC_P + 3]));
+                const A mxi2 =
+                    (A)((c.K * c.K + 2.0 * c.K) / (4.0 * fc[FC_P + 3] * fc[FC_P + 3]));
+                const A csA =
+                    (A)(c.shk * (1.0 - c.prandtl) * fc[FC_P + 3] * fc[FC_P + 3] / fc[FC_P]);
+                const A q0 = (A)fc[FC_Q], q1 = (A)fc[FC_Q + 1];
+                // normal node of this row index and its upwind side (as the
+                // per-node flux decides it, in the node arithmetic)
+                const A vn = (A)(isx ? u1[idx] : v1[idx]);
+                const int d = upwind_delta(vn - fk[fa]);
+                const double* as = fc + (d ? FC_AL : FC_AR);
+                const A s0 = (A)as[0], s1 = (A)as[1], s2 = (A)as[2], s3 = (A)(0.5 * as[3]);
+                const A s4 = s3 * mxi2;
+                const A pb0 = (A)fc[FC_AB], pb1 = (A)fc[FC_AB + 1], pb2 = (A)fc[FC_AB + 2];
+                const A pb3 = (A)(0.5 * fc[FC_AB + 3]), pb4 = pb3 * mxi2;
+                const A pt0 = (A)fc[FC_AT], pt1 = (A)fc[FC_AT + 1], pt2 = (A)fc[FC_AT + 2];
+                const A pt3 = (A)(0.5 * fc[FC_AT + 3]), pt4 = pt3 * mxi2;
+                const A tm2fn = (A)(fc[FC_TAU] * fc[FC_MT + 1] * ac[2 * slot]);
+                const A tm2ft = (A)(fc[FC_TAU] * fc[FC_MT + 1] * ac[2 * slot + 1]);
+                const A cu = vn - U;
+                const A al = mt2 * vn - tm2fn, b0 = mt2 * V - tm2ft;
+                const A a5 = (A)2 * lam * cu * cu + K - (A)5, a3 = a5 + (A)2;
+                const A m2 = vn * vn + V * V;
+                // h: the terms mt1 (H + H'), al g_h, be g_bh, mt3 g_Th by power of cv
+                const A P0 = pb0 + pb1 * vn + pb2 * V + pb3 * (m2 + kfac);
+                const A P1 = pb2 + (A)2 * pb3 * V;
+                const A T0 = pt0 + pt1 * vn + pt2 * V + pt3 * (m2 + kfac);
+                const A T1 = pt2 + (A)2 * pt3 * V;
+                const A S0 = s0 + s1 * vn + s2 * V + s3 * (m2 + kfac);
+                const A S1 = s2 + (A)2 * s3 * V;
+                const A mq = mt1 * csA;
+                const A qh[4] = {
+                    mt1 * ((A)1 + csA * q0 * cu * a5) + al * S0 + b0 * P0 + mt3 * T0,
+                    mq * q1 * a5 + al * S1 + b0 * P1 + mt2 * P0 + mt3 * T1,
+                    mq * (A)2 * lam * q0 * cu + al * s3 + b0 * pb3 + mt2 * P1 + mt3 * pt3,
+                    mq * (A)2 * lam * q1 + mt2 * pb3};
+                // b: B = kfac H, B' with K - 3, g_b = B P + a3/2 mxi2 H
+                const A P0b = kfac * (pb0 + pb1 * vn + pb2 * V + pb3 * m2) + pb4;
+                const A T0b = kfac * (pt0 + pt1 * vn + pt2 * V + pt3 * m2) + pt4;
+                const A S0b = kfac * (s0 + s1 * vn + s2 * V + s3 * m2) + s4;
+                const A qb[4] = {
+                    kfac * mt1 * ((A)1 + csA * q0 * cu * a3) + al * S0b + b0 * P0b + mt3 * T0b,
+                    kfac * (mq * q1 * a3 + al * S1 + b0 * P1 + mt3 * T1) + mt2 * P0b,
+                    kfac * (mq * (A)2 * lam * q0 * cu + al * s3 + b0 * pb3 + mt2 * P1 + mt3 * pt3),
+                    kfac * (mq * (A)2 * lam * q1 + mt2 * pb3)};
+                const A rlp = (A)(fc[FC_P] * (fc[FC_P + 3] / c.pi));
+                const A en = sg * fl * vn * rlp * fexp(-lam * cu * cu);
+                // tangential node of this row index
+                const A cv = (A)(isx ? v1[idx] : -u1[idx]) - V;
+                const A ec = fexp(-lam * cv * cv);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    nh[k] = en * qh[k];
+                    nb[k] = en * qb[k];
+                }
+                tk[0] = ec;
+                tk[1] = ec * cv;
+                tk[2] = tk[1] * cv;
+                tk[3] = tk[2] * cv;
+            }
+            const int fr = 4 * (fa & 1);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (isx) {
+                    NxB[(fr + k) * sH + idx] = O::put(nh[k]);
+                    NxB[(8 + fr + k) * sH + idx] = O::put(nb[k]);
+                    TxA[(fr + k) * sR + idx] = O::put(tk[k]);
+                } else {
+                    NyA[(fr + k) * sR + idx] = O::put(nh[k]);
+                    NyA[(8 + fr + k) * sR + idx] = O::put(nb[k]);
+                    TyB[(fr + k) * sH + idx] = O::put(tk[k]);
+                }
+            }
+        }
+    }
     // the first tile position's face fluxes (and f^n) in flight during the
     // R contraction; with one position per wavefront and staged G, from
     // the start
@@ -584,13 +748,21 @@ __device__ __forceinline__ void k_update2d_mx_body(
                 const int t = 4 * ks + lk;
                 a = M::mma(Rc[t * sR + jvb + lr], Lr[(8 * f + t) * sH + iu], a);
             }
+            if constexpr (FF) {
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    const int t = 4 * ks + lk;
+                    a = M::mma(TxA[t * sR + jvb + lr], NxB[(8 * f + t) * sH + iu], a);
+                    a = M::mma(NyA[(8 * f + t) * sR + jvb + lr], TyB[t * sH + iu], a);
+                }
+            }
             acc[f] = a;
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const uint32_t o = node_off(pi, v);
-            const A dh = fv[0][v] - fv[1][v] + fv[2][v] - fv[3][v];
-            const A db = fv[4][v] - fv[5][v] + fv[6][v] - fv[7][v];
+            const A dh = FF ? fv[0][v] : fv[0][v] - fv[1][v] + fv[2][v] - fv[3][v];
+            const A db = FF ? fv[4][v] : fv[4][v] - fv[5][v] + fv[6][v] - fv[7][v];
             const A ho = STAGE_G ? rhs[STAGE_G ? pi : 0][v] : rh[v];
             const A bo = STAGE_G ? rbs[STAGE_G ? pi : 0][v] : rb[v];
             const double hn = fma(c1, (double)mx_opaque(ho),
@@ -617,6 +789,23 @@ HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? 3 : 1) void k_update2d_mx(
     const double* __restrict__ v1) {
     k_update2d_mx_body<T, NW, MAXP, NVC, SB3>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode, u1,
                                               v1, (int)blockIdx.x, (int)gridDim.x);
+}
+// The same update with the equilibrium part of the interior faces' node
+// fluxes formed on its tiles and the rest of the divergence read per cell
+// (k_fdiv; no per-face node flux arrays).  FP32 storage: four wavefronts per
+// SIMD where nv is compiled in (at most 128 VGPRs; the run-time-nv instances
+// would spill there).
+template <typename T, int NW, int MAXP, int NVC, bool SB3 = false>
+__global__ __launch_bounds__(NW * WAVE)
+HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? (NVC ? 4 : 3) : 1) void k_update2d_mxf(
+    Consts c, const DevState* __restrict__ st, const double* __restrict__ cp,
+    T* __restrict__ h, T* __restrict__ b, const T* __restrict__ vfh, const T* __restrict__ vfb,
+    const T* __restrict__ hfh, const T* __restrict__ hfb,
+    const typename Arith<T>::A* __restrict__ wnode, const double* __restrict__ u1,
+    const double* __restrict__ v1, MxFace fx) {
+    k_update2d_mx_body<T, NW, MAXP, NVC, SB3, true>(c, st, cp, h, b, vfh, vfb, hfh, hfb, wnode,
+                                                    u1, v1, (int)blockIdx.x, (int)gridDim.x,
+                                                    fx);
 }
 #endif
 

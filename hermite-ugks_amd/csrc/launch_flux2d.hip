@@ -92,7 +92,8 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
                                    L.acc_h, L.fmaxw, L.fcoef, L.vflux, L.hflux, nvf, nint);
             }
         }
-        {
+        // (update mode 3: k_fdiv below instead)
+        if (L.update_mode != 3) {
             HookScope hs(L, "flux_node");
             auto kn = stn ? k_fnode<T, true> : k_fnode<T, false>;
             hipLaunchKernelGGL(kn, dim3(gw), dim3(WAVE * SPLIT_FPW), 0, s, c, L.st,
@@ -121,6 +122,20 @@ static hipError_t launch_flux2d_t(const Launch2D& L, hipStream_t s) {
                            (const T*)L.h, (const T*)L.b, (const T*)L.shx, (const T*)L.shy,
                            (const T*)L.sbx, (const T*)L.sby, un, vn, wn, (T*)L.vfh, (T*)L.vfb,
                            L.vflux, (T*)L.hfh, (T*)L.hfb, L.hflux, (int)L.nwall);
+    }
+    if (L.update_mode == 3) {
+        // the face divergence but the equilibrium part of the interior
+        // faces, which the update forms on its tiles (after the walls: it
+        // reads their node fluxes)
+        HookScope hs(L, "flux_div");
+        constexpr int VW = 16 / (int)sizeof(T);
+        auto kd = (c.nv * (int)sizeof(T)) % 16 == 0 ? k_fdiv<T, VW> : k_fdiv<T, 1>;
+        hipLaunchKernelGGL(kd, dim3((unsigned)((ncell + DIV_CPW - 1) / DIV_CPW)),
+                           dim3(WAVE * DIV_CPW), 0, s, c, L.st, (const T*)L.h, (const T*)L.b,
+                           (const T*)L.shx, (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby,
+                           L.u1, L.v1, L.acc_v, L.acc_h, L.fcoef, (const T*)L.vfh,
+                           (const T*)L.vfb, (const T*)L.hfh, (const T*)L.hfb, (T*)L.dvh,
+                           (T*)L.dvb);
     }
     return hipGetLastError();
 }

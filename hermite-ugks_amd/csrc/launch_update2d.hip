@@ -102,20 +102,46 @@ bool update2d_mx_supported(const Consts& c, bool fp64, bool b3, size_t lds_limit
 
 int update2d_mx_default_waves(const Consts& c) { return mx_default_nw(c.nv); }
 
+bool update2d_mxf_supported(const Launch2D& L, int nw) {
+    const Consts& c = L.c;
+    int mp, nvc;
+    const size_t fbytes = (size_t)c.nx * c.ny * c.ldv * (L.fp64 ? sizeof(double) : sizeof(float));
+    return !L.b3 && update2d_mfma_supported(c) && mx_key(c.nv, nw, &mp, &nvc, false) &&
+           mx_lds_bytes(c.nv, mx_elem(L.fp64, false), true) <= L.lds_limit && L.dvh && L.dvb &&
+           fbytes < (size_t)0xFFFFFFF0u;
+}
+
 template <typename T, bool SB3>
-static hipError_t launch_update_mx_t(const Launch2D& L, hipStream_t s, int nw, bool prepare) {
+static hipError_t launch_update_mx_t(const Launch2D& L, hipStream_t s, int nw, bool prepare,
+                                     bool ff) {
     const Consts& c = L.c;
     int maxp = 0, nvc = 0;
     if (!mx_key(c.nv, nw, &maxp, &nvc, SB3)) return hipErrorInvalidValue;
+    if (ff && SB3) return hipErrorInvalidValue;
     const unsigned ncell = (unsigned)(c.nx * c.ny);
-    const size_t lds = mx_lds_bytes(c.nv, mx_elem(sizeof(T) == 8, SB3));
+    const size_t lds = mx_lds_bytes(c.nv, mx_elem(sizeof(T) == 8, SB3), ff);
+    const MxFace fx{L.dvh, L.dvb, L.fcoef, L.acc_v, L.acc_h};
     if (!prepare) {
         HookScope hs(L, "cell_pre");
         hipLaunchKernelGGL((k_cell_pre), dim3((ncell + 255) / 256), dim3(256), 0, s, c, L.st, L.W,
                            L.part, L.vflux, L.hflux, L.acc_cell, L.cellpre);
     }
 #define HUGKS_UPDX_CASE(W, P, V)                                                              \
-    if (nw == W && maxp == P && nvc == V) {                                                   \
+    if (nw == W && maxp == P && nvc == V && ff) {                                             \
+        if constexpr (!SB3) {                                                                 \
+            if (prepare)                                                                      \
+                return hipFuncSetAttribute((const void*)k_update2d_mxf<T, W, P, V>,           \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                                           (int)lds);                                         \
+            HookScope hs(L, "update2d");                                                      \
+            hipLaunchKernelGGL((k_update2d_mxf<T, W, P, V>), dim3(ncell), dim3(W * WAVE),     \
+                lds, s, c, L.st, L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh,                \
+                (const T*)L.vfb, (const T*)L.hfh, (const T*)L.hfb,                            \
+                (const typename Arith<T>::A*)L.wnode, L.u1, L.v1, fx);                        \
+            return hipGetLastError();                                                         \
+        }                                                                                     \
+    }                                                                                         \
+    if (nw == W && maxp == P && nvc == V && !ff) {                                            \
         if (prepare)                                                                          \
             return hipFuncSetAttribute((const void*)k_update2d_mx<T, W, P, V, SB3>,           \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
@@ -135,13 +161,15 @@ static hipError_t launch_update_mx_t(const Launch2D& L, hipStream_t s, int nw, b
     return hipErrorInvalidValue;
 }
 
-static hipError_t launch_update_mx(const Launch2D& L, hipStream_t s, int nw, bool prepare) {
-    if (L.fp64) return launch_update_mx_t<double, false>(L, s, nw, prepare);
-    return L.b3 ? launch_update_mx_t<float, true>(L, s, nw, prepare)
-                : launch_update_mx_t<float, false>(L, s, nw, prepare);
+static hipError_t launch_update_mx(const Launch2D& L, hipStream_t s, int nw, bool prepare,
+                                   bool ff = false) {
+    if (L.fp64) return launch_update_mx_t<double, false>(L, s, nw, prepare, ff);
+    return L.b3 ? launch_update_mx_t<float, true>(L, s, nw, prepare, ff)
+                : launch_update_mx_t<float, false>(L, s, nw, prepare, ff);
 }
 
 hipError_t launch_update2d(const Launch2D& L, hipStream_t s) {
+    if (L.update_mode == 3) return launch_update_mx(L, s, L.update_nw, false, true);
     if (L.update_mode == 2) return launch_update_mx(L, s, L.update_nw, false);
     if (L.update_mode == 1)
         return L.fp64 ? launch_update_mfma_t<double>(L, s, false)
@@ -160,6 +188,8 @@ hipError_t prepare_update2d(const Launch2D& L) {
     for (int nw = 1; nw <= 16 && e == hipSuccess; ++nw) {
         if (!update2d_mx_supported(L.c, L.fp64, L.b3, L.lds_limit, nw)) continue;
         e = launch_update_mx(L, nullptr, nw, true);
+        if (e == hipSuccess && update2d_mxf_supported(L, nw))
+            e = launch_update_mx(L, nullptr, nw, true, true);
     }
     return e;
 }

@@ -24,7 +24,7 @@ FORCE_HERMITE_SYMMETRIC, FORCE_HERMITE_SYMMETRIC_B, FORCE_HERMITE_LEGACY = 0, 1,
 FORCE_FINITE_DIFFERENCE = 3
 STOP_NONE, STOP_MAXTIME, STOP_RESIDUAL, STOP_TIME, STOP_STEADY, STOP_ERROR = 0, 1, 2, 3, 4, 5
 FLUX_FUSED, FLUX_SPLIT, FLUX_STENCIL, FLUX_CELLSUM = 0, 1, 2, 3
-UPDATE_LDS, UPDATE_MFMA, UPDATE_MFMA_NODE = 0, 1, 2
+UPDATE_LDS, UPDATE_MFMA, UPDATE_MFMA_NODE, UPDATE_MFMA_FACE = 0, 1, 2, 3
 BATCH_AUTOTUNE = 1
 
 _dp = C.POINTER(C.c_double)

@@ -286,10 +286,23 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
  *                             (nv <= 48, Nm <= 14)
  *           HUGKS_UPDATE_MFMA_NODE  the force and the collision terms as
  *                             matrix tiles, each node updated from its
- *                             tile accumulator (default when nv <= 48,
- *                             Nm <= 14 and its LDS fits the device; the
+ *                             tile accumulator (nv <= 48, Nm <= 14 and its
+ *                             LDS on the device; the default where
+ *                             HUGKS_UPDATE_MFMA_FACE does not apply, and the
  *                             only update of HUGKS_MIXED_BF16X3, which
- *                             refuses the others with HUGKS_E_UNSUPPORTED) */
+ *                             refuses the others with HUGKS_E_UNSUPPORTED)
+ *           HUGKS_UPDATE_MFMA_FACE  the node update that also forms the
+ *                             interior faces' node fluxes: their equilibrium
+ *                             part as further matrix-tile rows, the rest as
+ *                             a per-cell divergence (one pass over the cells
+ *                             in place of the pipeline's per-face node-flux
+ *                             kernel), so no per-face node fluxes are
+ *                             written.  With HUGKS_FLUX_CELLSUM or
+ *                             HUGKS_FLUX_SPLIT, not with HUGKS_MIXED_BF16X3,
+ *                             and distribution arrays under 4 GiB (the
+ *                             default where it applies); hugks_debug_fetch
+ *                             then returns the interior faces' node-flux
+ *                             slots as last written. */
 #define HUGKS_FLUX_FUSED   0
 #define HUGKS_FLUX_SPLIT   1
 #define HUGKS_FLUX_STENCIL 2
@@ -297,12 +310,14 @@ int hugks_debug_fetch(hugks_handle *h, int32_t which, double *out);
 #define HUGKS_UPDATE_LDS   0
 #define HUGKS_UPDATE_MFMA  1
 #define HUGKS_UPDATE_MFMA_NODE 2
+#define HUGKS_UPDATE_MFMA_FACE 3
 /* A handle that belongs to a batch (hugks_batch_handle) keeps the schedule
  * and tile its batch captured at create: setting either returns
  * HUGKS_E_ARG. */
 int hugks_set_schedule(hugks_handle *h, int32_t flux_kernel, int32_t update_kernel);
 int hugks_get_schedule(hugks_handle *h, int32_t *flux_kernel, int32_t *update_kernel);
-/* Tile parameter of HUGKS_UPDATE_MFMA_NODE: wavefronts per cell, 1 to 16
+/* Tile parameter of HUGKS_UPDATE_MFMA_NODE and HUGKS_UPDATE_MFMA_FACE:
+ * wavefronts per cell, 1 to 16
  * where a compiled kernel instance exists for the velocity grid and the
  * precision (e.g. 3, 5 and, FP32/FP64 only, 9 at nv = 33..48; 2 and 4 at
  * 17..32; 1 up to 16); 0 = the default: 3 for nv 33..48, 4 for 17..32, 1 up

@@ -167,6 +167,7 @@ def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
     monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
     monkeypatch.setenv("HUGKS_FLUX_KERNEL",
                        "cellsum" if sched[0] == capi.FLUX_CELLSUM else "split")
+    monkeypatch.setenv("HUGKS_UPDATE_KERNEL", "mfma-node")
     with capi.Batch(cs) as b:
         assert not b.grouped
         for s in b.solvers:
@@ -175,6 +176,7 @@ def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
         wl = [s.macro(with_q=False) for s in b.solvers]
     monkeypatch.delenv("HUGKS_BATCH_GROUPED")
     monkeypatch.delenv("HUGKS_FLUX_KERNEL")
+    monkeypatch.delenv("HUGKS_UPDATE_KERNEL")
     for c, a, x in zip(cs, wg, wl):
         assert np.array_equal(a, x)
         o = oracle_py.Oracle(c.with_precision(capi.FP64))

@@ -31,10 +31,13 @@ def rt_cartesian(n, nv, nm):
                 maxtime=200.0, case_id=6, knudsen=1e-3, phi=1.0, TwL=1.0, TwR=1.0)
 
 
-def gpu_run(case, steps):
+def gpu_run(case, steps, sched=None):
     d = case.desc
-    print(f"run {d.nx}x{d.ny} nv={d.nv} nm={d.nm} prec={d.precision} steps={steps}", flush=True)
+    print(f"run {d.nx}x{d.ny} nv={d.nv} nm={d.nm} prec={d.precision} steps={steps} "
+          f"sched={sched}", flush=True)
     with capi.Solver(case) as s:
+        if sched:
+            s.set_schedule(*sched)
         w0 = s.macro(with_q=False)
         info = s.advance(steps)
         w = s.macro(with_q=False)
@@ -99,17 +102,28 @@ def test_arrays_over_4gib_match_narrow_channel(have_gpu, prec):
     is uniform along x, 11000 x 64 cells at GH40 (4.5 GB per array in FP32
     storage, 9 GB in FP64), must give in every column exactly what the same
     channel 4 cells wide gives (x-face fluxes cancel exactly between equal
-    cells)."""
+    cells).  Both run the schedule the wide channel takes by default (the
+    face-forming update needs arrays under 4 GiB)."""
     p = dict(nx=4, ny=64, xlength=0.1, ylength=1.0, nv=40, vmax=5.0, Tref=1.0,
              quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=8, cfl=0.8,
              maxtime=200.0, case_id=5, knudsen=0.1, phi=0.05, TwL=1.0, TwR=1.2)
     steps = 3
-    narrow = cases.build_2d(p, prec)
-    _, _, wn = gpu_run(narrow, steps)
     wide_p = dict(p, nx=11000, xlength=0.1 * 11000 / 4)
     wide = cases.build_2d(wide_p, prec)
     assert wide.ncell * wide.desc.nv ** 2 * 4 > 2 ** 32
-    _, _, ww = gpu_run(wide, steps)
+    sched = (capi.FLUX_CELLSUM, capi.UPDATE_MFMA_NODE)
+    narrow = cases.build_2d(p, prec)
+    _, _, wn = gpu_run(narrow, steps, sched)
+    print(f"run 11000x64 nv=40 prec={prec} steps={steps} sched={sched}", flush=True)
+    with capi.Solver(wide) as s:
+        # the default where the face-forming update cannot run, which refuses
+        assert s.schedule() == sched
+        with pytest.raises(capi.HugksError) as ei:
+            s.set_schedule(capi.FLUX_CELLSUM, capi.UPDATE_MFMA_FACE)
+        assert ei.value.code == capi.E_UNSUPPORTED
+        assert s.schedule() == sched
+        s.advance(steps)
+        ww = s.macro(with_q=False)
     wn = wn.reshape(64, 4, -1)
     ww = ww.reshape(64, 11000, -1)
     for i in (0, 1, 5000, 10999):

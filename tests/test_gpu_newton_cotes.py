@@ -30,7 +30,8 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 NC = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*_nc*.npz")) if "_fd_" not in p)
 NC2D = [p for p in NC if os.path.basename(p).startswith("p2d")]
 SCHED = [(f, u) for f in (capi.FLUX_FUSED, capi.FLUX_SPLIT, capi.FLUX_STENCIL, capi.FLUX_CELLSUM)
-         for u in (capi.UPDATE_LDS, capi.UPDATE_MFMA, capi.UPDATE_MFMA_NODE)]
+         for u in (capi.UPDATE_LDS, capi.UPDATE_MFMA, capi.UPDATE_MFMA_NODE)] + [
+    (capi.FLUX_CELLSUM, capi.UPDATE_MFMA_FACE), (capi.FLUX_SPLIT, capi.UPDATE_MFMA_FACE)]
 
 
 def _ids(paths):

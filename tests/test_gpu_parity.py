@@ -118,11 +118,12 @@ def test_flux_kernels_agree(have_gpu, monkeypatch, mode):
         assert np.all(err <= tol), err
 
 
-@pytest.mark.parametrize("mode", ["lds", "mfma", "mfma-node"])
+@pytest.mark.parametrize("mode", ["lds", "mfma", "mfma-node", "mfma-face"])
 def test_update_kernels_agree(have_gpu, monkeypatch, mode):
-    """Both cell-update implementations (LDS dot products; MFMA Hermite
-    tiles) against the oracle, FP64 and mixed, at a velocity grid that
-    is not a multiple of 16 and Nm = 10."""
+    """The cell-update implementations (LDS dot products; MFMA Hermite
+    tiles; MFMA node update; the node update forming the interior faces'
+    equilibrium flux) against the oracle, FP64 and mixed, at a velocity
+    grid that is not a multiple of 16 and Nm = 10."""
     monkeypatch.setenv("HUGKS_UPDATE_KERNEL", mode)
     for prec, tol in ((capi.FP64, TOL_FP64), (capi.MIXED_FP32, TOL_MIXED)):
         case = cases.build_2d(rt(nx=8, ny=10, nv=20, nm=10), prec, rt_variant="cartesian", seed=7)
