@@ -391,7 +391,7 @@ def bench_c4(a):
     # roofline of the dominant grouped kernel: algorithmic bytes of one
     # launch (summed over the cases it steps) over its average duration
     kmap = {"kg_slope": "slope2d", "kg_fsum": "flux_sum", "kg_fnode": "flux_node",
-            "kg_update": "update2d", "kg_cell_sum": "slope_sum"}
+            "kg_update": "update2d", "kg_cell_sum": "slope_sum", "kg_fdiv": "flux_div"}
     roof = None
     kt = rm["kt"]
     timed = [k for k in kt if k in kmap]
@@ -401,7 +401,7 @@ def bench_c4(a):
         # of one step over the time of all of its launches in one step
         steps_timed = kt["kg_timestep"][1] if "kg_timestep" in kt else 2
         ms_step = kt[dom][0] / steps_timed
-        ab = sum(algorithmic_bytes(c, kmap[dom]) for c in mixed)
+        ab = sum(algorithmic_bytes(c, kmap[dom], sc[1]) for c, sc in zip(mixed, rm["scheds"]))
         gbs = ab / (ms_step * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,

@@ -1334,7 +1334,7 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
         blocks.push_back(acc);
         return (int)tabs.size() - 1;
     };
-    std::vector<std::pair<int, int>> one, sl, fs, ff, fn, wl, cp;
+    std::vector<std::pair<int, int>> one, sl, fs, ff, fn, wl, cp, fd;
     std::map<std::tuple<int, int, int>, std::vector<std::pair<int, int>>> upd;
     std::map<std::tuple<int, int, int>, size_t> upd_lds;
     for (int i : ids) {
@@ -1348,6 +1348,7 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
         fn.push_back({i, s4});
         if (s5 > 0) wl.push_back({i, s5});
         cp.push_back({i, s6});
+        fd.push_back({i, group_fdiv_blocks(h->L)});
         group_update_key(h->L, &e, &t, &v);
         const auto key = std::make_tuple(e, t, v);
         upd[key].push_back({i, h->L.c.nx * h->L.c.ny});
@@ -1358,13 +1359,15 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
     if (upd.size() > sizeof(P.upd) / sizeof(P.upd[0])) return hipErrorInvalidValue;
     P.fp64 = b->cases[ids[0]]->fp64;
     P.cellsum = b->cases[ids[0]]->L.flux_mode == HUGKS_FLUX_CELLSUM;
+    P.face = b->cases[ids[0]]->L.update_mode == HUGKS_UPDATE_MFMA_FACE;
     P.cs_vec = true;
     for (int i : ids) {
         const int es = b->cases[i]->fp64 ? 8 : 4;
         P.cs_vec = P.cs_vec && (b->cases[i]->L.c.nv * es) % 16 == 0;
     }
     const int k_ts = table(one), k_sl = table(sl), k_fs = table(fs), k_ff = table(ff),
-              k_fn = table(fn), k_wl = wl.empty() ? -1 : table(wl), k_cp = table(cp);
+              k_fn = table(fn), k_wl = wl.empty() ? -1 : table(wl), k_cp = table(cp),
+              k_fd = table(fd);
     std::vector<int> k_up;
     for (auto& kv : upd) k_up.push_back(table(kv.second));
     size_t ints = 0;
@@ -1392,6 +1395,7 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
     P.t_wall = k_wl >= 0 ? tp(k_wl) : nullptr;
     P.g_wall = k_wl >= 0 ? blocks[k_wl] : 0;
     P.t_cpre = tp(k_cp); P.g_cpre = blocks[k_cp];
+    P.t_fdiv = tp(k_fd); P.g_fdiv = blocks[k_fd];
     P.nupd = 0;
     int q = 0;
     for (auto& kv : upd) {
@@ -1402,6 +1406,7 @@ static hipError_t batch_group_plan_of(hugks_batch* b, const std::vector<int>& id
         u.blocks = blocks[k_up[q]];
         u.tab = tp(k_up[q]);
         u.lds = upd_lds[kv.first];
+        u.ff = P.face;
         ++q;
         e = prepare_group_update(P.fp64, u);
         if (e != hipSuccess) return e;
@@ -1422,13 +1427,21 @@ static hipError_t batch_group_plan(hugks_batch* b) {
     for (int i = 0; i < n; ++i)
         if (b->in_group[i]) ids[b->cases[i]->fp64 ? 1 : 0].push_back(i);
     // the grouped schedule: the cell-sum flux where every case of the
-    // storage type supports it, else the split flux; the MFMA node update
+    // storage type supports it, else the split flux; the node update forming
+    // the interior faces' equilibrium flux where every case supports it
+    // (HUGKS_UPDATE_KERNEL=mfma-node: the plain node update), else the MFMA
+    // node update
+    const char* um = getenv("HUGKS_UPDATE_KERNEL");
+    const bool face_ok = !(um && strcmp(um, "mfma-node") == 0);
     for (int t = 0; t < 2; ++t) {
-        bool cs = true;
-        for (int i : ids[t]) cs = cs && b->cases[i]->L.cs_js >= 0;
+        bool cs = true, face = face_ok;
+        for (int i : ids[t]) {
+            cs = cs && b->cases[i]->L.cs_js >= 0;
+            face = face && update2d_mxf_supported(b->cases[i]->L, b->cases[i]->L.update_nw);
+        }
         for (int i : ids[t])
             hugks_set_schedule(b->cases[i], cs ? HUGKS_FLUX_CELLSUM : HUGKS_FLUX_SPLIT,
-                               HUGKS_UPDATE_MFMA_NODE);
+                               face ? HUGKS_UPDATE_MFMA_FACE : HUGKS_UPDATE_MFMA_NODE);
     }
     for (int i = 0; i < n; ++i) Ls[i] = b->cases[i]->L;
     e = hipMemcpy(lmem, Ls.data(), n * sizeof(Launch2D), hipMemcpyHostToDevice);
@@ -1460,7 +1473,7 @@ int hugks_batch_create(int32_t n, const hugks_desc* descs, int32_t flags, hugks_
         int rc = tune_case(&descs[i], b->in_group[i] != 0, &fk, &uk, &nw, nullptr);
         if (rc == HUGKS_OK && b->cases[i]->dim == 2) {
             rc = hugks_set_schedule(b->cases[i], fk, uk);
-            if (rc == HUGKS_OK && uk == HUGKS_UPDATE_MFMA_NODE)
+            if (rc == HUGKS_OK && uk >= HUGKS_UPDATE_MFMA_NODE)
                 rc = hugks_set_update_waves(b->cases[i], nw);
             if (rc != HUGKS_OK) g_create_error = b->cases[i]->err;
         }

@@ -127,6 +127,7 @@ hipError_t launch_convert_in(const double* src, void* dst, bool dst_fp64, size_t
 // one batch step; updates are grouped by kernel instance.
 struct GroupUpd {
     int nw, maxp, nvc;   // the node-update instance (mx_key)
+    bool ff;             // the instance forming the interior faces' equilibrium flux
     int blocks;
     size_t lds;          // largest LDS of the instance's cases
     const int* tab;
@@ -135,9 +136,10 @@ struct GroupPlan {
     bool fp64;
     bool cellsum;        // the cell-sum pipeline (every case supports it), else the split one
     bool cs_vec;         // its 16-byte loads (every case's velocity row a multiple of 16 bytes)
+    bool face;           // update mode 3 (every case supports it): k_fdiv instead of K5
     const Launch2D* Ls;  // per-case arguments, device memory
-    const int *t_ts, *t_slope, *t_fsum, *t_fface, *t_fnode, *t_wall, *t_cpre, *t_fin;
-    int g_ts, g_slope, g_fsum, g_fface, g_fnode, g_wall, g_cpre, g_fin;
+    const int *t_ts, *t_slope, *t_fsum, *t_fface, *t_fnode, *t_wall, *t_cpre, *t_fin, *t_fdiv;
+    int g_ts, g_slope, g_fsum, g_fface, g_fnode, g_wall, g_cpre, g_fin, g_fdiv;
     GroupUpd upd[16];
     int nupd;
     LaunchHook hook;     // may be null (timing of eager, not captured, launches)
@@ -153,6 +155,7 @@ void group_blocks(const Launch2D& L, int* slope, int* fsum, int* fface, int* fno
 // per wavefront, compiled-in nv or 0) and its dynamic LDS bytes
 void group_update_key(const Launch2D& L, int* nw, int* maxp, int* nvc);
 size_t group_update_lds(const Launch2D& L);
+int group_fdiv_blocks(const Launch2D& L);   // workgroups of k_fdiv for a case
 
 // 1D path
 hipError_t launch_step1d(const Launch2D& L, hipStream_t s);

@@ -961,14 +961,16 @@ __device__ __forceinline__ void k_fdiv_body(
     const FK kD = face_k(inD, false, nf * ny + tiled_index(i, j - oy, nx, nr), hd);
     const FK kU = face_k(inU, false, nf * ny + tiled_index(i, j + 1 - oy, nx, nr), hu);
     // one face's flux of h and b at a node: normal and tangential velocity,
-    // upwind side, the upwind f and slopes (normal first)
+    // upwind side, the upwind f and slopes (normal first).  Written as
+    // explicit fused multiply-adds, so every instance (one or VW nodes per
+    // lane, alone or in a grouped launch) rounds the same way.
     auto flux = [](const FK& k, A vn, A vt, int d, A hu_, A sn, A st_, A bu, A bn, A bt, A& fh,
                    A& fb) {
         const A ch = vn * k.f4;
-        const A cn = vn * ((d ? k.h4 : -k.h4) + k.f6n - k.f5 * vn);
-        const A ct = vn * (k.f6t - k.f5t * vt);
-        fh = ch * hu_ + cn * sn + ct * st_;
-        fb = ch * bu + cn * bn + ct * bt;
+        const A cn = vn * fma(-k.f5, vn, (d ? k.h4 : -k.h4) + k.f6n);
+        const A ct = vn * fma(-k.f5t, vt, k.f6t);
+        fh = fma(ch, hu_, fma(cn, sn, ct * st_));
+        fb = fma(ch, bu, fma(cn, bn, ct * bt));
     };
     const size_t ldv = (size_t)c.ldv;
     const uint32_t rowB = (uint32_t)(ldv * sizeof(T));
@@ -1021,7 +1023,6 @@ __device__ __forceinline__ void k_fdiv_body(
             const bool in = fa == 0 ? inL : fa == 1 ? inR : fa == 2 ? inD : inU;
             const FK& k = fa == 0 ? kL : fa == 1 ? kR : fa == 2 ? kD : kU;
             const uint32_t bn = fa == 0 ? bL : fa == 1 ? bR : fa == 2 ? bD : bU;
-            const A sg = lo ? (A)1 : (A)-1;
             if (in) {
                 int d[VW];
                 bool nb[VW];
@@ -1047,15 +1048,15 @@ __device__ __forceinline__ void k_fdiv_body(
                     else
                         flux(k, vv, -uu[q], d[q], (A)x.h[q], (A)x.sy[q], (A)x.sx[q], (A)x.b[q],
                              (A)x.by[q], (A)x.bx[q], fh, fb);
-                    th[q] += sg * fh;
-                    tb[q] += sg * fb;
+                    th[q] = lo ? th[q] + fh : th[q] - fh;
+                    tb[q] = lo ? tb[q] + fb : tb[q] - fb;
                 }
             } else {
                 const V wh = DV::ld(rW[fa], oe), wb = DV::ld(rW[4 + fa], oe);
 #pragma unroll
                 for (int q = 0; q < VW; ++q) {
-                    th[q] += sg * (A)wh[q];
-                    tb[q] += sg * (A)wb[q];
+                    th[q] = lo ? th[q] + (A)wh[q] : th[q] - (A)wh[q];
+                    tb[q] = lo ? tb[q] + (A)wb[q] : tb[q] - (A)wb[q];
                 }
             }
         }

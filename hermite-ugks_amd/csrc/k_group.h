@@ -143,16 +143,32 @@ __global__ __launch_bounds__(256) void kg_cell_pre(const int* __restrict__ tab,
                     g.nblk);
 }
 
-template <typename T, int NW, int MAXP, int NVC>
+// FF: the node update forming the interior faces' equilibrium flux (update
+// mode 3), with the waves per SIMD of its single-case kernel
+template <typename T, int NW, int MAXP, int NVC, bool FF>
 __global__ __launch_bounds__(NW * WAVE)
-HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? 3 : 1) void kg_update(
+HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? (FF && NVC ? 4 : 3) : 1) void kg_update(
     const int* __restrict__ tab, const Launch2D* __restrict__ Ls) {
     const GroupIdx g = group_index(tab);
     const Launch2D& L = Ls[g.cs];
-    k_update2d_mx_body<T, NW, MAXP, NVC>(L.c, L.st, L.cellpre, (T*)L.h, (T*)L.b,
-                                         (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,
-                                         (const T*)L.hfb, (const typename Arith<T>::A*)L.wnode,
-                                         L.u1, L.v1, g.bid, g.nblk);
+    const MxFace fx{L.dvh, L.dvb, L.fcoef, L.acc_v, L.acc_h};
+    k_update2d_mx_body<T, NW, MAXP, NVC, false, FF>(
+        L.c, L.st, L.cellpre, (T*)L.h, (T*)L.b, (const T*)L.vfh, (const T*)L.vfb,
+        (const T*)L.hfh, (const T*)L.hfb, (const typename Arith<T>::A*)L.wnode, L.u1, L.v1,
+        g.bid, g.nblk, fx);
+}
+
+// the per-cell face divergence of update mode 3 (k_fdiv)
+template <typename T, int VW>
+__global__ __launch_bounds__(WAVE * DIV_CPW)
+HUGKS_WAVES_PER_EU(sizeof(T) == 4 ? 4 : 3) void kg_fdiv(const int* __restrict__ tab,
+                                                       const Launch2D* __restrict__ Ls) {
+    const GroupIdx g = group_index(tab);
+    const Launch2D& L = Ls[g.cs];
+    k_fdiv_body<T, VW>(L.c, L.st, (const T*)L.h, (const T*)L.b, (const T*)L.shx,
+                       (const T*)L.shy, (const T*)L.sbx, (const T*)L.sby, L.u1, L.v1, L.acc_v,
+                       L.acc_h, L.fcoef, (const T*)L.vfh, (const T*)L.vfb, (const T*)L.hfh,
+                       (const T*)L.hfb, (T*)L.dvh, (T*)L.dvb, g.bid, g.nblk);
 }
 
 __global__ __launch_bounds__(RED_NT) void kg_finalize(const int* __restrict__ tab,

@@ -25,20 +25,27 @@ void group_update_key(const Launch2D& L, int* nw, int* maxp, int* nvc) {
 }
 
 size_t group_update_lds(const Launch2D& L) {
-    return mx_lds_bytes(L.c.nv, L.fp64 ? sizeof(double) : sizeof(float));
+    return mx_lds_bytes(L.c.nv, L.fp64 ? sizeof(double) : sizeof(float), L.update_mode == 3);
 }
+
+int group_fdiv_blocks(const Launch2D& L) { return (L.c.nx * L.c.ny + DIV_CPW - 1) / DIV_CPW; }
 
 template <typename T>
 static hipError_t group_update_t(const GroupUpd& u, const Launch2D* Ls, hipStream_t s,
                                  bool prepare) {
 #define HUGKS_GUPD_CASE(W, P, V)                                                            \
     if (u.nw == W && u.maxp == P && u.nvc == V) {                                           \
+        const void* kf = u.ff ? (const void*)kg_update<T, W, P, V, true>                    \
+                              : (const void*)kg_update<T, W, P, V, false>;                  \
         if (prepare)                                                                        \
-            return hipFuncSetAttribute((const void*)kg_update<T, W, P, V>,                  \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,          \
+            return hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                        (int)u.lds);                                         \
-        hipLaunchKernelGGL((kg_update<T, W, P, V>), dim3(u.blocks), dim3(W * WAVE), u.lds, \
-                           s, u.tab, Ls);                                                   \
+        if (u.ff)                                                                           \
+            hipLaunchKernelGGL((kg_update<T, W, P, V, true>), dim3(u.blocks),               \
+                               dim3(W * WAVE), u.lds, s, u.tab, Ls);                        \
+        else                                                                                \
+            hipLaunchKernelGGL((kg_update<T, W, P, V, false>), dim3(u.blocks),              \
+                               dim3(W * WAVE), u.lds, s, u.tab, Ls);                        \
         return hipGetLastError();                                                           \
     }
     HUGKS_UPDX(HUGKS_GUPD_CASE)
@@ -86,7 +93,7 @@ static hipError_t group_step_t(const GroupPlan& P, hipStream_t s) {
             hipLaunchKernelGGL((kg_fface), dim3(P.g_fface), dim3(256), 0, s, P.t_fface, P.Ls);
         }
     }
-    {
+    if (!P.face) {
         HookCall hc(P.hook, P.hook_ctx, "kg_fnode");
         hipLaunchKernelGGL((kg_fnode<T>), dim3(P.g_fnode), dim3(WAVE * SPLIT_FPW), 0, s,
                            P.t_fnode, P.Ls);
@@ -94,6 +101,13 @@ static hipError_t group_step_t(const GroupPlan& P, hipStream_t s) {
     if (P.g_wall > 0) {
         HookCall hc(P.hook, P.hook_ctx, "kg_wall");
         hipLaunchKernelGGL((kg_wall<T>), dim3(P.g_wall), dim3(256), 0, s, P.t_wall, P.Ls);
+    }
+    if (P.face) {
+        // after the walls: the divergence reads their node fluxes
+        constexpr int VW = 16 / (int)sizeof(T);
+        HookCall hc(P.hook, P.hook_ctx, "kg_fdiv");
+        auto kd = P.cs_vec ? kg_fdiv<T, VW> : kg_fdiv<T, 1>;
+        hipLaunchKernelGGL(kd, dim3(P.g_fdiv), dim3(WAVE * DIV_CPW), 0, s, P.t_fdiv, P.Ls);
     }
     {
         HookCall hc(P.hook, P.hook_ctx, "kg_cell_pre");

@@ -29,7 +29,8 @@ module hugks_capi
   integer(c_int32_t), parameter :: HUGKS_STOP_NONE = 0, HUGKS_STOP_MAXTIME = 1, &
        HUGKS_STOP_RESIDUAL = 2, HUGKS_STOP_TIME = 3, HUGKS_STOP_STEADY = 4, HUGKS_STOP_ERROR = 5
   integer(c_int32_t), parameter :: HUGKS_FLUX_FUSED = 0, HUGKS_FLUX_SPLIT = 1, HUGKS_FLUX_STENCIL = 2, &
-       HUGKS_UPDATE_LDS = 0, HUGKS_UPDATE_MFMA = 1, HUGKS_BATCH_AUTOTUNE = 1
+       HUGKS_FLUX_CELLSUM = 3, HUGKS_UPDATE_LDS = 0, HUGKS_UPDATE_MFMA = 1, &
+       HUGKS_UPDATE_MFMA_NODE = 2, HUGKS_UPDATE_MFMA_FACE = 3, HUGKS_BATCH_AUTOTUNE = 1
 
   type, bind(c) :: hugks_desc
     integer(c_int32_t) :: abi_version = HUGKS_ABI_VERSION

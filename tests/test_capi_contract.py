@@ -172,3 +172,31 @@ def test_bf16x3_refused_in_1d():
     rc, msg = create(d)
     assert rc == capi.E_UNSUPPORTED, (rc, msg)
     assert "BF16X3" in msg and "2D" in msg
+
+
+def header_defines():
+    return {k: int(v) for k, v in re.findall(r"^#define\s+(HUGKS_\w+)\s+(-?\d+)\b", header_text(),
+                                             flags=re.M)}
+
+
+def test_fortran_module_constants_match_the_header():
+    """The Fortran binding's named constants carry the header's values, and
+    it names every schedule the header defines."""
+    with open(os.path.join(ROOT, "hermite-ugks_amd", "driver", "hugks_capi.f90")) as f:
+        src = f.read()
+    fort = {k: int(v) for k, v in re.findall(r"\b(HUGKS_\w+)\s*=\s*(-?\d+)\b", src)}
+    hdr = header_defines()
+    common = sorted(set(fort) & set(hdr))
+    assert common
+    for k in common:
+        assert fort[k] == hdr[k], k
+    for k in hdr:
+        if k.startswith(("HUGKS_FLUX_", "HUGKS_UPDATE_")):
+            assert k in fort, k
+
+
+def test_python_schedule_constants_match_the_header():
+    hdr = header_defines()
+    for k, v in hdr.items():
+        if k.startswith(("HUGKS_FLUX_", "HUGKS_UPDATE_")):
+            assert getattr(capi, k[len("HUGKS_"):]) == v, k

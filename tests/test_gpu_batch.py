@@ -158,8 +158,8 @@ def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
         assert b.grouped
         sched = b.solvers[0].schedule()
         assert all(s.schedule() == sched for s in b.solvers)
-        assert sched in ((capi.FLUX_CELLSUM, capi.UPDATE_MFMA_NODE),
-                         (capi.FLUX_SPLIT, capi.UPDATE_MFMA_NODE))
+        assert sched in ((capi.FLUX_CELLSUM, capi.UPDATE_MFMA_FACE),
+                         (capi.FLUX_SPLIT, capi.UPDATE_MFMA_FACE))
         b.advance(5)
         wg = [s.macro(with_q=False) for s in b.solvers]
     # case by case with the schedule of the grouped launches, which the
@@ -167,7 +167,7 @@ def test_grouped_batch_parity_and_exits(have_gpu, monkeypatch):
     monkeypatch.setenv("HUGKS_BATCH_GROUPED", "0")
     monkeypatch.setenv("HUGKS_FLUX_KERNEL",
                        "cellsum" if sched[0] == capi.FLUX_CELLSUM else "split")
-    monkeypatch.setenv("HUGKS_UPDATE_KERNEL", "mfma-node")
+    monkeypatch.setenv("HUGKS_UPDATE_KERNEL", "mfma-face")
     with capi.Batch(cs) as b:
         assert not b.grouped
         for s in b.solvers:
@@ -233,7 +233,7 @@ def test_tune_tiles_picks_a_compiled_tile(have_gpu):
     assert ms > 0
     with capi.Solver(c) as s:
         s.set_schedule(fk, uk)
-        if uk == capi.UPDATE_MFMA_NODE:
+        if uk in (capi.UPDATE_MFMA_NODE, capi.UPDATE_MFMA_FACE):
             s.set_update_waves(nw)
             assert s.update_waves() == nw
     again = capi.tune_tiles(c)
@@ -255,3 +255,22 @@ def test_batch_cases_refuse_schedule_changes(have_gpu):
             s.set_update_waves(1)
         assert ei.value.code == capi.E_ARG
         assert s.schedule() == before
+
+
+def test_grouped_batch_plain_node_update(have_gpu, monkeypatch):
+    """The grouped launches with the plain MFMA node update (the per-face
+    node-flux kernel instead of the per-cell divergence), asked for with
+    HUGKS_UPDATE_KERNEL=mfma-node: each case as stepped alone, bitwise."""
+    monkeypatch.setenv("HUGKS_UPDATE_KERNEL", "mfma-node")
+    cs = grouped_cases(capi.MIXED_FP32)
+    with capi.Batch(cs) as b:
+        assert b.grouped
+        scheds = [s.schedule() for s in b.solvers]
+        assert all(sc[1] == capi.UPDATE_MFMA_NODE for sc in scheds)
+        b.advance(9)
+        wg = [s.macro(with_q=False) for s in b.solvers]
+    for c, sc, a in zip(cs, scheds, wg):
+        with capi.Solver(c) as s:
+            s.set_schedule(*sc)
+            s.advance(9)
+            assert np.array_equal(s.macro(with_q=False), a)
