@@ -614,6 +614,7 @@ int hugks_create(const hugks_desc* d, hugks_handle** out) {
             if (strcmp(um, "mfma-node") == 0 && mx) L.update_mode = 2;
             if (strcmp(um, "mfma-face") == 0 && mxf) L.update_mode = 3;
         }
+        if (L.update_mode >= 2) L.update_nw = update2d_default_waves(L, L.update_mode);
         if (L.update_lds > (size_t)prop.sharedMemPerBlockOptin && prop.sharedMemPerBlockOptin > 0) {
             release(h);
             delete h;
@@ -1097,7 +1098,7 @@ int hugks_set_update_waves(hugks_handle* h, int32_t waves) {
     if (h->in_batch)
         return fail(h, HUGKS_E_ARG, "the case belongs to a batch, whose graphs fix its schedule");
     if (h->dim != 2) return HUGKS_OK;
-    const int nw = waves > 0 ? waves : update2d_mx_default_waves(h->L.c);
+    const int nw = waves > 0 ? waves : update2d_default_waves(h->L, h->L.update_mode);
     if (nw > 16 || !update2d_mx_supported(h->L.c, h->fp64, h->b3, h->L.lds_limit, nw) ||
         (h->L.update_mode == HUGKS_UPDATE_MFMA_FACE && !update2d_mxf_supported(h->L, nw)))
         return fail(h, HUGKS_E_UNSUPPORTED,

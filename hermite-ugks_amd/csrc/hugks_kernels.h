@@ -167,6 +167,8 @@ bool update2d_mfma_supported(const Consts& c);
 // FP32 storage only)
 bool update2d_mx_supported(const Consts& c, bool fp64, bool b3, size_t lds_limit, int nw);
 int update2d_mx_default_waves(const Consts& c);
+// the default wavefronts per cell of update mode 2 or 3 for the case
+int update2d_default_waves(const Launch2D& L, int mode);
 // update mode 3 with nw wavefronts per cell: as mode 2 (not b3), the LDS of its
 // face rows within the limit, the divergence arrays allocated and each
 // distribution array under 4 GiB - 16 bytes (32-bit offsets)

@@ -320,8 +320,9 @@ int hugks_get_schedule(hugks_handle *h, int32_t *flux_kernel, int32_t *update_ke
  * wavefronts per cell, 1 to 16
  * where a compiled kernel instance exists for the velocity grid and the
  * precision (e.g. 3, 5 and, FP32/FP64 only, 9 at nv = 33..48; 2 and 4 at
- * 17..32; 1 up to 16); 0 = the default: 3 for nv 33..48, 4 for 17..32, 1 up
- * to 16.  HUGKS_E_UNSUPPORTED when no instance exists. */
+ * 17..32; 1 up to 16); 0 = the default for the handle's update: 3 for nv
+ * 33..48 (5 for HUGKS_UPDATE_MFMA_FACE in FP64), 4 for 17..32, 1 up to 16;
+ * hugks_create sets it.  HUGKS_E_UNSUPPORTED when no instance exists. */
 int hugks_set_update_waves(hugks_handle *h, int32_t waves);
 int hugks_get_update_waves(hugks_handle *h, int32_t *waves);
 

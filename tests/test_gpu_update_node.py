@@ -125,3 +125,21 @@ def test_face_update_weights_not_a_tensor_product(have_gpu):
     w *= 1.0 + 1e-3 * np.random.default_rng(3).uniform(-1.0, 1.0, w.shape)
     print("face-forming update, split flux, perturbed weights nv=16", flush=True)
     check(case, 5, sched=FACE_SPLIT)
+
+
+def test_default_waves_follow_the_update(have_gpu):
+    """The default wavefronts per cell at nv = 40: three for both node
+    updates in FP32 storage and for the plain node update in FP64, five
+    for the face-forming update in FP64 (its faster tile there)."""
+    case = cases.build_2d(rt(nx=4, ny=4, nv=40, nm=8), capi.FP64, rt_variant="cartesian")
+    with capi.Solver(case) as s:
+        assert s.schedule()[1] == capi.UPDATE_MFMA_FACE
+        assert s.update_waves() == 5
+        s.set_update_waves(0)
+        assert s.update_waves() == 5
+        s.set_schedule(capi.FLUX_CELLSUM, capi.UPDATE_MFMA_NODE)
+        s.set_update_waves(0)
+        assert s.update_waves() == 3
+    with capi.Solver(case.with_precision(capi.MIXED_FP32)) as s:
+        assert s.schedule()[1] == capi.UPDATE_MFMA_FACE
+        assert s.update_waves() == 3
