@@ -563,6 +563,21 @@ def main():
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(wl, dom),
                 "kernel": dom,
                 "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": ab}
+    # every step kernel with algorithmic bytes: GB/s on them and on the
+    # measured traffic of the committed counter summary, as fractions of peak
+    per_kernel = {}
+    for k, (tot, n) in kt.items():
+        kb = algorithmic_bytes(mixed, k, rm["schedule"][1])
+        if kb is None or n == 0:
+            continue
+        kms = tot / n
+        g = kb / (kms * 1e-3) / 1e9
+        tr = traffic_from_profiles(wl, k)
+        per_kernel[k] = {"ms": round(kms, 4), "algorithmic_GBs": round(g, 1),
+                         "frac": round(g / HBM_PEAK_GBS, 4),
+                         "traffic_GBs": round(tr / (kms * 1e-3) / 1e9, 1) if tr else None,
+                         "traffic_frac": round(tr / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                         if tr else None}
     # the Hermite force on the matrix cores: its algorithmic flops over the
     # update kernel's time (the bound of that kernel is HBM; this is the
     # MFMA utilisation the force term reaches inside it)
@@ -597,6 +612,7 @@ def main():
         "schedule": {"flux": SCHED_NAMES["flux"].get(rm["schedule"][0]),
                      "update": SCHED_NAMES["update"].get(rm["schedule"][1])},
         "roofline": roof,
+        "kernels_roofline": per_kernel,
         "mfma": mfma,
         "cpu_baseline": cpu,
         "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
