@@ -215,6 +215,18 @@ struct MxFace {
     const double *fcoef, *acc_v, *acc_h;
 };
 
+// Slots of a face's factors in the FF table (node arithmetic): force shift,
+// face state, Mt(1..3), K/(2 lam), (K^2 + 2K)/(4 lam^2), the Shakhov factor,
+// heat flux, tau Mt(2) fn and ft, the normal rows' prefactor (divergence
+// sign x 1/(area (1 + dt/2tau)) x face length x rho lam/pi), then the
+// micro-slope coefficients a0, a1, a2, a3/2 of the tangential slope, the
+// time slope, and the left and right spatial slopes.
+enum : int {
+    FR_VS, FR_U, FR_V, FR_LAM, FR_MT1, FR_MT2, FR_MT3, FR_KFAC, FR_MXI2, FR_CSA, FR_Q0, FR_Q1,
+    FR_TM2FN, FR_TM2FT, FR_PRE, FR_PB, FR_PT = FR_PB + 4, FR_SL = FR_PT + 4, FR_SR = FR_SL + 4,
+    FR_N = FR_SR + 4
+};
+
 template <typename T, int NW, int MAXP, int NVC, bool SB3 = false, bool FF = false>
 __device__ __forceinline__ void k_update2d_mx_body(
     const Consts& c, const DevState* __restrict__ st, const double* __restrict__ cp,
@@ -231,8 +243,9 @@ __device__ __forceinline__ void k_update2d_mx_body(
     HIP_DYNAMIC_SHARED(double, smem_raw)
     // unscaled coefficients C10, C30, C12, C01, C03, C21 of h, C10, C01 of b
     __shared__ double qm[8];
-    // FF: the force shift 0.5 fn dt of each face (left, right, down, up)
-    __shared__ typename Arith<T>::A fk[4];
+    // FF: the node-independent factors of each face's rows (left, right,
+    // down, up; FR_* slots), in the node arithmetic
+    __shared__ typename Arith<T>::A fr[4][FR_N];
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const int cell = tiled_cell((int)xcd_block(bid, nblk), c.nx, c.ny);
@@ -303,11 +316,58 @@ __device__ __forceinline__ void k_update2d_mx_body(
     };
     if constexpr (FF) {
         if (tid < 4) {
+            // the face's state, time coefficients, slopes and force, with the
+            // divergence sign, 1/(area (1 + dt/2tau)) and the face length
+            // folded into the prefactor of its normal rows
             const int fa = tid;
+            const bool isx = fa < 2;
             const bool in = face_pick(fa, inL, inR, inD, inU);
-            const int slot = face_pick(fa, vl, vr, hd, hu);
-            const double* ac = fa < 2 ? fx.acc_v : fx.acc_h;
-            fk[fa] = in ? (A)(0.5 * ac[2 * slot] * dt) : (A)0;
+            A k[FR_N];
+#pragma unroll
+            for (int q = 0; q < FR_N; ++q) k[q] = (A)0;
+            if (in) {
+                const int nf = xper ? nx + 1 : nx - 1, nr = yper ? ny + 1 : ny - 1;
+                const int ox = xper ? 0 : 1, oy = yper ? 0 : 1;
+                const int g = face_pick(fa, tiled_index(i - ox, j, nf, ny),
+                                        tiled_index(i + 1 - ox, j, nf, ny),
+                                        nf * ny + tiled_index(i, j - oy, nx, nr),
+                                        nf * ny + tiled_index(i, j + 1 - oy, nx, nr));
+                const int slot = face_pick(fa, vl, vr, hd, hu);
+                const double* fc = fx.fcoef + (size_t)g * FACE_COEF_LD;
+                const double* ac = isx ? fx.acc_v : fx.acc_h;
+                const double fn = ac[2 * slot], ft = ac[2 * slot + 1];
+                const double lam = fc[FC_P + 3], rho = fc[FC_P];
+                const double cdd = iden / (c.dx * c.dy);
+                k[FR_VS] = (A)(0.5 * fn * dt);
+                k[FR_U] = (A)fc[FC_P + 1];
+                k[FR_V] = (A)fc[FC_P + 2];
+                k[FR_LAM] = (A)lam;
+                k[FR_MT1] = (A)fc[FC_MT];
+                k[FR_MT2] = (A)fc[FC_MT + 1];
+                k[FR_MT3] = (A)fc[FC_MT + 2];
+                k[FR_KFAC] = (A)(c.K / (2.0 * lam));
+                k[FR_MXI2] = (A)((c.K * c.K + 2.0 * c.K) / (4.0 * lam * lam));
+                k[FR_CSA] = (A)(c.shk * (1.0 - c.prandtl) * lam * lam / rho);
+                k[FR_Q0] = (A)fc[FC_Q];
+                k[FR_Q1] = (A)fc[FC_Q + 1];
+                k[FR_TM2FN] = (A)(fc[FC_TAU] * fc[FC_MT + 1] * fn);
+                k[FR_TM2FT] = (A)(fc[FC_TAU] * fc[FC_MT + 1] * ft);
+                k[FR_PRE] = (A)((fa & 1) ? -cdd : cdd) * (A)(isx ? c.dy : c.dx) *
+                            (A)(rho * (lam / c.pi));
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    k[FR_PB + q] = (A)fc[FC_AB + q];
+                    k[FR_PT + q] = (A)fc[FC_AT + q];
+                    k[FR_SL + q] = (A)fc[FC_AL + q];
+                    k[FR_SR + q] = (A)fc[FC_AR + q];
+                }
+                k[FR_PB + 3] = (A)(0.5 * fc[FC_AB + 3]);
+                k[FR_PT + 3] = (A)(0.5 * fc[FC_AT + 3]);
+                k[FR_SL + 3] = (A)(0.5 * fc[FC_AL + 3]);
+                k[FR_SR + 3] = (A)(0.5 * fc[FC_AR + 3]);
+            }
+#pragma unroll
+            for (int q = 0; q < FR_N; ++q) fr[fa][q] = k[q];
         }
     }
 
@@ -580,48 +640,30 @@ __device__ __forceinline__ void k_update2d_mx_body(
     E* const TyB = sm + ly.oFB;    // [8][sH]: y faces' T_k over u
     E* const NxB = Hx;             // [2][8][sH]: x faces' N_k over u (Hx is dead)
     if constexpr (FF) {
-        const double cdd = iden / (c.dx * c.dy);
         for (int t = tid; t < 4 * np; t += NT) {
             const int fa = t / np, idx = t - fa * np;
             const bool isx = fa < 2;
             const bool in = face_pick(fa, inL, inR, inD, inU);
             A nh[4] = {0, 0, 0, 0}, nb[4] = {0, 0, 0, 0}, tk[4] = {0, 0, 0, 0};
             if (in && idx < nv) {
-                const int nf = xper ? nx + 1 : nx - 1, nr = yper ? ny + 1 : ny - 1;
-                const int ox = xper ? 0 : 1, oy = yper ? 0 : 1;
-                const int g = face_pick(fa, tiled_index(i - ox, j, nf, ny),
-                                        tiled_index(i + 1 - ox, j, nf, ny),
-                                        nf * ny + tiled_index(i, j - oy, nx, nr),
-                                        nf * ny + tiled_index(i, j + 1 - oy, nx, nr));
-                const int slot = face_pick(fa, vl, vr, hd, hu);
-                const double* fc = fx.fcoef + (size_t)g * FACE_COEF_LD;
-                const double* ac = isx ? fx.acc_v : fx.acc_h;
-                const A fl = (A)(isx ? c.dy : c.dx);
-                const A sg = (A)((fa & 1) ? -cdd : cdd);
-                // the row arithmetic in the node arithmetic, as the per-node
-                // flux forms these terms
-                const A U = (A)fc[FC_P + 1], V = (A)fc[FC_P + 2], lam = (A)fc[FC_P + 3];
-                const A mt1 = (A)fc[FC_MT], mt2 = (A)fc[FC_MT + 1], mt3 = (A)fc[FC_MT + 2];
+                const A* k = fr[fa];
+                const A U = k[FR_U], V = k[FR_V], lam = k[FR_LAM];
+                const A mt1 = k[FR_MT1], mt2 = k[FR_MT2], mt3 = k[FR_MT3];
                 const A K = (A)c.K;
-                const A kfac = (A)(c.K / (2.0 * fc[FC_P + 3]));
-                const A mxi2 =
-                    (A)((c.K * c.K + 2.0 * c.K) / (4.0 * fc[FC_P + 3] * fc[FC_P + 3]));
-                const A csA =
-                    (A)(c.shk * (1.0 - c.prandtl) * fc[FC_P + 3] * fc[FC_P + 3] / fc[FC_P]);
-                const A q0 = (A)fc[FC_Q], q1 = (A)fc[FC_Q + 1];
+                const A kfac = k[FR_KFAC], mxi2 = k[FR_MXI2], csA = k[FR_CSA];
+                const A q0 = k[FR_Q0], q1 = k[FR_Q1];
                 // normal node of this row index and its upwind side (as the
                 // per-node flux decides it, in the node arithmetic)
                 const A vn = (A)(isx ? u1[idx] : v1[idx]);
-                const int d = upwind_delta(vn - fk[fa]);
-                const double* as = fc + (d ? FC_AL : FC_AR);
-                const A s0 = (A)as[0], s1 = (A)as[1], s2 = (A)as[2], s3 = (A)(0.5 * as[3]);
+                const int d = upwind_delta(vn - k[FR_VS]);
+                const A* as = k + (d ? FR_SL : FR_SR);
+                const A s0 = as[0], s1 = as[1], s2 = as[2], s3 = as[3];
                 const A s4 = s3 * mxi2;
-                const A pb0 = (A)fc[FC_AB], pb1 = (A)fc[FC_AB + 1], pb2 = (A)fc[FC_AB + 2];
-                const A pb3 = (A)(0.5 * fc[FC_AB + 3]), pb4 = pb3 * mxi2;
-                const A pt0 = (A)fc[FC_AT], pt1 = (A)fc[FC_AT + 1], pt2 = (A)fc[FC_AT + 2];
-                const A pt3 = (A)(0.5 * fc[FC_AT + 3]), pt4 = pt3 * mxi2;
-                const A tm2fn = (A)(fc[FC_TAU] * fc[FC_MT + 1] * ac[2 * slot]);
-                const A tm2ft = (A)(fc[FC_TAU] * fc[FC_MT + 1] * ac[2 * slot + 1]);
+                const A pb0 = k[FR_PB], pb1 = k[FR_PB + 1], pb2 = k[FR_PB + 2];
+                const A pb3 = k[FR_PB + 3], pb4 = pb3 * mxi2;
+                const A pt0 = k[FR_PT], pt1 = k[FR_PT + 1], pt2 = k[FR_PT + 2];
+                const A pt3 = k[FR_PT + 3], pt4 = pt3 * mxi2;
+                const A tm2fn = k[FR_TM2FN], tm2ft = k[FR_TM2FT];
                 const A cu = vn - U;
                 const A al = mt2 * vn - tm2fn, b0 = mt2 * V - tm2ft;
                 const A a5 = (A)2 * lam * cu * cu + K - (A)5, a3 = a5 + (A)2;
@@ -648,8 +690,7 @@ __device__ __forceinline__ void k_update2d_mx_body(
                     kfac * (mq * q1 * a3 + al * S1 + b0 * P1 + mt3 * T1) + mt2 * P0b,
                     kfac * (mq * (A)2 * lam * q0 * cu + al * s3 + b0 * pb3 + mt2 * P1 + mt3 * pt3),
                     kfac * (mq * (A)2 * lam * q1 + mt2 * pb3)};
-                const A rlp = (A)(fc[FC_P] * (fc[FC_P + 3] / c.pi));
-                const A en = sg * fl * vn * rlp * fexp(-lam * cu * cu);
+                const A en = k[FR_PRE] * vn * fexp(-lam * cu * cu);
                 // tangential node of this row index
                 const A cv = (A)(isx ? v1[idx] : -u1[idx]) - V;
                 const A ec = fexp(-lam * cv * cv);
