@@ -500,6 +500,14 @@ __device__ __forceinline__ void k_update2d_mx_body(
     };
     A fv[8][4], rh[4], rb[4];
     if constexpr (STAGE_G && MAXP == 1) load_faces(0, fv, rh, rb);
+    // FF with staged G (4-byte operands) and at most three tile positions
+    // per wavefront: every position's divergence is requested in the
+    // collision-rows phase, a whole contraction ahead of its use (4.44 ->
+    // 4.14 ms at 512^2 cells, GH40; with 8-byte operands, and before the
+    // face rows' factors moved to LDS, the registers it takes cost more
+    // occupancy than it hides)
+    constexpr bool PRE = FF && MAXP <= 3 && STAGE_G;
+    A dvp[PRE ? MAXP : 1][2][4];
     __syncthreads();
 
     // ---- Tm = Hx G: tile (f, jt), rows r, columns jv
@@ -722,7 +730,19 @@ __device__ __forceinline__ void k_update2d_mx_body(
     // the first tile position's face fluxes (and f^n) in flight during the
     // R contraction; with one position per wavefront and staged G, from
     // the start
-    if constexpr (!(STAGE_G && MAXP == 1)) load_faces(0, fv, rh, rb);
+    if constexpr (PRE) {
+#pragma unroll
+        for (int pi = 0; pi < MAXP; ++pi) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const uint32_t o = node_off(pi, v);
+                dvp[pi][0][v] = (A)buf_ld<T>(rsdh, o);
+                dvp[pi][1][v] = (A)buf_ld<T>(rsdb, o);
+            }
+        }
+    } else if constexpr (!(STAGE_G && MAXP == 1)) {
+        load_faces(0, fv, rh, rb);
+    }
     // ---- R(r, jv) = (ax sum_s D(r-1, s) Hy(s, jv) + ay sum_s D(r, s) Hy(s+1, jv)) OMy(jv)
     {
         const A axA = (A)P[CP_AX], ayA = (A)P[CP_AY];
@@ -772,7 +792,8 @@ __device__ __forceinline__ void k_update2d_mx_body(
     for (int pi = 0; pi < MAXP; ++pi) {
         const int p = wv + NW * pi;
         if (p >= npos) continue;
-        if (pi > 0) load_faces(pi, fv, rh, rb);
+        if constexpr (!PRE)
+            if (pi > 0) load_faces(pi, fv, rh, rb);
         const int it = p % nt, jt = p / nt;
         const int iu = 16 * it + lc, jvb = 16 * jt;
         V acc[2];
@@ -802,8 +823,12 @@ __device__ __forceinline__ void k_update2d_mx_body(
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const uint32_t o = node_off(pi, v);
-            const A dh = FF ? fv[0][v] : fv[0][v] - fv[1][v] + fv[2][v] - fv[3][v];
-            const A db = FF ? fv[4][v] : fv[4][v] - fv[5][v] + fv[6][v] - fv[7][v];
+            const A dh = PRE ? dvp[PRE ? pi : 0][0][v]
+                       : FF  ? fv[0][v]
+                             : fv[0][v] - fv[1][v] + fv[2][v] - fv[3][v];
+            const A db = PRE ? dvp[PRE ? pi : 0][1][v]
+                       : FF  ? fv[4][v]
+                             : fv[4][v] - fv[5][v] + fv[6][v] - fv[7][v];
             const A ho = STAGE_G ? rhs[STAGE_G ? pi : 0][v] : rh[v];
             const A bo = STAGE_G ? rbs[STAGE_G ? pi : 0][v] : rb[v];
             const double hn = fma(c1, (double)mx_opaque(ho),
