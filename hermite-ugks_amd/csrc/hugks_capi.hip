@@ -1161,8 +1161,12 @@ hipError_t time_schedule(hugks_handle* h, float* ms) {
 
 static int tune_case(const hugks_desc* d, bool grouped, int* flux_kernel, int* update_kernel,
                      int* waves, double* best_ms) {
+    // grouped: 1, or 2 with the grouped launches' face-forming update asked
+    // off (HUGKS_UPDATE_KERNEL=mfma-node)
+    const char* um = getenv("HUGKS_UPDATE_KERNEL");
+    const bool plain = um && strcmp(um, "mfma-node") == 0;
     const TuneKey key{d->dim, d->nx, d->ny, d->nv, d->nm, d->precision, d->bc[0], d->bc[2],
-                      d->force_variant, grouped ? 1 : 0};
+                      d->force_variant, grouped ? (plain ? 2 : 1) : 0};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
@@ -1180,12 +1184,18 @@ static int tune_case(const hugks_desc* d, bool grouped, int* flux_kernel, int* u
     TuneResult r{h->L.flux_mode, h->L.update_mode, h->dim == 2 ? h->L.update_nw : 0};
     double best = 1e300;
     if (d->dim == 2) {
-        const int f0 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_FUSED;
-        const int f1 = grouped ? HUGKS_FLUX_SPLIT : HUGKS_FLUX_CELLSUM;
-        const int u0 = grouped ? HUGKS_UPDATE_MFMA_NODE : HUGKS_UPDATE_LDS;
+        // a case of the grouped launches: the tile for the schedule its
+        // plan runs (the cell-sum flux and the face-forming update where the
+        // case supports them, as batch_group_plan picks them)
+        const bool gface = !plain && update2d_mxf_supported(h->L, h->L.update_nw);
+        const int gfk = h->L.cs_js >= 0 ? HUGKS_FLUX_CELLSUM : HUGKS_FLUX_SPLIT;
+        const int guk = gface ? HUGKS_UPDATE_MFMA_FACE : HUGKS_UPDATE_MFMA_NODE;
+        const int f0 = grouped ? gfk : HUGKS_FLUX_FUSED;
+        const int f1 = grouped ? gfk : HUGKS_FLUX_CELLSUM;
+        const int u0 = grouped ? guk : HUGKS_UPDATE_LDS;
+        const int u1 = grouped ? guk : HUGKS_UPDATE_MFMA_FACE;
         for (int fk = f0; fk <= f1; ++fk)
-            for (int uk = u0; uk <= (grouped ? HUGKS_UPDATE_MFMA_NODE : HUGKS_UPDATE_MFMA_FACE);
-                 ++uk) {
+            for (int uk = u0; uk <= u1; ++uk) {
                 if (hugks_set_schedule(h, fk, uk) != HUGKS_OK) continue;
                 const bool node = uk >= HUGKS_UPDATE_MFMA_NODE;
                 for (int nw = node ? 1 : 0; nw <= (node ? 16 : 0); ++nw) {
