@@ -273,16 +273,29 @@ __device__ __forceinline__ void k_cell_sum_body(
     const int su = upwind_delta((A)u);
     const double hlx = 0.5 * c.dx, hly = 0.5 * c.dy;
     const double sgnV = su ? hlx : -hlx;
-    double av[16], ah[18];
+    // av: the vertical faces' accumulators over the whole column; per pass
+    // (v < 0, then v >= 0) the column's row moments w v^k of h, b and the
+    // slopes, from which the pass's horizontal accumulators and its share of
+    // av follow (the face values h +- len/2 slope are linear in them):
+    // qh[k] = sum h c_k (k < 4), mx = sx (k < 4), my = sy (k < 5),
+    // mb = b (k < 2), mtx = tx (k < 2), mty = ty (k < 3)
+    double av[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) av[q] = 0.0;
-#pragma unroll
-    for (int q = 0; q < 18; ++q) ah[q] = 0.0;
     typedef T Vec __attribute__((ext_vector_type(VW)));
 #pragma unroll 1
     for (int p = 0; p < 2; ++p) {
         const int lo = p ? js : 0, hi = p ? nv : js;
         const double sgnH = p ? hly : -hly;
+        double qh[4], qx[4], qy[5], qb[2], qtx[2], qty[3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qh[k] = qx[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) qy[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) qb[k] = qtx[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) qty[k] = 0.0;
 #pragma unroll 1
         for (int j0 = lo; j0 < hi; j0 += CR) {
             const int cr = hi - j0 < CR ? hi - j0 : CR;
@@ -365,54 +378,52 @@ __device__ __forceinline__ void k_cell_sum_body(
                 const double hu = (double)my[e], bu = (double)my[cw + e];
                 const double sx = (double)my[2 * cw + e], sy = (double)my[3 * cw + e];
                 const double tx = (double)my[4 * cw + e], ty = (double)my[5 * cw + e];
-                {
-                    // vertical faces: sn = sx, st = sy (and the b slopes)
-                    const double a = hu + sgnV * sx, g = bu + sgnV * tx;
-                    av[0] += a * c0;
-                    av[1] += a * c1;
-                    av[2] += a * c2;
-                    av[3] += a * c3;
-                    av[4] += sx * c0;
-                    av[5] += sx * c1;
-                    av[6] += sx * c2;
-                    av[7] += sy * c0;
-                    av[8] += sy * c1;
-                    av[9] += sy * c2;
-                    av[10] += sy * c3;
-                    av[11] += g * c0;
-                    av[12] += g * c1;
-                    av[13] += tx * c0;
-                    av[14] += ty * c0;
-                    av[15] += ty * c1;
-                }
-                {
-                    // horizontal faces: sn = sy, st = sx
-                    const double a = hu + sgnH * sy, g = bu + sgnH * ty;
-                    ah[0] += a * c0;
-                    ah[1] += a * c1;
-                    ah[2] += a * c2;
-                    ah[3] += a * c3;
-                    ah[4] += sy * c1;
-                    ah[5] += sy * c2;
-                    ah[6] += sy * c3;
-                    ah[7] += sy * c4;
-                    ah[8] += sx * c0;
-                    ah[9] += sx * c1;
-                    ah[10] += sx * c2;
-                    ah[11] += sx * c3;
-                    ah[12] += g * c0;
-                    ah[13] += g * c1;
-                    ah[14] += ty * c1;
-                    ah[15] += ty * c2;
-                    ah[16] += tx * c0;
-                    ah[17] += tx * c1;
-                }
+                qh[0] += hu * c0;
+                qh[1] += hu * c1;
+                qh[2] += hu * c2;
+                qh[3] += hu * c3;
+                qx[0] += sx * c0;
+                qx[1] += sx * c1;
+                qx[2] += sx * c2;
+                qx[3] += sx * c3;
+                qy[0] += sy * c0;
+                qy[1] += sy * c1;
+                qy[2] += sy * c2;
+                qy[3] += sy * c3;
+                qy[4] += sy * c4;
+                qb[0] += bu * c0;
+                qb[1] += bu * c1;
+                qtx[0] += tx * c0;
+                qtx[1] += tx * c1;
+                qty[0] += ty * c0;
+                qty[1] += ty * c1;
+                qty[2] += ty * c2;
             }
         }
-        // the horizontal bin of this pass's rows
-        cs_reduce_bin<false>(ah, u, wu, true, csum + csum_at(2 + p, 0, cell), 1);
+        // the horizontal bin of this pass's rows, from the horizontal
+        // faces' accumulators (a = h + sgnH sy, g = b + sgnH ty)
+        {
+            const double ah[18] = {qh[0] + sgnH * qy[0], qh[1] + sgnH * qy[1],
+                                   qh[2] + sgnH * qy[2], qh[3] + sgnH * qy[3],
+                                   qy[1], qy[2], qy[3], qy[4],
+                                   qx[0], qx[1], qx[2], qx[3],
+                                   qb[0] + sgnH * qty[0], qb[1] + sgnH * qty[1],
+                                   qty[1], qty[2], qtx[0], qtx[1]};
+            cs_reduce_bin<false>(ah, u, wu, true, csum + csum_at(2 + p, 0, cell), 1);
+        }
+        // this pass's share of the vertical accumulators (a = h + sgnV sx,
+        // g = b + sgnV tx)
 #pragma unroll
-        for (int t = 0; t < 18; ++t) ah[t] = 0.0;
+        for (int k = 0; k < 4; ++k) av[k] += qh[k] + sgnV * qx[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) av[4 + k] += qx[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) av[7 + k] += qy[k];
+        av[11] += qb[0] + sgnV * qtx[0];
+        av[12] += qb[1] + sgnV * qtx[1];
+        av[13] += qtx[0];
+        av[14] += qty[0];
+        av[15] += qty[1];
     }
 #pragma unroll 1
     for (int side = 0; side < 2; ++side)
@@ -420,7 +431,7 @@ __device__ __forceinline__ void k_cell_sum_body(
 }
 #ifndef HUGKS_BODIES_ONLY
 template <typename T, int VW>
-__global__ __launch_bounds__(WAVE * SLOPE_CPW) void k_cell_sum(
+__global__ __launch_bounds__(WAVE * SLOPE_CPW) HUGKS_WAVES_PER_EU(3) void k_cell_sum(
     Consts c, const DevState* __restrict__ st, const T* __restrict__ h, const T* __restrict__ b,
     T* __restrict__ shx, T* __restrict__ shy, T* __restrict__ sbx, T* __restrict__ sby,
     const double* __restrict__ u1, const double* __restrict__ v1,

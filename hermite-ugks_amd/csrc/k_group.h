@@ -76,7 +76,7 @@ __global__ __launch_bounds__(WAVE * SPLIT_FPW) void kg_fsum(const int* __restric
 // the cell-sum pipeline (every case of the plan supports it): slopes and
 // per-cell sums, then each face's sums from two bins
 template <typename T, int VW>
-__global__ __launch_bounds__(WAVE * SLOPE_CPW) void kg_cell_sum(const int* __restrict__ tab,
+__global__ __launch_bounds__(WAVE * SLOPE_CPW) HUGKS_WAVES_PER_EU(3) void kg_cell_sum(const int* __restrict__ tab,
                                                                 const Launch2D* __restrict__ Ls) {
     __shared__ __align__(16) unsigned char lds[SLOPE_CPW * cs_lds_per_wave<T>()];
     const GroupIdx g = group_index(tab);
