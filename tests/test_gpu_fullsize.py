@@ -129,3 +129,36 @@ def test_arrays_over_4gib_match_narrow_channel(have_gpu, prec):
     for i in (0, 1, 5000, 10999):
         assert np.array_equal(ww[:, i, :], wn[:, 0, :]), i
     assert np.array_equal(ww, np.repeat(ww[:, :1, :], 11000, axis=1))
+
+
+@pytest.mark.parametrize("prec", [capi.MIXED_FP32, capi.FP64])
+def test_face_update_arrays_near_4gib_match_narrow_channel(have_gpu, prec):
+    """The face-forming update and its per-cell divergence address whole
+    distribution arrays with 32-bit byte offsets, which the library allows up
+    to 4 GiB - 16 bytes.  A channel uniform along x with each array just
+    under that (10485 x 64 cells at GH40 in FP32 storage, 5242 x 64 in FP64)
+    runs that schedule and gives in every column exactly what the channel 4
+    cells wide gives."""
+    p = dict(nx=4, ny=64, xlength=0.1, ylength=1.0, nv=40, vmax=5.0, Tref=1.0,
+             quad_type="gauss_hermite", force_type="hermite_symmetric", Nm=8, cfl=0.8,
+             maxtime=200.0, case_id=5, knudsen=0.1, phi=0.05, TwL=1.0, TwR=1.2)
+    steps = 3
+    nx = 10485 if prec == capi.MIXED_FP32 else 5242
+    es = 4 if prec == capi.MIXED_FP32 else 8
+    sched = (capi.FLUX_CELLSUM, capi.UPDATE_MFMA_FACE)
+    wide = cases.build_2d(dict(p, nx=nx, xlength=0.1 * nx / 4), prec)
+    nbytes = wide.ncell * 1600 * es
+    assert 2 ** 32 - 2 ** 26 < nbytes < 2 ** 32 - 16
+    narrow = cases.build_2d(p, prec)
+    _, _, wn = gpu_run(narrow, steps, sched)
+    print(f"run {nx}x64 nv=40 prec={prec} steps={steps} sched={sched} ({nbytes} B per array)",
+          flush=True)
+    with capi.Solver(wide) as s:
+        assert s.schedule() == sched
+        s.advance(steps)
+        ww = s.macro(with_q=False)
+    wn = wn.reshape(64, 4, -1)
+    ww = ww.reshape(64, nx, -1)
+    for i in (0, 1, nx // 2, nx - 1):
+        assert np.array_equal(ww[:, i, :], wn[:, 0, :]), i
+    assert np.array_equal(ww, np.repeat(ww[:, :1, :], nx, axis=1))
