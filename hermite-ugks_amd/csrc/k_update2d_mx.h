@@ -39,6 +39,12 @@
 // b^{n+1} are written from the accumulators.  One workgroup of NW
 // wavefronts per cell; wavefront w owns the tile positions w, w + NW, ...
 // (MAXP at most).
+//
+// The face-forming instances (FF; k_update2d_mxf) take the four faces'
+// equilibrium node fluxes as 16 more K rows of the F^T tiles, built after
+// the D contraction from each face's coefficients, and read the rest of
+// the face divergence per node from the per-cell arrays of k_fdiv (two
+// values instead of eight face fluxes).
 #pragma once
 #include "k_update2d_mfma.h"
 
