@@ -218,10 +218,11 @@ __host__ __device__ constexpr int cs_chunk_rows(int nv, int es) {
          : CS_CHUNK_BYTES / (nv * es);
 }
 // LDS of one wavefront: the chunk of the six arrays, then each chunk row's
-// v and weight factor (FP64)
+// weight factor times v^k, k = 0..4 (FP64, six to a row for 16-byte reads)
+constexpr int CS_ROWC = 6;
 template <typename T>
 constexpr int cs_lds_per_wave() {
-    return 6 * CS_CHUNK_BYTES + 2 * CS_MAX_ROWS * 8;
+    return 6 * CS_CHUNK_BYTES + CS_ROWC * CS_MAX_ROWS * 8;
 }
 
 // The slopes and the four half-space bins of one cell.  The rows are
@@ -261,7 +262,9 @@ __device__ __forceinline__ void k_cell_sum_body(
     }
     const int nv = c.nv;
     const int CR = cs_chunk_rows(nv, (int)sizeof(T));
-    const int cw = CR * nv;                  // LDS elements per array
+    // LDS elements per array: the fixed chunk size, so that the row loop
+    // reads the six arrays from one address with immediate offsets
+    constexpr int cw = CS_CHUNK_BYTES / (int)sizeof(T);
     unsigned char* myb = lds + (size_t)(threadIdx.x / WAVE) * cs_lds_per_wave<T>();
     T* my = reinterpret_cast<T*>(myb);
     double* rowc = reinterpret_cast<double*>(myb + 6 * CS_CHUNK_BYTES);
@@ -303,6 +306,7 @@ __device__ __forceinline__ void k_cell_sum_body(
             const int base = j0 * nv;
             // every load of the chunk first, then the LDS stores
             constexpr int NL = VW > 1 ? 1 : (CS_CHUNK_BYTES / (int)sizeof(T) + WAVE - 1) / WAVE;
+            double vr = 0.0, wr = 0.0;
 #pragma unroll
             for (int t = 0; t < NL; ++t) {
                 const int e0 = (lane + WAVE * t) * VW;
@@ -350,12 +354,9 @@ __device__ __forceinline__ void k_cell_sum_body(
                 if (t == 0) {
                     // the chunk rows' v and weight factor, with the loads
                     const int jr = lane < cr ? j0 + lane : j0;
-                    const double vr = v1[jr], wr = wy[jr];
+                    vr = v1[jr];
+                    wr = wy[jr];
                     wave_lds_sync();     // the previous chunk's reads are done
-                    if (lane < cr) {
-                        rowc[2 * lane] = vr;
-                        rowc[2 * lane + 1] = wr;
-                    }
                 }
                 if (in) {
                     *reinterpret_cast<Vec*>(my + e0) = x0;
@@ -366,15 +367,25 @@ __device__ __forceinline__ void k_cell_sum_body(
                     *reinterpret_cast<Vec*>(my + 5 * cw + e0) = x5;
                 }
             }
+            // the rows' weight factor times powers of their v, formed once
+            // per row here rather than by every lane of the row loop
+            if (lane < cr) {
+                double* rc = rowc + CS_ROWC * lane;
+                const double c1 = wr * vr, c2 = c1 * vr, c3 = c2 * vr;
+                rc[0] = wr;
+                rc[1] = c1;
+                rc[2] = c2;
+                rc[3] = c3;
+                rc[4] = c3 * vr;
+            }
             wave_lds_sync();
+            const int jn = act ? cr : 0;
 #pragma unroll 1
-            for (int rr = 0; rr < cr; rr += R) {
-                const int jj = rr + r;
-                if (!act || jj >= cr) continue;
+            for (int jj = r; jj < jn; jj += R) {
                 const int e = jj * nv + iu;
                 // the row's weight factor times powers of its v
-                const double v = rowc[2 * jj], c0 = rowc[2 * jj + 1];
-                const double c1 = c0 * v, c2 = c1 * v, c3 = c2 * v, c4 = c3 * v;
+                const double* rc = rowc + CS_ROWC * jj;
+                const double c0 = rc[0], c1 = rc[1], c2 = rc[2], c3 = rc[3], c4 = rc[4];
                 const double hu = (double)my[e], bu = (double)my[cw + e];
                 const double sx = (double)my[2 * cw + e], sy = (double)my[3 * cw + e];
                 const double tx = (double)my[4 * cw + e], ty = (double)my[5 * cw + e];
