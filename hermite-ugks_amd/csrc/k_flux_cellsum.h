@@ -208,6 +208,26 @@ __device__ __forceinline__ void cs_reduce_bin(const double (&x0)[N], double u, d
     }
 }
 
+// The van Leer slope of vl_slope.  FP32 takes the limiter on the raw
+// differences, (sgn dR + sgn dL) |dR| |dL| / (|dR| + |dL| + smv len) / len,
+// the same quantity with the 1/len factor applied once instead of to both
+// differences (one VALU fewer per slope; the rounding differs), and the
+// quotient from the hardware reciprocal.  FP64 is vl_slope's own arithmetic.
+template <typename A>
+__device__ __forceinline__ A cs_slope(A fl, A fn, A fr, bool inner, A inv_den, A smv, A smv_len) {
+    if constexpr (sizeof(A) == 4) {
+        if (inner) {
+            const A dL = fn - fl, dR = fr - fn;
+            const A num = (copysign((A)1, dR) + copysign((A)1, dL)) * fabs(dR) * fabs(dL);
+            const A den = fabs(dR) + fabs(dL) + smv_len;
+            return fast_div(num, den) * inv_den;
+        }
+        return (fr - fl) * inv_den;
+    } else {
+        return vl_slope<A, true>(fl, fn, fr, inner, inv_den, smv);
+    }
+}
+
 // Bytes of one LDS chunk per array and wavefront: whole rows, at most
 // what one 16-byte load per lane covers.
 constexpr int CS_CHUNK_BYTES = WAVE * 16;
@@ -249,7 +269,7 @@ __device__ __forceinline__ void k_cell_sum_body(
     // the slope stencils, as k_slope2d takes them
     const A smv = (A)c.smv;
     bool inner[2];
-    A inv_den[2];
+    A inv_den[2], smv_len[2];
     size_t on[2][2];
 #pragma unroll
     for (int dir = 0; dir < 2; ++dir) {
@@ -257,6 +277,7 @@ __device__ __forceinline__ void k_cell_sum_body(
         const A len = (A)(dir == 0 ? c.dx : c.dy);
         inner[dir] = nb.inner;
         inv_den[dir] = (A)1 / ((A)0.5 * len + (A)0.5 * len);
+        smv_len[dir] = smv / inv_den[dir];
         on[dir][0] = (size_t)nb.cl * c.ldv;
         on[dir][1] = (size_t)nb.cr * c.ldv;
     }
@@ -334,14 +355,14 @@ __device__ __forceinline__ void k_cell_sum_body(
                     const Vec byr = *reinterpret_cast<const Vec*>(b + on[1][1] + gr);
 #pragma unroll
                     for (int q = 0; q < VW; ++q) {
-                        x2[q] = (T)vl_slope<A, true>((A)hxl[q], (A)x0[q], (A)hxr[q], inner[0],
-                                                     inv_den[0], smv);
-                        x3[q] = (T)vl_slope<A, true>((A)hyl[q], (A)x0[q], (A)hyr[q], inner[1],
-                                                     inv_den[1], smv);
-                        x4[q] = (T)vl_slope<A, true>((A)bxl[q], (A)x1[q], (A)bxr[q], inner[0],
-                                                     inv_den[0], smv);
-                        x5[q] = (T)vl_slope<A, true>((A)byl[q], (A)x1[q], (A)byr[q], inner[1],
-                                                     inv_den[1], smv);
+                        x2[q] = (T)cs_slope<A>((A)hxl[q], (A)x0[q], (A)hxr[q], inner[0],
+                                               inv_den[0], smv, smv_len[0]);
+                        x3[q] = (T)cs_slope<A>((A)hyl[q], (A)x0[q], (A)hyr[q], inner[1],
+                                               inv_den[1], smv, smv_len[1]);
+                        x4[q] = (T)cs_slope<A>((A)bxl[q], (A)x1[q], (A)bxr[q], inner[0],
+                                               inv_den[0], smv, smv_len[0]);
+                        x5[q] = (T)cs_slope<A>((A)byl[q], (A)x1[q], (A)byr[q], inner[1],
+                                               inv_den[1], smv, smv_len[1]);
                     }
                     if (in) {
                         // streamed out: read back only by the face kernels
